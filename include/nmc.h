@@ -1,0 +1,217 @@
+/*
+ * nmc.h - C ABI of the MI355X photon-transport library (libnmc_hip.so).
+ *
+ * The library replaces the Metal host runtime of NMC (`MetalMC`,
+ * NMC/NMCCode/MetalMC.h:25-31) and the kernels it dispatches
+ * (NMC/NMCCode/kernels.metal:59-754).  Signatures use plain C types and
+ * pointers only; every entry point returns an `int` status (NMC_OK == 0) and
+ * records a message readable through nmc_last_error().  Nothing in the library
+ * calls exit().
+ *
+ * Geometry/material inputs are the reference's own structs, byte-compatible
+ * with NMC/NMCCode/definitions.h:186-218 (checked by static asserts below), so
+ * a caller that already fills `RunParams`/`TissueParams` for NMC can pass them
+ * unchanged.  Everything NMC fixed at compile time (PHOTON_TOTAL, BINS, dr/dz,
+ * THRESHOLD, CHANCE, ...; definitions.h:16-82) travels in `nmc_ext_config`.
+ */
+#ifndef NMC_H
+#define NMC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- constants shared with the reference (definitions.h:57-58,66-67) ---- */
+#define NMC_NTISS 19          /* tissue table length, index 0 unused       */
+#define NMC_STRLEN 32
+#define NMC_THRESHOLD 0.0001f /* roulette threshold                         */
+#define NMC_CHANCE 0.1f       /* roulette survival chance                   */
+
+/* ---- status codes ---- */
+#define NMC_OK 0
+#define NMC_ERR_ARG -1        /* invalid argument / configuration           */
+#define NMC_ERR_STATE -2      /* call out of order (e.g. run before prepare) */
+#define NMC_ERR_DEVICE -3     /* HIP runtime error                          */
+#define NMC_ERR_NOMEM -4      /* allocation failed / larger than device     */
+#define NMC_ERR_UNSUPPORTED -5
+
+/* ---- simulation kinds (definitions.h:151-164, MC_SIM_TYPE) ---- */
+#define NMC_SIM_MCSUB 0       /* semi-infinite medium, cylindrical tallies  */
+#define NMC_SIM_MCXYZ 1       /* voxelised 3-D medium                       */
+#define NMC_SIM_MCXYZ_TT 2    /* mcxyz with the two-term phase function     */
+
+/* ---- detection side (definitions.h:167-171) ---- */
+#define NMC_DET_REFLECTANCE 0
+#define NMC_DET_TRANSMITTANCE 1
+
+/*
+ * RunParams, byte-compatible with definitions.h:186-209 (172 bytes).
+ * The three enums of the reference are stored as int32.
+ */
+typedef struct RunParams {
+    char myname[NMC_STRLEN];
+    int32_t mcflag, launchflag, boundaryflag, speckleflag, semiflag;
+    float xfocus, yfocus, zfocus;
+    float ux0, uy0, uz0;
+    float radius, det_radius, xd;
+    float waist;
+    float zsurf;
+    float n1, n2;
+    float na, lambda, lc;
+    int32_t det_state;
+    int32_t ph_quant;
+    float dx, dy, dz;
+    int32_t Nx, Ny, Nz, Nt;
+    float xs, ys, zs;
+    float time_min;
+    int32_t mckernelflag;
+} RunParams;
+
+/* TissueParams, byte-compatible with definitions.h:211-218 (608 bytes). */
+typedef struct TissueParams {
+    float muav[NMC_NTISS];
+    float musv[NMC_NTISS];
+    float gv[NMC_NTISS];
+    float gf[NMC_NTISS], af[NMC_NTISS], gb[NMC_NTISS], ab[NMC_NTISS], CC[NMC_NTISS];
+} TissueParams;
+
+#define NMC_EXT_VERSION 1
+
+/* flags in nmc_ext_config.flags */
+#define NMC_FLAG_NO_FLUENCE 1u   /* skip the 3-D / (r,z) absorption tally   */
+
+/*
+ * Run-time settings that NMC fixed at compile time (definitions.h:16-82) plus
+ * the Philox stream selection.  No counterpart struct exists in the reference.
+ */
+typedef struct nmc_ext_config {
+    uint32_t struct_size;   /* sizeof(nmc_ext_config)                       */
+    uint32_t version;       /* NMC_EXT_VERSION                              */
+    uint64_t seed;          /* Philox key = (seed lo, seed hi + stream_id)  */
+    uint32_t stream_id;
+    uint32_t flags;         /* NMC_FLAG_*                                   */
+    float threshold;        /* roulette threshold (default NMC_THRESHOLD)   */
+    float chance;           /* roulette survival  (default NMC_CHANCE)      */
+    uint32_t max_events;    /* per-photon substep+interaction watchdog      */
+    /* mcsub cylindrical binning (replaces BINS, dr_mcsub, dz_mcsub)        */
+    int32_t nr, nz, na;     /* real bins; one overflow bin is appended to r and z */
+    float dr, dz;
+    int32_t reserved[8];
+} nmc_ext_config;
+
+/*
+ * Scalar totals of one tally set, all weights in photon-weight units (not yet
+ * divided by the photon count).  The integer fields are exact event counts.
+ */
+typedef struct nmc_totals {
+    double specular;          /* weight reflected at launch (Rsp)            */
+    double rd;                /* weight escaping through the detector window */
+    double rd_outside;        /* weight escaping the surface outside it      */
+    double absorbed;          /* weight deposited inside the grid / medium   */
+    double absorbed_outside;  /* weight deposited outside the grid           */
+    double tt;                /* weight leaving through the bottom (z >= Nz*dz) */
+    double lateral;           /* weight leaving through the x/y sides        */
+    double top;               /* weight leaving through the grid top (z < 0) */
+    double roulette_gain;     /* weight created by roulette survivals        */
+    double roulette_loss;     /* weight removed by roulette kills            */
+    double capped;            /* weight of photons stopped by max_events     */
+    uint64_t photons;
+    uint64_t detections;      /* escapes through the detector window         */
+    uint64_t surface_hits;    /* arrivals at the zsurf plane / mcsub surface */
+    uint64_t escapes_bottom, escapes_side, escapes_top;
+    uint64_t roulette_kills, roulette_survivals, capped_photons;
+    uint64_t substeps_same;   /* hop pieces that end in an interaction       */
+    uint64_t substeps_cross;  /* hop pieces that end on a voxel face / plane */
+    uint64_t interactions;    /* scattering events (spin + roulette)         */
+} nmc_totals;
+
+/*
+ * Host-side result pointers; any may be NULL to skip that array.  Sizes:
+ *  mcxyz: fluence Nz*Nx*Ny (index iz*Ny*Nx + ix*Ny + iy, kernels.metal:482),
+ *         ryx and det2d Nx*Ny (index ix*Ny + iy).
+ *  mcsub: rd_ra (nr+1)*na   (index ir*na + ia, ir == nr is the overflow bin),
+ *         a_rz  (nr+1)*(nz+1) (index ir*(nz+1) + iz; F[ir][iz] of kernels.metal:233).
+ * Values are raw weights in double precision, accumulated over every
+ * nmc_run since the last nmc_reset_tallies.
+ */
+typedef struct nmc_results {
+    double* fluence;
+    double* ryx;
+    uint32_t* det2d;
+    double* rd_ra;
+    double* a_rz;
+    nmc_totals totals;
+} nmc_results;
+
+typedef struct nmc_handle nmc_handle;
+
+/* Defaults for every field of nmc_ext_config. */
+void nmc_ext_config_init(nmc_ext_config* cfg);
+
+/* Checks a configuration without touching a device (the checks nmc_prepare
+ * runs first): NMC_OK or an error code with nmc_last_error() set.  The
+ * reference performs no validation (mcio.h:16-189 reads unchecked). */
+int nmc_validate(int sim_type, const RunParams* rp, const TissueParams* tp, const uint8_t* v,
+                 const nmc_ext_config* ext);
+
+/* Replaces -[MetalMC initWithDevice::] (MetalMC.h:25, MetalMC.m:96-177):
+ * selects the device and the simulation kind. */
+int nmc_create(int sim_type, int device, nmc_handle** out);
+
+/* Replaces -[MetalMC prepareData:::] (MetalMC.h:26, MetalMC.m:179-383):
+ * validates and copies parameters and the voxel map (NULL for mcsub) to the
+ * device and allocates the device tallies (zeroed).  `ext` may be NULL. */
+int nmc_prepare(nmc_handle* h, const RunParams* rp, const TissueParams* tp,
+                const uint8_t* v, const nmc_ext_config* ext);
+
+/* Replaces -[MetalMC setNumRunsSeed::] (MetalMC.h:30, MetalMC.m:498-505):
+ * the 64-bit Philox seed and the id of the next photon to launch. */
+int nmc_set_seed(nmc_handle* h, uint64_t seed, uint64_t first_photon);
+
+/* Replaces -[MetalMC sendComputeCommand] (MetalMC.h:27, MetalMC.m:385-407):
+ * transports photons [first_photon, first_photon + n) and blocks until done;
+ * afterwards first_photon advances by n. */
+int nmc_run(nmc_handle* h, uint64_t n_photons);
+
+/* Same as nmc_run but returns after enqueueing on the handle's stream. */
+int nmc_run_async(nmc_handle* h, uint64_t n_photons);
+int nmc_synchronize(nmc_handle* h);
+
+/* Milliseconds between HIP events recorded around the transport kernel(s) of
+ * the last nmc_run / nmc_run_async, on the stream they were launched on. */
+int nmc_last_kernel_ms(nmc_handle* h, float* ms);
+
+/* Replaces -[MetalMC getComputeResults:...] (MetalMC.h:28, MetalMC.m:508-824):
+ * copies the accumulated tallies into the caller's arrays. */
+int nmc_get_results(nmc_handle* h, nmc_results* out);
+
+/* Zeroes the device tallies (the reference re-creates buffers instead). */
+int nmc_reset_tallies(nmc_handle* h);
+
+void nmc_destroy(nmc_handle* h);
+
+/* Message of the last failing call on this host thread ("" if none). */
+const char* nmc_last_error(void);
+
+/* Library build string (offload target, compile flags that affect results). */
+const char* nmc_version(void);
+
+#ifdef __cplusplus
+}  /* extern "C" */
+#endif
+
+#ifdef __cplusplus
+static_assert(sizeof(RunParams) == 172, "RunParams must match definitions.h:186-209");
+static_assert(sizeof(TissueParams) == 608, "TissueParams must match definitions.h:211-218");
+static_assert(offsetof(RunParams, n1) == 96 && offsetof(RunParams, det_state) == 116 &&
+                  offsetof(RunParams, Nx) == 136 && offsetof(RunParams, mckernelflag) == 168,
+              "RunParams field offsets");
+#else
+_Static_assert(sizeof(RunParams) == 172, "RunParams must match definitions.h:186-209");
+_Static_assert(sizeof(TissueParams) == 608, "TissueParams must match definitions.h:211-218");
+#endif
+
+#endif /* NMC_H */

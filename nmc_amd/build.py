@@ -1,0 +1,45 @@
+"""Compiles the HIP library nmc_amd/libnmc_hip.so for gfx950 with hipcc.
+
+Flags that affect results:
+  -ffp-contract=off   no implicit fma: float expressions round exactly as
+                      written, on the host and the device alike;
+  -Xclang -target-feature -Xclang -packed-fp32-ops
+                      keep the compiler from emitting packed FP32 ops;
+  -munsafe-fp-atomics hardware FP32/FP64 atomic add for the tallies.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libnmc_hip.so")
+SOURCES = ["nmc_abi.hip", "mcxyz.hip", "mcsub.hip"]
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("NMC_OFFLOAD_ARCH", "gfx950")
+
+FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "-ffp-contract=off", "-munsafe-fp-atomics",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-command-line-argument",
+]
+
+
+def build(verbose: bool = False) -> str:
+    """Builds the library in-tree (always recompiles) and returns its path."""
+    cmd = [HIPCC, *FLAGS, "-o", LIB + ".tmp", *[os.path.join(CSRC, s) for s in SOURCES]]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("hipcc failed building libnmc_hip.so")
+    os.replace(LIB + ".tmp", LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

@@ -1,0 +1,375 @@
+// mcxyz.hip - voxelised 3-D photon transport (reference MCXYZKernel,
+// NMC/NMCCode/kernels.metal:321-754, and the two-term variant's sampling,
+// kernels.metal:1053-1070).
+//
+// Launch model: persistent wave64 workgroups.  Each lane carries one photon;
+// the loop body advances every live lane by one piece of its current step
+// (to the next interaction, voxel face or detector plane) and, for lanes
+// whose step is used up, performs the interaction (spin + roulette + next
+// step length).  Lanes whose photon ended are refilled at the top of the
+// loop from a wave-private pool of photon ids that is replenished, `chunk`
+// ids at a time, from one global counter.  A photon's random numbers depend
+// only on its id, never on the lane or the order of execution.
+//
+// Tallies: fluence is summed per voxel visit in a register (double) and added
+// with one FP64 global atomic when the photon leaves the voxel; rare events
+// (detections, escapes, roulette) go to LDS totals; frequent counters stay in
+// registers until the workgroup ends.
+#include "nmc_internal.h"
+
+namespace nmc {
+
+namespace {
+
+struct Photon {
+  float x, y, z, ux, uy, uz, rux, ruy, ruz, W, sleft, mua, mus;
+  int ix, iy, iz, tis;
+  bool inside, alive;
+  uint64_t pid;
+  uint32_t ev, events, b1, b2, b3;
+  double vdep;
+};
+
+__device__ __forceinline__ int clampi(int v, int n) { return v < 0 ? 0 : (v >= n ? n - 1 : v); }
+
+__device__ __forceinline__ int64_t vidx(const XyzArgs& a, int ix, int iy, int iz) {
+  return (int64_t)iz * a.Ny * a.Nx + (int64_t)ix * a.Ny + iy;
+}
+
+__device__ __forceinline__ void lds_addf(double* p, double v) { unsafeAtomicAdd(p, v); }
+__device__ __forceinline__ void lds_addi(nmc_totals* t, int k, unsigned long long v) {
+  atomicAdd(&((unsigned long long*)&t->photons)[k], v);
+}
+__device__ __forceinline__ void lds_addf(nmc_totals* t, int k, double v) {
+  unsafeAtomicAdd(&((double*)t)[k], v);
+}
+
+// Escape through a grid face under boundaryflag 1 (or the top under 2).
+__device__ __forceinline__ void tally_escape(nmc_totals* st, const XyzArgs& a, int iz, float W) {
+  if (iz >= a.Nz) {
+    lds_addf(st, TF_TT, (double)W);
+    lds_addi(st, TI_ESC_BOTTOM, 1);
+  } else if (iz < 0) {
+    lds_addf(st, TF_TOP, (double)W);
+    lds_addi(st, TI_ESC_TOP, 1);
+  } else {
+    lds_addf(st, TF_LATERAL, (double)W);
+    lds_addi(st, TI_ESC_SIDE, 1);
+  }
+}
+
+// LAUNCH (kernels.metal:366-486).  Leaves p.alive false if the photon starts
+// outside a grid whose faces absorb it.
+__device__ __forceinline__ void launch(Photon& p, const XyzArgs& a, const XyzTissue* T,
+                                      nmc_totals* st) {
+  float x, y, z, ux, uy, uz, W = 1.0f;
+  if (a.launchflag == 1) {
+    x = a.xs; y = a.ys; z = a.zs;
+    ux = a.ux0; uy = a.uy0; uz = a.uz0;
+  } else {
+    U4 b = philox(p.pid, 0u, 0u, a.key0, a.key1);
+    if (a.mcflag == 0 || a.mcflag == 1) {
+      float r;
+      if (a.mcflag == 0) r = a.radius * sqrtf(u01nz(b.x));
+      else r = a.radius * sqrtf(-log_pos(u01nz(b.x)));
+      float sp, cp;
+      sincos2pi(u01nz(b.y), &sp, &cp);
+      x = a.xs + r * cp;
+      y = a.ys + r * sp;
+      z = a.zs;
+      float rf = a.waist * sqrtf(u01nz(b.z));
+      sincos2pi(u01nz(b.w), &sp, &cp);
+      float xf = rf * cp, yf = rf * sp;
+      float ddx = x - xf, ddy = y - yf;
+      float tmp = sqrtf(ddx * ddx + ddy * ddy + a.zfocus * a.zfocus);
+      ux = -ddx / tmp;
+      uy = -ddy / tmp;
+      uz = sqrtf(1.0f - ux * ux - uy * uy);
+      if (a.mcflag == 1) {
+        float ca2;
+        float rsp = fresnel(a.n2, a.n1, fabsf(uz), &ca2);
+        lds_addf(st, TF_SPECULAR, (double)(W * rsp));
+        W = W * (1.0f - rsp);
+        snell(a.n2, a.n1, &ux, &uy, &uz);
+      }
+    } else if (a.mcflag == 2) {
+      float ct = 1.0f - 2.0f * u01(b.x);
+      float stt = sqrtf(1.0f - ct * ct);
+      float sp, cp;
+      sincos2pi(u01(b.y), &sp, &cp);
+      x = a.xs; y = a.ys; z = a.zs;
+      ux = stt * cp; uy = stt * sp; uz = ct;
+    } else if (a.mcflag == 3) {
+      x = a.radius * (u01nz(b.x) * 2.0f - 1.0f);
+      y = a.radius * (u01nz(b.y) * 2.0f - 1.0f);
+      z = a.zs;
+      ux = 0.0f; uy = 0.0f; uz = 1.0f;
+    } else {
+      x = 0.0f; y = 0.0f; z = a.zs;
+      ux = 0.0f; uy = 0.0f; uz = 1.0f;
+    }
+  }
+  p.x = x; p.y = y; p.z = z; p.ux = ux; p.uy = uy; p.uz = uz; p.W = W;
+  p.ix = (int)floorf(a.hx + x / a.dx);
+  p.iy = (int)floorf(a.hy + y / a.dy);
+  p.iz = (int)floorf(z / a.dz);
+  p.inside = p.ix >= 0 && p.ix < a.Nx && p.iy >= 0 && p.iy < a.Ny && p.iz >= 0 && p.iz < a.Nz;
+  p.vdep = 0.0;
+  if (!p.inside && (a.bmode == 1 || (a.bmode == 2 && p.iz < 0))) {
+    tally_escape(st, a, p.iz, W);
+    p.alive = false;
+    return;
+  }
+  p.tis = a.V[vidx(a, clampi(p.ix, a.Nx), clampi(p.iy, a.Ny), clampi(p.iz, a.Nz))];
+  p.mua = T[p.tis].mua;
+  p.mus = T[p.tis].mus;
+  p.rux = 1.0f / ux;
+  p.ruy = 1.0f / uy;
+  p.ruz = 1.0f / uz;
+  p.ev = 1;
+  p.events = 0;
+  U4 b = philox(p.pid, 1u, 0u, a.key0, a.key1);
+  p.b1 = b.y; p.b2 = b.z; p.b3 = b.w;
+  p.sleft = -log_pos(u01nz(b.x));
+  p.alive = true;
+}
+
+template <bool TWO_TERM>
+__global__ __launch_bounds__(kXyzBlock) void mcxyz_kernel(XyzArgs a) {
+  __shared__ XyzTissue T[NMC_NTISS];
+  __shared__ nmc_totals st;
+  for (int i = threadIdx.x; i < NMC_NTISS; i += blockDim.x) T[i] = a.tissue[i];
+  if (threadIdx.x < TF_COUNT) ((double*)&st)[threadIdx.x] = 0.0;
+  if (threadIdx.x < TI_COUNT) (&st.photons)[threadIdx.x] = 0ull;
+  __syncthreads();
+
+  const int lane = lane_id();
+  Photon p;
+  p.alive = false;
+  p.x = p.y = p.z = 0.0f;
+  p.ux = p.uy = 0.0f;
+  p.uz = 1.0f;
+  p.W = 0.0f;
+  p.vdep = 0.0;
+  bool finished = false;
+  uint64_t pool_next = 0, pool_end = 0;  // wave-uniform
+  double absorbed = 0.0, absorbed_out = 0.0;
+  uint32_t c_same = 0, c_cross = 0, c_int = 0, c_phot = 0;
+
+  for (;;) {
+    // ---- refill lanes whose photon ended ----
+    uint64_t need = ballot(!p.alive && !finished);
+    while (need) {
+      if (pool_next >= pool_end) {
+        unsigned long long base = 0;
+        if (lane == 0) base = atomicAdd(a.counter, (unsigned long long)a.chunk);
+        base = __shfl(base, 0);
+        if (base >= a.n) {
+          finished = finished || !p.alive;
+          break;
+        }
+        pool_next = base;
+        pool_end = base + a.chunk < a.n ? base + a.chunk : a.n;
+      }
+      uint64_t avail = pool_end - pool_next;
+      uint64_t cnt = (uint64_t)__popcll(need);
+      uint32_t take = (uint32_t)(cnt < avail ? cnt : avail);
+      uint32_t rank = (uint32_t)__popcll(need & lanemask_lt());
+      if (((need >> lane) & 1ull) && rank < take) {
+        p.pid = a.first + pool_next + rank;
+        c_phot++;
+        launch(p, a, T, &st);
+      }
+      pool_next += take;
+      need = ballot(!p.alive && !finished);
+    }
+    if (ballot(p.alive) == 0ull) break;
+    if (!p.alive) continue;
+
+    // ---- HOP: one piece of the current step (kernels.metal:505-685) ----
+    float s_int = p.sleft / p.mus;
+    float tx = kInf, ty = kInf, tz = kInf;
+    if (p.ux > 0.0f) tx = (((float)(p.ix + 1) - a.hx) * a.dx - p.x) * p.rux;
+    else if (p.ux < 0.0f) tx = (((float)p.ix - a.hx) * a.dx - p.x) * p.rux;
+    if (p.uy > 0.0f) ty = (((float)(p.iy + 1) - a.hy) * a.dy - p.y) * p.ruy;
+    else if (p.uy < 0.0f) ty = (((float)p.iy - a.hy) * a.dy - p.y) * p.ruy;
+    if (p.uz > 0.0f) tz = ((float)(p.iz + 1) * a.dz - p.z) * p.ruz;
+    else if (p.uz < 0.0f) tz = ((float)p.iz * a.dz - p.z) * p.ruz;
+    tx = fmaxf(tx, 0.0f);
+    ty = fmaxf(ty, 0.0f);
+    tz = fmaxf(tz, 0.0f);
+    int axis;
+    float s_face;
+    if (tx <= ty && tx <= tz) {
+      axis = 0;
+      s_face = tx;
+    } else if (ty <= tz) {
+      axis = 1;
+      s_face = ty;
+    } else {
+      axis = 2;
+      s_face = tz;
+    }
+    float s_surf = kInf;
+    if (a.det_state == NMC_DET_REFLECTANCE) {
+      if (p.uz < 0.0f && p.z > a.zsurf) s_surf = fmaxf((a.zsurf - p.z) * p.ruz, 0.0f);
+    } else {
+      if (p.uz > 0.0f && p.z < a.zsurf) s_surf = fmaxf((a.zsurf - p.z) * p.ruz, 0.0f);
+    }
+    int kind;
+    float s;
+    if (s_int <= s_face && s_int <= s_surf) {
+      kind = 0;
+      s = s_int;
+    } else if (s_surf <= s_face) {
+      kind = 1;
+      s = s_surf;
+    } else {
+      kind = 2;
+      s = s_face;
+    }
+    p.x = fmaf(s, p.ux, p.x);
+    p.y = fmaf(s, p.uy, p.y);
+    p.z = fmaf(s, p.uz, p.z);
+    // DROP (kernels.metal:523-533)
+    float em = expm1_f(-(p.mua * s));
+    float dep = -(p.W * em);
+    p.W = p.W - dep;
+    if (p.inside) {
+      p.vdep += (double)dep;
+      absorbed += (double)dep;
+    } else {
+      absorbed_out += (double)dep;
+    }
+    p.events++;
+    bool alive = true;
+    if (kind == 0) {
+      p.sleft = 0.0f;
+      c_same++;
+    } else {
+      c_cross++;
+      p.sleft = p.sleft - s * p.mus;
+      if (!(p.sleft > 0.0f)) p.sleft = 0.0f;
+      if (kind == 1) {
+        // detector plane (kernels.metal:556-630)
+        p.z = a.zsurf;
+        lds_addi(&st, TI_SURFACE_HITS, 1);
+        float uzt;
+        float rsp = fresnel(a.n1, a.n2, fabsf(p.uz), &uzt);
+        if (rsp < 1.0f) {
+          float wt = p.W * (1.0f - rsp);
+          float ro = sqrtf(p.x * p.x + p.y * p.y);
+          if (ro >= a.rmin && ro <= a.rmax && uzt >= a.cos_na) {
+            int64_t k = (int64_t)clampi(p.ix, a.Nx) * a.Ny + clampi(p.iy, a.Ny);
+            unsafeAtomicAdd(&a.ryx[k], (double)wt);
+            atomicAdd(&a.det2d[k], 1u);
+            lds_addf(&st, TF_RD, (double)wt);
+            lds_addi(&st, TI_DETECTIONS, 1);
+          } else {
+            lds_addf(&st, TF_RD_OUTSIDE, (double)wt);
+          }
+        }
+        p.uz = -p.uz;
+        p.ruz = -p.ruz;
+        p.W = p.W * rsp;
+        if (!(p.W > 0.0f)) alive = false;
+      } else {
+        // voxel face (kernels.metal:641-679)
+        if (p.inside && a.F && p.vdep != 0.0) unsafeAtomicAdd(&a.F[vidx(a, p.ix, p.iy, p.iz)], p.vdep);
+        p.vdep = 0.0;
+        if (axis == 0) p.ix += p.ux > 0.0f ? 1 : -1;
+        else if (axis == 1) p.iy += p.uy > 0.0f ? 1 : -1;
+        else p.iz += p.uz > 0.0f ? 1 : -1;
+        p.inside = p.ix >= 0 && p.ix < a.Nx && p.iy >= 0 && p.iy < a.Ny && p.iz >= 0 && p.iz < a.Nz;
+        if (!p.inside && (a.bmode == 1 || (a.bmode == 2 && p.iz < 0))) {
+          tally_escape(&st, a, p.iz, p.W);
+          alive = false;
+        } else {
+          p.tis = a.V[vidx(a, clampi(p.ix, a.Nx), clampi(p.iy, a.Ny), clampi(p.iz, a.Nz))];
+          p.mua = T[p.tis].mua;
+          p.mus = T[p.tis].mus;
+        }
+      }
+    }
+    if (alive && p.events >= a.max_events) {
+      lds_addf(&st, TF_CAPPED, (double)p.W);
+      lds_addi(&st, TI_CAPPED, 1);
+      alive = false;
+    }
+    if (alive && !(p.sleft > 0.0f)) {
+      // SPIN (kernels.metal:695-730)
+      float ct = TWO_TERM ? tt_cos(T[p.tis].tt, u01(p.b1)) : hg_cos(T[p.tis].g, u01(p.b1));
+      spin(ct, u01(p.b2), p.ux, p.uy, p.uz);
+      p.rux = 1.0f / p.ux;
+      p.ruy = 1.0f / p.uy;
+      p.ruz = 1.0f / p.uz;
+      c_int++;
+      // ROULETTE (kernels.metal:740-744)
+      if (p.W < a.thr) {
+        if (u01(p.b3) <= a.chance) {
+          float nw = p.W / a.chance;
+          lds_addf(&st, TF_ROULETTE_GAIN, (double)nw - (double)p.W);
+          lds_addi(&st, TI_ROULETTE_SURVIVALS, 1);
+          p.W = nw;
+        } else {
+          lds_addf(&st, TF_ROULETTE_LOSS, (double)p.W);
+          lds_addi(&st, TI_ROULETTE_KILLS, 1);
+          alive = false;
+        }
+      }
+      if (alive) {
+        p.ev++;
+        U4 b = philox(p.pid, p.ev, 0u, a.key0, a.key1);
+        p.b1 = b.y;
+        p.b2 = b.z;
+        p.b3 = b.w;
+        p.sleft = -log_pos(u01nz(b.x));
+      }
+    }
+    if (!alive) {
+      if (p.inside && a.F && p.vdep != 0.0) unsafeAtomicAdd(&a.F[vidx(a, p.ix, p.iy, p.iz)], p.vdep);
+      p.vdep = 0.0;
+      p.alive = false;
+    }
+  }
+
+  // ---- workgroup totals -> global ----
+  lds_addf(&st, TF_ABSORBED, absorbed);
+  lds_addf(&st, TF_ABSORBED_OUTSIDE, absorbed_out);
+  lds_addi(&st, TI_SUB_SAME, c_same);
+  lds_addi(&st, TI_SUB_CROSS, c_cross);
+  lds_addi(&st, TI_INTERACTIONS, c_int);
+  lds_addi(&st, TI_PHOTONS, c_phot);
+  __syncthreads();
+  if (threadIdx.x < TF_COUNT) unsafeAtomicAdd(&((double*)a.tot)[threadIdx.x], ((double*)&st)[threadIdx.x]);
+  if (threadIdx.x < TI_COUNT)
+    atomicAdd(&(&a.tot->photons)[threadIdx.x], (unsigned long long)(&st.photons)[threadIdx.x]);
+}
+
+__global__ void zero_u64_kernel(unsigned long long* p, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
+
+}  // namespace
+
+hipError_t launch_mcxyz(const XyzArgs& a, bool two_term, int grid, hipStream_t s) {
+  if (two_term) hipLaunchKernelGGL(mcxyz_kernel<true>, dim3(grid), dim3(kXyzBlock), 0, s, a);
+  else hipLaunchKernelGGL(mcxyz_kernel<false>, dim3(grid), dim3(kXyzBlock), 0, s, a);
+  return hipGetLastError();
+}
+
+int mcxyz_blocks_per_cu(bool two_term) {
+  int nb = 0;
+  hipError_t e = two_term
+      ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, mcxyz_kernel<true>, kXyzBlock, 0)
+      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, mcxyz_kernel<false>, kXyzBlock, 0);
+  return (e == hipSuccess && nb > 0) ? nb : 4;
+}
+
+hipError_t launch_zero_u64(unsigned long long* p, int n, hipStream_t s) {
+  hipLaunchKernelGGL(zero_u64_kernel, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
+  return hipGetLastError();
+}
+
+}  // namespace nmc

@@ -1,0 +1,250 @@
+// nmc_device.h - device building blocks shared by the transport kernels.
+//
+// Every routine here is written so that, compiled with -ffp-contract=off, it
+// performs exactly the IEEE binary32 operations listed in its body (explicit
+// fmaf only); the CPU oracle performs the same operations, which makes photon
+// trajectories identical on both sides for the same Philox stream.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace nmc {
+
+constexpr float kInf = __builtin_inff();
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).  Counter = (photon id lo,
+// photon id hi, event index, tag); key = (seed lo, seed hi + stream).  The key
+// schedule is wave-uniform, so it stays in scalar registers.
+// ---------------------------------------------------------------------------
+struct U4 { uint32_t x, y, z, w; };
+
+__device__ __forceinline__ U4 philox(uint64_t pid, uint32_t ev, uint32_t tag, uint32_t k0,
+                                     uint32_t k1) {
+  uint32_t c0 = (uint32_t)pid, c1 = (uint32_t)(pid >> 32), c2 = ev, c3 = tag;
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+    uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+    uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+    uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+    c1 = (uint32_t)p1;
+    c3 = (uint32_t)p0;
+    c0 = n0;
+    c2 = n2;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  return U4{c0, c1, c2, c3};
+}
+
+// [0,1): top 24 bits scaled by 2^-24 (the reference's rand() mapping,
+// helpers.metal:319-371); (0,1]: the same shifted by one ulp of 2^-24, used
+// where the reference rejects zero.
+__device__ __forceinline__ float u01(uint32_t x) { return (float)(x >> 8) * 0x1p-24f; }
+__device__ __forceinline__ float u01nz(uint32_t x) { return (float)((x >> 8) + 1u) * 0x1p-24f; }
+
+// ---------------------------------------------------------------------------
+// Float transcendentals (argument domains as used by the transport loop).
+// ---------------------------------------------------------------------------
+
+// log of a positive normal float: x = 2^k m, m in [sqrt2/2, sqrt2), s = f/(2+f),
+// log(1+f) = f - (hfsq - s (hfsq + R)), R = atanh series 2z/3 + ... + 2z^4/9.
+__device__ __forceinline__ float log_pos(float x) {
+  uint32_t b = __float_as_uint(x);
+  int k = (int)(b >> 23) - 127;
+  float m = __uint_as_float((b & 0x007fffffu) | 0x3f800000u);
+  if (m > 0x1.6a09e6p+0f) {
+    m = m * 0.5f;
+    k += 1;
+  }
+  float f = m - 1.0f;
+  float s = f / (2.0f + f);
+  float z = s * s;
+  float R = z * fmaf(z, fmaf(z, fmaf(z, 0x1.c71c72p-3f, 0x1.24924ap-2f), 0x1.99999ap-2f),
+                     0x1.555556p-1f);
+  float hfsq = 0.5f * f * f;
+  float fk = (float)k;
+  return fk * 0x1.62e4p-1f - ((hfsq - (s * (hfsq + R) + fk * 0x1.7f7d1cp-20f)) - f);
+}
+
+// expm1(r), |r| <= ln2/2: Taylor series to r^9, Horner with fma.
+__device__ __forceinline__ float em1_core(float r) {
+  float p = fmaf(r, 0x1.71de3ap-19f, 0x1.a01a02p-16f);
+  p = fmaf(r, p, 0x1.a01a02p-13f);
+  p = fmaf(r, p, 0x1.6c16c2p-10f);
+  p = fmaf(r, p, 0x1.111112p-7f);
+  p = fmaf(r, p, 0x1.555556p-5f);
+  p = fmaf(r, p, 0x1.555556p-3f);
+  p = fmaf(r, p, 0.5f);
+  return fmaf(r * r, p, r);
+}
+
+// exp(x) by Cody-Waite reduction k = rint(x/ln2); flushed to 0 below x = -87.
+__device__ __forceinline__ float exp_f(float x) {
+  if (x < -87.0f) return 0.0f;
+  if (x > 88.5f) return kInf;
+  float k = rintf(x * 0x1.715476p+0f);
+  float r = fmaf(-k, 0x1.62e4p-1f, x);
+  r = fmaf(-k, 0x1.7f7d1cp-20f, r);
+  float e = 1.0f + em1_core(r);
+  int ki = (int)k;
+  if (ki > 127) {
+    e = e * 2.0f;
+    ki -= 1;
+  }
+  return e * __uint_as_float((uint32_t)(ki + 127) << 23);
+}
+
+__device__ __forceinline__ float expm1_f(float x) {
+  if (fabsf(x) <= 0x1.62e430p-2f) return em1_core(x);
+  return exp_f(x) - 1.0f;
+}
+
+// sin(2 pi u), cos(2 pi u), u in [0,1]: exact quarter-turn reduction, Taylor
+// polynomials of sin (to f^9) and cos (to f^10) of 2 pi f, |f| <= 1/8.
+__device__ __forceinline__ void sincos2pi(float u, float* s, float* c) {
+  float q = rintf(u * 4.0f);
+  float f = u - q * 0.25f;
+  float t = f * f;
+  float ps = fmaf(t, 0x1.507834p+5f, -0x1.32d2ccp+6f);
+  ps = fmaf(t, ps, 0x1.466bc6p+6f);
+  ps = fmaf(t, ps, -0x1.4abbcep+5f);
+  ps = fmaf(t, ps, 0x1.921fb6p+2f);
+  ps = ps * f;
+  float pc = fmaf(t, -0x1.a6d1f2p+4f, 0x1.e1f506p+5f);
+  pc = fmaf(t, pc, -0x1.55d3c8p+6f);
+  pc = fmaf(t, pc, 0x1.03c1f0p+6f);
+  pc = fmaf(t, pc, -0x1.3bd3ccp+4f);
+  pc = fmaf(t, pc, 1.0f);
+  int qi = ((int)q) & 3;
+  float ss = (qi & 1) ? pc : ps;
+  float cc = (qi & 1) ? ps : pc;
+  *s = (qi & 2) ? -ss : ss;
+  *c = (qi == 1 || qi == 2) ? -cc : cc;
+}
+
+// ---------------------------------------------------------------------------
+// Optics (reference helpers.metal:21-25, 243-314).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool flt_eq(float a, float b) {
+  float e = b * 0x1p-23f * 2.0f;
+  return (a >= b - e) && (a <= b + e);
+}
+
+// Unpolarised Fresnel reflectance from n1 into n2 at incidence cosine ca1;
+// writes the transmission cosine.
+__device__ __forceinline__ float fresnel(float n1, float n2, float ca1, float* ca2) {
+  if (flt_eq(n1, n2)) {
+    *ca2 = ca1;
+    return 0.0f;
+  }
+  if (ca1 > 1.0f - 1.0e-6f) {
+    *ca2 = ca1;
+    float r = (n2 - n1) / (n2 + n1);
+    return r * r;
+  }
+  if (ca1 < 1.0e-6f) {
+    *ca2 = 0.0f;
+    return 1.0f;
+  }
+  float sa1 = sqrtf(1.0f - ca1 * ca1);
+  float sa2 = n1 * sa1 / n2;
+  if (sa2 >= 1.0f) {
+    *ca2 = 0.0f;
+    return 1.0f;
+  }
+  float c2 = sqrtf(1.0f - sa2 * sa2);
+  *ca2 = c2;
+  float cap = ca1 * c2 - sa1 * sa2;
+  float cam = ca1 * c2 + sa1 * sa2;
+  float sap = sa1 * c2 + ca1 * sa2;
+  float sam = sa1 * c2 - ca1 * sa2;
+  return 0.5f * sam * sam * (cam * cam + cap * cap) / (sap * sap * cam * cam);
+}
+
+__device__ __forceinline__ void snell(float n1, float n2, float* ux, float* uy, float* uz) {
+  float c = 1.0f - (1.0f - *uz * *uz) * n1 * n1 / n2 / n2;
+  if (c < 0.0f) {
+    if (fabsf(*uz) > 1.0f) *uz = *uz < 0.0f ? -1.0f : 1.0f;
+    return;
+  }
+  *ux = *ux * n1 / n2;
+  *uy = *uy * n1 / n2;
+  *uz = sqrtf(c);
+}
+
+// ---------------------------------------------------------------------------
+// Scattering (reference kernels.metal:695-730, 1053-1070).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float hg_cos(float g, float u) {
+  float ct;
+  if (flt_eq(g, 0.0f)) {
+    ct = 2.0f * u - 1.0f;
+  } else if (flt_eq(g, 1.0f)) {
+    ct = 1.0f;
+  } else {
+    float t = (1.0f - g * g) / (1.0f - g + 2.0f * g * u);
+    ct = (1.0f + g * g - t * t) / (2.0f * g);
+  }
+  return fminf(fmaxf(ct, -1.0f), 1.0f);
+}
+
+// Two-term phase-function constants per tissue, derived on the host in double
+// from (gf, af, gb, ab, CC): c1 = (1+g^2)/(2g), c2 = 1/(2g), iq2 = (1-g)^(-2a),
+// iq3 = (1+g)^(-2a), e = -1/a; iso* marks g == 0 (isotropic branch).
+struct TTCoef {
+  float cc, c1f, c2f, iq2f, iq3f, ef, c1b, c2b, iq2b, iq3b, eb;
+  int isof, isob;
+};
+
+__device__ __forceinline__ float pow_neg(float b, float e) {
+  if (e == -2.0f) return 1.0f / (b * b);
+  return exp_f(e * log_pos(b));
+}
+
+__device__ __forceinline__ float tt_cos(const TTCoef& t, float u) {
+  float ct;
+  if (u < t.cc) {
+    float v = u / t.cc;
+    if (t.isof) ct = 2.0f * v - 1.0f;
+    else ct = t.c1f - t.c2f * pow_neg(v * t.iq2f + (1.0f - v) * t.iq3f, t.ef);
+  } else {
+    float v = (u - t.cc) / (1.0f - t.cc);
+    if (t.isob) ct = 2.0f * v - 1.0f;
+    else ct = -(t.c1b - t.c2b * pow_neg(v * t.iq2b + (1.0f - v) * t.iq3b, t.eb));
+  }
+  return fminf(fmaxf(ct, -1.0f), 1.0f);
+}
+
+// Rotate direction (ux,uy,uz) by polar cosine `cost` and azimuth 2*pi*u_psi.
+__device__ __forceinline__ void spin(float cost, float u_psi, float& ux, float& uy, float& uz) {
+  float sint = sqrtf(1.0f - cost * cost);
+  float sp, cp;
+  sincos2pi(u_psi, &sp, &cp);
+  float x = ux, y = uy, z = uz;
+  if (fabsf(z) > 1.0f - 1.0e-6f) {
+    ux = sint * cp;
+    uy = sint * sp;
+    uz = z >= 0.0f ? cost : -cost;
+  } else {
+    float tmp = sqrtf(x * x + y * y);
+    float st = sint / tmp;
+    ux = st * (x * z * cp - y * sp) + x * cost;
+    uy = st * (y * z * cp + x * sp) + y * cost;
+    uz = -(sint * cp * tmp) + z * cost;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Wave helpers (wave64).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+__device__ __forceinline__ uint64_t lanemask_lt() {
+  int l = __lane_id();
+  return l == 0 ? 0ull : (~0ull >> (64 - l));
+}
+
+}  // namespace nmc

@@ -1,0 +1,118 @@
+"""Python host interface mirroring the reference's MetalMC object.
+
+The reference driver (NMC/NMCCode/main.m:151-191) creates a MetalMC for one
+simulation kind, hands it the parameter structs and voxel map, runs batches
+and collects the tallies.  `MetalMC` below keeps those method names
+(snake_cased) and their meaning, on top of the C ABI in include/nmc.h:
+
+    initWithDevice::     -> MetalMC(kernel_type, device)      MetalMC.h:25
+    prepareData:::       -> prepare_data(rp, tp, v, ext)      MetalMC.h:26
+    setNumRunsSeed::     -> set_num_runs_seed(seed, first)    MetalMC.h:30
+    sendComputeCommand   -> send_compute_command(n)           MetalMC.h:27
+    getComputeResults:.. -> get_compute_results()             MetalMC.h:28
+
+Errors raise `NmcError` (the reference logs and continues or exits).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import params as P
+from ._lib import NmcError, check, lib
+
+
+class MetalMC:
+    def __init__(self, kernel_type: int, device: int = 0):
+        self._h = C.c_void_p()
+        check(lib().nmc_create(int(kernel_type), int(device), C.byref(self._h)))
+        self.kernel_type = int(kernel_type)
+        self.rp = self.tp = self.ext = None
+
+    # -- configuration -------------------------------------------------
+    def prepare_data(self, run_params: P.RunParams, tiss_params: P.TissueParams,
+                     v: np.ndarray | None = None, ext: P.ExtConfig | None = None) -> None:
+        self.rp, self.tp = run_params, tiss_params
+        self.ext = ext if ext is not None else P.ExtConfig.default()
+        vp = C.POINTER(C.c_uint8)()
+        if v is not None:
+            self._v = np.ascontiguousarray(v, np.uint8)
+            vp = self._v.ctypes.data_as(C.POINTER(C.c_uint8))
+        check(lib().nmc_prepare(self._h, C.byref(run_params), C.byref(tiss_params), vp,
+                                C.byref(self.ext)))
+
+    def set_num_runs_seed(self, seed: int, first_photon: int = 0) -> None:
+        check(lib().nmc_set_seed(self._h, int(seed), int(first_photon)))
+
+    # -- execution ----------------------------------------------------
+    def send_compute_command(self, n_photons: int) -> None:
+        """Transports the next n_photons photons and waits for completion."""
+        check(lib().nmc_run(self._h, int(n_photons)))
+
+    def send_compute_command_async(self, n_photons: int) -> None:
+        check(lib().nmc_run_async(self._h, int(n_photons)))
+
+    def synchronize(self) -> None:
+        check(lib().nmc_synchronize(self._h))
+
+    def last_kernel_ms(self) -> float:
+        ms = C.c_float()
+        check(lib().nmc_last_kernel_ms(self._h, C.byref(ms)))
+        return ms.value
+
+    # -- results --------------------------------------------------------
+    def get_compute_results(self, fluence: bool = True) -> P.TallyArrays:
+        if self.rp is None:
+            raise NmcError(P.ERR_STATE, "prepare_data has not been called")
+        no_fl = bool(self.ext.flags & P.FLAG_NO_FLUENCE)
+        t = P.TallyArrays(self.rp, self.ext, self.kernel_type, fluence=fluence and not no_fl)
+        check(lib().nmc_get_results(self._h, C.byref(t.res)))
+        return t
+
+    def reset_tallies(self) -> None:
+        check(lib().nmc_reset_tallies(self._h))
+
+    def close(self) -> None:
+        if self._h:
+            lib().nmc_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def validate(sim: int, rp: P.RunParams, tp: P.TissueParams, v: np.ndarray | None = None,
+             ext: P.ExtConfig | None = None) -> None:
+    """Runs the library's configuration checks (no device needed)."""
+    vp = C.POINTER(C.c_uint8)()
+    if v is not None:
+        v = np.ascontiguousarray(v, np.uint8)
+        vp = v.ctypes.data_as(C.POINTER(C.c_uint8))
+    check(lib().nmc_validate(int(sim), C.byref(rp), C.byref(tp), vp,
+                             C.byref(ext) if ext is not None else C.POINTER(P.ExtConfig)()))
+
+
+def simulate(w, n_photons: int | None = None, seed: int | None = None, first_photon: int = 0,
+             device: int = 0, fluence: bool = True) -> P.TallyArrays:
+    """One-shot run of a workload; returns the host tallies (with .kernel_ms)."""
+    ext = P.ExtConfig.default()
+    C.memmove(C.byref(ext), C.byref(w.ext), C.sizeof(ext))
+    if not fluence:
+        ext.flags |= P.FLAG_NO_FLUENCE
+    with MetalMC(w.sim, device) as mc:
+        mc.prepare_data(w.rp, w.tp, w.v, ext)
+        mc.set_num_runs_seed(ext.seed if seed is None else seed, first_photon)
+        mc.send_compute_command(w.n_photons if n_photons is None else n_photons)
+        t = mc.get_compute_results(fluence=fluence)
+        t.kernel_ms = mc.last_kernel_ms()
+    return t

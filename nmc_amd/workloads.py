@@ -1,0 +1,157 @@
+"""Synthetic workloads: the benchmark configurations and small parity cases.
+
+Each builder returns a `Workload` holding the reference-format structs
+(`RunParams`, `TissueParams`), the uint8 voxel map (mcxyz) and an `ExtConfig`.
+Units are cm throughout.
+"""
+from __future__ import annotations
+
+import dataclasses
+
+import numpy as np
+
+from . import params as P
+
+
+@dataclasses.dataclass
+class Workload:
+    name: str
+    sim: int
+    rp: P.RunParams
+    tp: P.TissueParams
+    v: np.ndarray | None          # uint8, index iz*Ny*Nx + ix*Ny + iy
+    ext: P.ExtConfig
+    n_photons: int
+
+
+def _rp(name: str, **kw) -> P.RunParams:
+    rp = P.RunParams()
+    rp.myname = name.encode()[: P.STRLEN - 1]
+    rp.n1 = rp.n2 = 1.0
+    rp.na = 90.0
+    rp.det_radius = 1e6
+    rp.zfocus = 1e12
+    for k, v in kw.items():
+        setattr(rp, k, v)
+    return rp
+
+
+def _tp(props: dict[int, tuple]) -> P.TissueParams:
+    tp = P.TissueParams()
+    for t, p in props.items():
+        tp.muav[t], tp.musv[t], tp.gv[t] = p[:3]
+        if len(p) > 3:
+            tp.gf[t], tp.af[t], tp.gb[t], tp.ab[t], tp.CC[t] = p[3:8]
+    return tp
+
+
+def voxel_index(ix, iy, iz, nx, ny):
+    """Linear voxel index of kernels.metal:482 (iz*Ny*Nx + ix*Ny + iy)."""
+    return iz * ny * nx + ix * ny + iy
+
+
+def layered_volume(nx, ny, nz, dx, dz, labeler) -> np.ndarray:
+    """uint8 map labelled by voxel centre, stored in the reference's order.
+
+    labeler(xc, yc, zc) takes broadcast arrays of voxel centres (x, y centred on
+    the grid axis, z from 0 at the top) and returns the tissue id per voxel.
+    """
+    xc = (np.arange(nx) + 0.5 - nx / 2) * dx
+    yc = (np.arange(ny) + 0.5 - ny / 2) * dx
+    zc = (np.arange(nz) + 0.5) * dz
+    Z, X, Y = np.meshgrid(zc, xc, yc, indexing="ij")     # axes (iz, ix, iy)
+    t = labeler(X, Y, Z).astype(np.uint8)
+    return np.ascontiguousarray(t).reshape(-1)
+
+
+def skin_vessel(n_photons: int = 100_000_000, n: int = 200, seed: int = 1) -> Workload:
+    """200^3 skin-vessel mcxyz case: air / epidermis / dermis / blood cylinder.
+
+    0.005 cm voxels, air above z = 0.01, 60 um epidermis, dermis below, a blood
+    cylinder along x of radius 0.05 cm centred 0.06 cm deep; n = 1 everywhere;
+    flat collimated beam of radius 0.06 cm launched at z = 0.0101 with the
+    detector plane at zsurf = 0.01 and escape at every grid face.
+    """
+    d = 1.0 / n
+    rp = _rp("skinvessel", Nx=n, Ny=n, Nz=n, dx=d, dy=d, dz=d, Nt=4,
+             mcflag=0, launchflag=0, boundaryflag=1, xs=0.0, ys=0.0, zs=0.0101,
+             radius=0.06, waist=0.06, zsurf=0.01, mckernelflag=P.SIM_MCXYZ,
+             det_state=P.DET_REFLECTANCE)
+    tp = _tp({1: (1e-4, 1.0, 1.0), 2: (15.0, 300.0, 0.9), 3: (0.5, 200.0, 0.9),
+              4: (230.0, 100.0, 0.98)})
+
+    def lab(X, Y, Z):
+        t = np.full(X.shape, 3, np.uint8)
+        t[(Z >= 0.01) & (Z < 0.016)] = 2
+        t[(Y ** 2 + (Z - 0.06) ** 2) <= 0.05 ** 2] = 4
+        t[Z < 0.01] = 1
+        return t
+
+    v = layered_volume(n, n, n, d, d, lab)
+    return Workload("mcxyz_skinvessel_%d" % n, P.SIM_MCXYZ, rp, tp, v,
+                    P.ExtConfig.default(seed=seed), n_photons)
+
+
+def mcsub_semi_infinite(n_photons: int = 1_000_000, seed: int = 1) -> Workload:
+    """Semi-infinite homogeneous mcsub case: mua 1, mus 100, g 0.9, n 1.33/1.0,
+    pencil beam, Rd(r, alpha) 100 x 30 with dr 0.01, A(r, z) 100 x 100."""
+    rp = _rp("mcsub", mcflag=0, boundaryflag=1, radius=0.0, n1=1.33, n2=1.0,
+             mckernelflag=P.SIM_MCSUB)
+    tp = _tp({1: (1.0, 100.0, 0.9)})
+    ext = P.ExtConfig.default(seed=seed, nr=100, nz=100, na=30, dr=0.01, dz=0.01)
+    return Workload("mcsub_semi_infinite", P.SIM_MCSUB, rp, tp, None, ext, n_photons)
+
+
+def tiny_voxel_case(kind: str = "mismatch", n_photons: int = 2000, seed: int = 7,
+                    n: int = 16) -> Workload:
+    """Small mcxyz cases for parity: every launch mode / boundary mode can be set
+    through `kind`, keeping the oracle run under a second."""
+    d = 0.02
+    base = dict(Nx=n, Ny=n, Nz=n, dx=d, dy=d, dz=d, Nt=3, mcflag=0, launchflag=0,
+                boundaryflag=1, xs=0.0, ys=0.0, zs=0.0201, radius=0.05, waist=0.05,
+                zsurf=0.02, mckernelflag=P.SIM_MCXYZ, det_state=P.DET_REFLECTANCE,
+                det_radius=0.1, na=60.0)
+    tp = _tp({1: (0.01, 1.0, 1.0), 2: (2.0, 60.0, 0.8), 3: (8.0, 30.0, 0.5),
+              4: (1.0, 40.0, 0.9, 0.9, 0.5, 0.3, 0.5, 0.8)})
+    if kind == "mismatch":
+        base.update(n1=1.4, n2=1.0)
+    elif kind == "matched":
+        base.update(n1=1.0, n2=1.0, det_radius=1e6, na=90.0)
+    elif kind == "gaussian":
+        base.update(mcflag=1, n1=1.37, n2=1.0, zs=0.0, radius=0.03, waist=0.02, zfocus=0.1)
+    elif kind == "iso":
+        base.update(mcflag=2, zs=0.1, xs=0.01, ys=-0.02, n1=1.33, n2=1.0)
+    elif kind == "rect":
+        base.update(mcflag=3, radius=0.08, n1=1.33, n2=1.0)
+    elif kind == "point":
+        base.update(mcflag=4, n1=1.33, n2=1.0)
+    elif kind == "manual":
+        base.update(launchflag=1, xs=0.01, ys=0.0, zs=0.05, ux0=0.6, uy0=0.0, uz0=0.8,
+                    n1=1.33, n2=1.0)
+    elif kind == "infinite":
+        base.update(boundaryflag=0, n1=1.33, n2=1.0)
+    elif kind == "surface_only":
+        base.update(boundaryflag=2, n1=1.33, n2=1.0)
+    elif kind == "transmission":
+        base.update(det_state=P.DET_TRANSMITTANCE, zsurf=0.2, n1=1.4, n2=1.0, zs=0.0201,
+                    det_radius=0.2, xd=0.05)
+    elif kind == "two_term":
+        base.update(mckernelflag=P.SIM_MCXYZ_TT, n1=1.4, n2=1.0)
+    else:
+        raise ValueError(kind)
+    rp = _rp("tiny_" + kind, **base)
+
+    def lab(X, Y, Z):
+        t = np.full(X.shape, 2, np.uint8)
+        t[(X ** 2 + (Z - 0.12) ** 2) <= 0.06 ** 2] = 3
+        t[Z < 0.02] = 1
+        if kind == "two_term":
+            t[t == 2] = 4
+        return t
+
+    v = layered_volume(n, n, n, d, d, lab)
+    if kind == "two_term":
+        tp.gf[1], tp.af[1], tp.gb[1], tp.ab[1], tp.CC[1] = 0.0, 0.5, 0.0, 0.5, 1.0
+        tp.gf[3], tp.af[3], tp.gb[3], tp.ab[3], tp.CC[3] = 0.7, 0.7, 0.2, 1.0, 0.6
+    return Workload("tiny_" + kind, rp.mckernelflag, rp, tp, v,
+                    P.ExtConfig.default(seed=seed), n_photons)

@@ -1,0 +1,858 @@
+/*
+ * nmc_oracle.c - CPU restatement of NMC's photon-transport hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP library, never linked into
+ * it.  Written photon-by-photon in the shape of the reference kernels so that
+ * it reads against them line by line; the GPU library reorganises the same
+ * arithmetic into a lane-refilling state machine.
+ *
+ * Algorithm sources (paths relative to the reference root):
+ *   mcxyz  NMC/NMCCode/kernels.metal:321-754   (MCXYZKernel)
+ *   mcsub  NMC/NMCCode/kernels.metal:59-319    (MCSubKernel)
+ *   two-term phase function kernels.metal:894-905,1053-1070 (MCXYZKernelTT)
+ *   Fresnel / Snell NMC/NMCCode/helpers.metal:243-314, FltEq helpers.metal:21-25
+ *   constants NMC/NMCCode/definitions.h:16-82
+ *
+ * Deliberate departures from the reference, shared with the GPU library:
+ *   - RNG: Philox4x32-10 (Salmon et al., SC'11, "Parallel random numbers: as
+ *     easy as 1, 2, 3") keyed by (seed, stream) with counter (photon id,
+ *     event index, tag) instead of RandomGen (helpers.metal:319-371).
+ *   - log/exp/expm1/sin/cos are this file's own float routines (below) so
+ *     that CPU and GPU produce identical trajectories; +,-,*,/,sqrt, fma are
+ *     IEEE-754 binary32 with round-to-nearest, no contraction.
+ *   - Voxel stepping tracks integer voxel indices (a DDA walk) instead of the
+ *     `ls` face nudge of kernels.metal:538 and SameVoxel (helpers.metal:27-39);
+ *     the photon still deposits into the voxel the piece of step lies in.
+ *   - absorb = -W*expm1(-mua*s) instead of W*(1-exp(-mua*s)) (same quantity,
+ *     no cancellation for small mua*s).
+ *   - Reference defects are not reproduced: mcsub HG overwrite (kernels.metal
+ *     :252), bflag never restored (:487), zsurf sleft reassignment (:553-571),
+ *     unreflected photons outside the detector aperture (:580-623), hard
+ *     kills W<1e-6 / num_scatt>10000 (:747-748), RAY_DEPTH truncation
+ *     (:503,683).  A per-photon event watchdog (max_events) replaces the kills
+ *     and its weight is tallied separately.
+ */
+#include "nmc_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define INF_F __builtin_inff()
+
+/* ------------------------------------------------------------------------ */
+/* Philox4x32-10 (Salmon, Moraes, Dror, Shaw 2011).                          */
+/* ------------------------------------------------------------------------ */
+void nmco_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4])
+{
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+    uint32_t k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; ++r) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c1 = (uint32_t)p1;
+        c3 = (uint32_t)p0;
+        c0 = n0;
+        c2 = n2;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+/* Stream of one photon: counter (id lo, id hi, event index, tag). */
+static inline void block(const uint32_t key[2], uint64_t pid, uint32_t ev, uint32_t tag,
+                         uint32_t out[4])
+{
+    uint32_t ctr[4] = {(uint32_t)pid, (uint32_t)(pid >> 32), ev, tag};
+    nmco_philox4x32_10(ctr, key, out);
+}
+
+/* uniform in [0,1): top 24 bits, the reference's (u32>>8)*2^-24 mapping */
+static inline float u01(uint32_t x) { return (float)(x >> 8) * 0x1p-24f; }
+/* uniform in (0,1]: used where the reference loops `while ((rnd=rand())<=0)` */
+static inline float u01nz(uint32_t x) { return (float)((x >> 8) + 1u) * 0x1p-24f; }
+
+/* ------------------------------------------------------------------------ */
+/* Transcendentals.                                                          */
+/* ------------------------------------------------------------------------ */
+static inline float bits_f(uint32_t b) { float f; memcpy(&f, &b, 4); return f; }
+static inline uint32_t f_bits(float f) { uint32_t b; memcpy(&b, &f, 4); return b; }
+
+/*
+ * Natural log of a positive normal float.  x = 2^k * m with m in
+ * [sqrt(2)/2, sqrt(2)); log(m) = log(1+f) = 2 atanh(s), s = f/(2+f), written as
+ * f - (hfsq - s*(hfsq + R)) with R the atanh series 2z/3 + 2z^2/5 + 2z^3/7 +
+ * 2z^4/9, z = s^2 (the reduction of Cody & Waite / fdlibm's e_logf, with the
+ * plain Taylor coefficients).  k*ln2 is split hi+lo, hi with 7 trailing zero
+ * bits so k*hi is exact.
+ */
+float nmco_logf(float x)
+{
+    if (!(x > 0.0f)) return x == 0.0f ? -INF_F : __builtin_nanf("");
+    uint32_t b = f_bits(x);
+    int k = (int)(b >> 23) - 127;
+    float m = bits_f((b & 0x007fffffu) | 0x3f800000u); /* [1,2) */
+    if (m > 0x1.6a09e6p+0f) { m = m * 0.5f; k += 1; }
+    float f = m - 1.0f;
+    float s = f / (2.0f + f);
+    float z = s * s;
+    float R = z * fmaf(z, fmaf(z, fmaf(z, 0x1.c71c72p-3f, 0x1.24924ap-2f), 0x1.99999ap-2f),
+                       0x1.555556p-1f);
+    float hfsq = 0.5f * f * f;
+    float fk = (float)k;
+    return fk * 0x1.62e4p-1f - ((hfsq - (s * (hfsq + R) + fk * 0x1.7f7d1cp-20f)) - f);
+}
+
+/* expm1(r) for |r| <= ln2/2: Taylor series to r^9 in Horner form. */
+static inline float em1_core(float r)
+{
+    float p = fmaf(r, 0x1.71de3ap-19f, 0x1.a01a02p-16f);
+    p = fmaf(r, p, 0x1.a01a02p-13f);
+    p = fmaf(r, p, 0x1.6c16c2p-10f);
+    p = fmaf(r, p, 0x1.111112p-7f);
+    p = fmaf(r, p, 0x1.555556p-5f);
+    p = fmaf(r, p, 0x1.555556p-3f);
+    p = fmaf(r, p, 0.5f);
+    return fmaf(r * r, p, r);
+}
+
+/*
+ * exp(x) = 2^k exp(r), k = rint(x/ln2), r = x - k ln2 (Cody-Waite, two fma
+ * steps).  Results below 2^-126 (x < -87) are flushed to 0, above FLT_MAX
+ * give +inf.
+ */
+float nmco_expf(float x)
+{
+    if (x < -87.0f) return 0.0f;
+    if (x > 88.5f) return INF_F;
+    float k = rintf(x * 0x1.715476p+0f);
+    float r = fmaf(-k, 0x1.62e4p-1f, x);
+    r = fmaf(-k, 0x1.7f7d1cp-20f, r);
+    float e = 1.0f + em1_core(r);
+    int ki = (int)k;
+    if (ki > 127) { e = e * 2.0f; ki -= 1; }
+    return e * bits_f((uint32_t)(ki + 127) << 23);
+}
+
+float nmco_expm1f(float x)
+{
+    if (fabsf(x) <= 0x1.62e430p-2f) return em1_core(x);
+    return nmco_expf(x) - 1.0f;
+}
+
+/*
+ * sin(2 pi u), cos(2 pi u) for u in [0, 1]: exact quarter-turn reduction
+ * u = q/4 + f, |f| <= 1/8, then Taylor series of sin(2 pi f) to f^9 and of
+ * cos(2 pi f) to f^10 (coefficients (2pi)^n/n! rounded to float).
+ */
+void nmco_sincos2pi(float u, float* s, float* c)
+{
+    float q = rintf(u * 4.0f);
+    float f = u - q * 0.25f;
+    float t = f * f;
+    float ps = fmaf(t, 0x1.507834p+5f, -0x1.32d2ccp+6f);
+    ps = fmaf(t, ps, 0x1.466bc6p+6f);
+    ps = fmaf(t, ps, -0x1.4abbcep+5f);
+    ps = fmaf(t, ps, 0x1.921fb6p+2f);
+    ps = ps * f;
+    float pc = fmaf(t, -0x1.a6d1f2p+4f, 0x1.e1f506p+5f);
+    pc = fmaf(t, pc, -0x1.55d3c8p+6f);
+    pc = fmaf(t, pc, 0x1.03c1f0p+6f);
+    pc = fmaf(t, pc, -0x1.3bd3ccp+4f);
+    pc = fmaf(t, pc, 1.0f);
+    switch (((int)q) & 3) {
+    case 0: *s = ps; *c = pc; break;
+    case 1: *s = pc; *c = -ps; break;
+    case 2: *s = -ps; *c = -pc; break;
+    default: *s = -pc; *c = ps; break;
+    }
+}
+
+void nmco_logf_n(const float* x, float* y, int64_t n) { for (int64_t i = 0; i < n; ++i) y[i] = nmco_logf(x[i]); }
+void nmco_expf_n(const float* x, float* y, int64_t n) { for (int64_t i = 0; i < n; ++i) y[i] = nmco_expf(x[i]); }
+void nmco_expm1f_n(const float* x, float* y, int64_t n) { for (int64_t i = 0; i < n; ++i) y[i] = nmco_expm1f(x[i]); }
+void nmco_sincos2pi_n(const float* u, float* s, float* c, int64_t n)
+{
+    for (int64_t i = 0; i < n; ++i) nmco_sincos2pi(u[i], s + i, c + i);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Optics (helpers.metal:21-25, 243-314).                                    */
+/* ------------------------------------------------------------------------ */
+static inline int flt_eq(float a, float b)
+{
+    float e = b * 0x1p-23f * 2.0f; /* FLT_EPSILON * (1 + error), error = 1 */
+    return (a >= b - e) && (a <= b + e);
+}
+
+/* Unpolarised Fresnel reflectance n1 -> n2 at incidence cosine ca1; writes
+ * the transmission cosine (helpers.metal:243-291). */
+float nmco_fresnel(float n1, float n2, float ca1, float* ca2)
+{
+    if (flt_eq(n1, n2)) { *ca2 = ca1; return 0.0f; }
+    if (ca1 > 1.0f - 1.0e-6f) {
+        *ca2 = ca1;
+        float r = (n2 - n1) / (n2 + n1);
+        return r * r;
+    }
+    if (ca1 < 1.0e-6f) { *ca2 = 0.0f; return 1.0f; }
+    float sa1 = sqrtf(1.0f - ca1 * ca1);
+    float sa2 = n1 * sa1 / n2;
+    if (sa2 >= 1.0f) { *ca2 = 0.0f; return 1.0f; }
+    float c2 = sqrtf(1.0f - sa2 * sa2);
+    *ca2 = c2;
+    float cap = ca1 * c2 - sa1 * sa2;
+    float cam = ca1 * c2 + sa1 * sa2;
+    float sap = sa1 * c2 + ca1 * sa2;
+    float sam = sa1 * c2 - ca1 * sa2;
+    return 0.5f * sam * sam * (cam * cam + cap * cap) / (sap * sap * cam * cam);
+}
+
+/* Snell refraction of a direction from index n1 into n2 (helpers.metal:293-314). */
+static void snell(float n1, float n2, float* ux, float* uy, float* uz)
+{
+    float c = 1.0f - (1.0f - *uz * *uz) * n1 * n1 / n2 / n2;
+    if (c < 0.0f) {
+        if (fabsf(*uz) > 1.0f) *uz = *uz < 0.0f ? -1.0f : 1.0f;
+        return;
+    }
+    *ux = *ux * n1 / n2;
+    *uy = *uy * n1 / n2;
+    *uz = sqrtf(c);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Scattering (kernels.metal:695-730, 1053-1070).                            */
+/* ------------------------------------------------------------------------ */
+float nmco_hg_cos(float g, float u)
+{
+    float ct;
+    if (flt_eq(g, 0.0f)) ct = 2.0f * u - 1.0f;
+    else if (flt_eq(g, 1.0f)) ct = 1.0f;
+    else {
+        float t = (1.0f - g * g) / (1.0f - g + 2.0f * g * u);
+        ct = (1.0f + g * g - t * t) / (2.0f * g);
+    }
+    return fminf(fmaxf(ct, -1.0f), 1.0f);
+}
+
+/* Per-tissue constants of the two-term function, derived in double on the
+ * host (q1, q2, q3 of kernels.metal:900-905). */
+typedef struct { float cc, c1f, c2f, iq2f, iq3f, ef, c1b, c2b, iq2b, iq3b, eb; int isof, isob; } tt_coef;
+
+static void tt_prepare(float gf, float af, float gb, float ab, float cc, tt_coef* t)
+{
+    t->cc = cc;
+    t->isof = gf == 0.0f;
+    t->isob = gb == 0.0f;
+    double g = gf, a = af;
+    if (!t->isof) {
+        t->c1f = (float)((1.0 + g * g) / (2.0 * g));
+        t->c2f = (float)(1.0 / (2.0 * g));
+        t->iq2f = (float)(1.0 / pow(1.0 - g, 2.0 * a));
+        t->iq3f = (float)(1.0 / pow(1.0 + g, 2.0 * a));
+        t->ef = (float)(-1.0 / a);
+    }
+    g = gb; a = ab;
+    if (!t->isob) {
+        t->c1b = (float)((1.0 + g * g) / (2.0 * g));
+        t->c2b = (float)(1.0 / (2.0 * g));
+        t->iq2b = (float)(1.0 / pow(1.0 - g, 2.0 * a));
+        t->iq3b = (float)(1.0 / pow(1.0 + g, 2.0 * a));
+        t->eb = (float)(-1.0 / a);
+    }
+}
+
+static inline float pow_neg(float b, float e)
+{
+    if (e == -2.0f) return 1.0f / (b * b);
+    return nmco_expf(e * nmco_logf(b));
+}
+
+/* One uniform picks the branch (forward with probability CC) and, rescaled
+ * to [0,1) within the branch, samples it. */
+static float tt_cos(const tt_coef* t, float u)
+{
+    float ct;
+    if (u < t->cc) {
+        float v = u / t->cc;
+        if (t->isof) ct = 2.0f * v - 1.0f;
+        else ct = t->c1f - t->c2f * pow_neg(v * t->iq2f + (1.0f - v) * t->iq3f, t->ef);
+    } else {
+        float v = (u - t->cc) / (1.0f - t->cc);
+        if (t->isob) ct = 2.0f * v - 1.0f;
+        else ct = -(t->c1b - t->c2b * pow_neg(v * t->iq2b + (1.0f - v) * t->iq3b, t->eb));
+    }
+    return fminf(fmaxf(ct, -1.0f), 1.0f);
+}
+
+float nmco_tt_cos(float gf, float af, float gb, float ab, float cc, float u)
+{
+    tt_coef t;
+    tt_prepare(gf, af, gb, ab, cc, &t);
+    return tt_cos(&t, u);
+}
+
+/* New direction after deflection cos(theta) = cost and azimuth 2*pi*u_psi
+ * (kernels.metal:705-730; sin(psi) is computed directly). */
+void nmco_spin(float cost, float u_psi, float* ux, float* uy, float* uz)
+{
+    float sint = sqrtf(1.0f - cost * cost);
+    float sp, cp;
+    nmco_sincos2pi(u_psi, &sp, &cp);
+    float x = *ux, y = *uy, z = *uz;
+    if (fabsf(z) > 1.0f - 1.0e-6f) {
+        *ux = sint * cp;
+        *uy = sint * sp;
+        *uz = z >= 0.0f ? cost : -cost;
+    } else {
+        float tmp = sqrtf(x * x + y * y);
+        float st = sint / tmp;
+        *ux = st * (x * z * cp - y * sp) + x * cost;
+        *uy = st * (y * z * cp + x * sp) + y * cost;
+        *uz = -(sint * cp * tmp) + z * cost;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Shared run bookkeeping.                                                   */
+/* ------------------------------------------------------------------------ */
+static double g_last_seconds;
+double nmco_last_seconds(void) { return g_last_seconds; }
+int nmco_max_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+static double now_s(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+static void ext_defaults(const nmc_ext_config* in, nmc_ext_config* e)
+{
+    memset(e, 0, sizeof(*e));
+    e->struct_size = sizeof(*e);
+    e->version = NMC_EXT_VERSION;
+    e->seed = 1;
+    e->threshold = NMC_THRESHOLD;
+    e->chance = NMC_CHANCE;
+    e->max_events = 100000000u;
+    e->nr = 100; e->nz = 100; e->na = 30;
+    e->dr = 0.01f; e->dz = 0.01f;
+    if (in) *e = *in;
+}
+
+static void totals_add(nmc_totals* a, const nmc_totals* b)
+{
+    a->specular += b->specular; a->rd += b->rd; a->rd_outside += b->rd_outside;
+    a->absorbed += b->absorbed; a->absorbed_outside += b->absorbed_outside;
+    a->tt += b->tt; a->lateral += b->lateral; a->top += b->top;
+    a->roulette_gain += b->roulette_gain; a->roulette_loss += b->roulette_loss;
+    a->capped += b->capped;
+    a->photons += b->photons; a->detections += b->detections; a->surface_hits += b->surface_hits;
+    a->escapes_bottom += b->escapes_bottom; a->escapes_side += b->escapes_side;
+    a->escapes_top += b->escapes_top; a->roulette_kills += b->roulette_kills;
+    a->roulette_survivals += b->roulette_survivals; a->capped_photons += b->capped_photons;
+    a->substeps_same += b->substeps_same; a->substeps_cross += b->substeps_cross;
+    a->interactions += b->interactions;
+}
+
+/* Roulette (kernels.metal:740-744).  Returns 0 if the photon is killed. */
+static inline int roulette(float* W, float u, float thr, float chance, nmc_totals* t)
+{
+    if (*W < thr) {
+        if (u <= chance) {
+            float nw = *W / chance;
+            t->roulette_gain += (double)nw - (double)*W;
+            t->roulette_survivals++;
+            *W = nw;
+        } else {
+            t->roulette_loss += (double)*W;
+            t->roulette_kills++;
+            *W = 0.0f;
+            return 0;
+        }
+    }
+    return 1;
+}
+
+/* ------------------------------------------------------------------------ */
+/* mcxyz (kernels.metal:321-754).                                            */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    int Nx, Ny, Nz;
+    float dx, dy, dz, hx, hy;
+    int bmode, det_state, mcflag, launchflag, two_term;
+    float zsurf, n1, n2, rmin, rmax, cos_na;
+    float xs, ys, zs, ux0, uy0, uz0, radius, waist, zfocus;
+    float thr, chance;
+    uint32_t max_events;
+    uint32_t key[2];
+    const uint8_t* V;
+    float mua[NMC_NTISS], mus[NMC_NTISS], g[NMC_NTISS];
+    tt_coef tt[NMC_NTISS];
+} xyz_ctx;
+
+typedef struct { double* F; double* ryx; uint32_t* det; nmc_totals t; } xyz_acc;
+
+static inline float xface(int i, float h, float d) { return ((float)i - h) * d; }
+
+static void xyz_photon(const xyz_ctx* c, uint64_t pid, xyz_acc* a)
+{
+    nmc_totals* t = &a->t;
+    uint32_t b[4];
+    float x, y, z, ux, uy, uz, W = 1.0f;
+    t->photons++;
+
+    /* LAUNCH (kernels.metal:366-463) */
+    if (c->launchflag == 1) {
+        x = c->xs; y = c->ys; z = c->zs;
+        ux = c->ux0; uy = c->uy0; uz = c->uz0;
+    } else {
+        block(c->key, pid, 0u, 0u, b);
+        if (c->mcflag == 0 || c->mcflag == 1) {
+            float r;
+            if (c->mcflag == 0) r = c->radius * sqrtf(u01nz(b[0]));
+            else r = c->radius * sqrtf(-nmco_logf(u01nz(b[0])));
+            float sp, cp;
+            nmco_sincos2pi(u01nz(b[1]), &sp, &cp);
+            x = c->xs + r * cp;
+            y = c->ys + r * sp;
+            z = c->zs;
+            float rf = c->waist * sqrtf(u01nz(b[2]));
+            nmco_sincos2pi(u01nz(b[3]), &sp, &cp);
+            float xf = rf * cp, yf = rf * sp;
+            float ddx = x - xf, ddy = y - yf;
+            float tmp = sqrtf(ddx * ddx + ddy * ddy + c->zfocus * c->zfocus);
+            ux = -ddx / tmp;
+            uy = -ddy / tmp;
+            uz = sqrtf(1.0f - ux * ux - uy * uy);
+            if (c->mcflag == 1) {
+                float ca2;
+                float rsp = nmco_fresnel(c->n2, c->n1, fabsf(uz), &ca2);
+                t->specular += (double)(W * rsp);
+                W = W * (1.0f - rsp);
+                snell(c->n2, c->n1, &ux, &uy, &uz);
+            }
+        } else if (c->mcflag == 2) {
+            float ct = 1.0f - 2.0f * u01(b[0]);
+            float st = sqrtf(1.0f - ct * ct);
+            float sp, cp;
+            nmco_sincos2pi(u01(b[1]), &sp, &cp);
+            x = c->xs; y = c->ys; z = c->zs;
+            ux = st * cp; uy = st * sp; uz = ct;
+        } else if (c->mcflag == 3) {
+            x = c->radius * (u01nz(b[0]) * 2.0f - 1.0f);
+            y = c->radius * (u01nz(b[1]) * 2.0f - 1.0f);
+            z = c->zs;
+            ux = 0.0f; uy = 0.0f; uz = 1.0f;
+        } else {
+            x = 0.0f; y = 0.0f; z = c->zs;
+            ux = 0.0f; uy = 0.0f; uz = 1.0f;
+        }
+    }
+
+    int ix = (int)floorf(c->hx + x / c->dx);
+    int iy = (int)floorf(c->hy + y / c->dy);
+    int iz = (int)floorf(z / c->dz);
+    int inside = ix >= 0 && ix < c->Nx && iy >= 0 && iy < c->Ny && iz >= 0 && iz < c->Nz;
+    if (!inside && (c->bmode == 1 || (c->bmode == 2 && iz < 0))) {
+        if (iz >= c->Nz) { t->tt += (double)W; t->escapes_bottom++; }
+        else if (iz < 0) { t->top += (double)W; t->escapes_top++; }
+        else { t->lateral += (double)W; t->escapes_side++; }
+        return;
+    }
+#define CLAMPI(v, n) ((v) < 0 ? 0 : ((v) >= (n) ? (n) - 1 : (v)))
+#define VIDX(i, j, k) ((int64_t)(k) * c->Ny * c->Nx + (int64_t)(i) * c->Ny + (j))
+    int tis = c->V[VIDX(CLAMPI(ix, c->Nx), CLAMPI(iy, c->Ny), CLAMPI(iz, c->Nz))];
+    float rux = 1.0f / ux, ruy = 1.0f / uy, ruz = 1.0f / uz;
+
+    uint32_t ev = 1;
+    block(c->key, pid, ev, 0u, b);
+    float sleft = -nmco_logf(u01nz(b[0]));
+    uint32_t events = 0;
+    /* Deposits of one voxel visit are summed in double and added to F when
+     * the photon leaves the voxel or ends. */
+    double vdep = 0.0;
+#define FLUSH()                                                        \
+    do {                                                               \
+        if (inside && a->F && vdep != 0.0) a->F[VIDX(ix, iy, iz)] += vdep; \
+        vdep = 0.0;                                                    \
+    } while (0)
+
+    for (;;) {
+        /* HOP: one piece of the step (kernels.metal:505-685) */
+        float mua = c->mua[tis], mus = c->mus[tis];
+        float s_int = sleft / mus;
+        float tx = INF_F, ty = INF_F, tz = INF_F;
+        if (ux > 0.0f) tx = (xface(ix + 1, c->hx, c->dx) - x) * rux;
+        else if (ux < 0.0f) tx = (xface(ix, c->hx, c->dx) - x) * rux;
+        if (uy > 0.0f) ty = (xface(iy + 1, c->hy, c->dy) - y) * ruy;
+        else if (uy < 0.0f) ty = (xface(iy, c->hy, c->dy) - y) * ruy;
+        if (uz > 0.0f) tz = (xface(iz + 1, 0.0f, c->dz) - z) * ruz;
+        else if (uz < 0.0f) tz = (xface(iz, 0.0f, c->dz) - z) * ruz;
+        tx = fmaxf(tx, 0.0f); ty = fmaxf(ty, 0.0f); tz = fmaxf(tz, 0.0f);
+        int axis;
+        float s_face;
+        if (tx <= ty && tx <= tz) { axis = 0; s_face = tx; }
+        else if (ty <= tz) { axis = 1; s_face = ty; }
+        else { axis = 2; s_face = tz; }
+        float s_surf = INF_F;
+        if (c->det_state == NMC_DET_REFLECTANCE) {
+            if (uz < 0.0f && z > c->zsurf) s_surf = fmaxf((c->zsurf - z) * ruz, 0.0f);
+        } else {
+            if (uz > 0.0f && z < c->zsurf) s_surf = fmaxf((c->zsurf - z) * ruz, 0.0f);
+        }
+        int kind; /* 0 interaction, 1 surface, 2 face */
+        float s;
+        if (s_int <= s_face && s_int <= s_surf) { kind = 0; s = s_int; }
+        else if (s_surf <= s_face) { kind = 1; s = s_surf; }
+        else { kind = 2; s = s_face; }
+
+        x = fmaf(s, ux, x);
+        y = fmaf(s, uy, y);
+        z = fmaf(s, uz, z);
+        /* DROP (kernels.metal:523-533) */
+        float em = nmco_expm1f(-(mua * s));
+        float dep = -(W * em);
+        W = W - dep;
+        if (inside) {
+            vdep += (double)dep;
+            t->absorbed += (double)dep;
+        } else {
+            t->absorbed_outside += (double)dep;
+        }
+        events++;
+        int alive = 1;
+        if (kind == 0) {
+            sleft = 0.0f;
+            t->substeps_same++;
+        } else {
+            t->substeps_cross++;
+            sleft = sleft - s * mus;
+            if (!(sleft > 0.0f)) sleft = 0.0f;
+            if (kind == 1) {
+                /* detector plane (kernels.metal:556-630) */
+                z = c->zsurf;
+                t->surface_hits++;
+                float uzt;
+                float rsp = nmco_fresnel(c->n1, c->n2, fabsf(uz), &uzt);
+                if (rsp < 1.0f) {
+                    float wt = W * (1.0f - rsp);
+                    float ro = sqrtf(x * x + y * y);
+                    if (ro >= c->rmin && ro <= c->rmax && uzt >= c->cos_na) {
+                        int px = CLAMPI(ix, c->Nx), py = CLAMPI(iy, c->Ny);
+                        if (a->ryx) a->ryx[(int64_t)px * c->Ny + py] += (double)wt;
+                        if (a->det) a->det[(int64_t)px * c->Ny + py] += 1u;
+                        t->rd += (double)wt;
+                        t->detections++;
+                    } else {
+                        t->rd_outside += (double)wt;
+                    }
+                }
+                uz = -uz;
+                ruz = -ruz;
+                W = W * rsp;
+                if (!(W > 0.0f)) alive = 0;
+            } else {
+                /* voxel face (kernels.metal:641-679) */
+                FLUSH();
+                if (axis == 0) ix += ux > 0.0f ? 1 : -1;
+                else if (axis == 1) iy += uy > 0.0f ? 1 : -1;
+                else iz += uz > 0.0f ? 1 : -1;
+                inside = ix >= 0 && ix < c->Nx && iy >= 0 && iy < c->Ny && iz >= 0 && iz < c->Nz;
+                if (!inside && (c->bmode == 1 || (c->bmode == 2 && iz < 0))) {
+                    if (iz >= c->Nz) { t->tt += (double)W; t->escapes_bottom++; }
+                    else if (iz < 0) { t->top += (double)W; t->escapes_top++; }
+                    else { t->lateral += (double)W; t->escapes_side++; }
+                    alive = 0;
+                } else {
+                    tis = c->V[VIDX(CLAMPI(ix, c->Nx), CLAMPI(iy, c->Ny), CLAMPI(iz, c->Nz))];
+                }
+            }
+        }
+        if (!alive) { FLUSH(); return; }
+        if (events >= c->max_events) {
+            FLUSH();
+            t->capped += (double)W;
+            t->capped_photons++;
+            return;
+        }
+        if (sleft > 0.0f) continue;
+
+        /* SPIN (kernels.metal:695-730) */
+        float ct = c->two_term ? tt_cos(&c->tt[tis], u01(b[1])) : nmco_hg_cos(c->g[tis], u01(b[1]));
+        nmco_spin(ct, u01(b[2]), &ux, &uy, &uz);
+        rux = 1.0f / ux; ruy = 1.0f / uy; ruz = 1.0f / uz;
+        t->interactions++;
+        /* ROULETTE (kernels.metal:740-744) */
+        if (!roulette(&W, u01(b[3]), c->thr, c->chance, t)) { FLUSH(); return; }
+        ev++;
+        block(c->key, pid, ev, 0u, b);
+        sleft = -nmco_logf(u01nz(b[0]));
+    }
+#undef FLUSH
+#undef VIDX
+}
+
+static int xyz_setup(const RunParams* rp, const TissueParams* tp, const uint8_t* v,
+                     const nmc_ext_config* e, xyz_ctx* c)
+{
+    memset(c, 0, sizeof(*c));
+    if (!rp || !tp || !v) return NMC_ERR_ARG;
+    if (rp->Nx <= 0 || rp->Ny <= 0 || rp->Nz <= 0 || !(rp->dx > 0) || !(rp->dy > 0) || !(rp->dz > 0))
+        return NMC_ERR_ARG;
+    c->Nx = rp->Nx; c->Ny = rp->Ny; c->Nz = rp->Nz;
+    c->dx = rp->dx; c->dy = rp->dy; c->dz = rp->dz;
+    c->hx = 0.5f * (float)rp->Nx; c->hy = 0.5f * (float)rp->Ny;
+    c->bmode = rp->boundaryflag; c->det_state = rp->det_state;
+    c->mcflag = rp->mcflag; c->launchflag = rp->launchflag;
+    c->two_term = rp->mckernelflag == NMC_SIM_MCXYZ_TT;
+    c->zsurf = rp->zsurf; c->n1 = rp->n1; c->n2 = rp->n2;
+    if (rp->xd == 0.0f) { c->rmin = 0.0f; c->rmax = rp->det_radius; }
+    else { c->rmin = rp->xd - rp->det_radius; c->rmax = c->rmin + 2.0f * rp->det_radius; }
+    c->cos_na = rp->na >= 90.0f ? -1.0f : (float)cos((double)rp->na * 3.14159265358979323846 / 180.0);
+    c->xs = rp->xs; c->ys = rp->ys; c->zs = rp->zs;
+    c->ux0 = rp->ux0; c->uy0 = rp->uy0; c->uz0 = rp->uz0;
+    c->radius = rp->radius; c->waist = rp->waist; c->zfocus = rp->zfocus;
+    c->thr = e->threshold; c->chance = e->chance; c->max_events = e->max_events;
+    c->key[0] = (uint32_t)e->seed;
+    c->key[1] = (uint32_t)(e->seed >> 32) + e->stream_id;
+    c->V = v;
+    for (int i = 0; i < NMC_NTISS; ++i) {
+        c->mua[i] = tp->muav[i]; c->mus[i] = tp->musv[i]; c->g[i] = tp->gv[i];
+        if (c->two_term) tt_prepare(tp->gf[i], tp->af[i], tp->gb[i], tp->ab[i], tp->CC[i], &c->tt[i]);
+    }
+    int64_t nv = (int64_t)rp->Nx * rp->Ny * rp->Nz;
+    for (int64_t i = 0; i < nv; ++i)
+        if (v[i] >= NMC_NTISS) return NMC_ERR_ARG;
+    return NMC_OK;
+}
+
+int nmco_mcxyz(const RunParams* rp, const TissueParams* tp, const uint8_t* v,
+               const nmc_ext_config* ext, uint64_t first, uint64_t n, int n_threads,
+               nmc_results* out)
+{
+    nmc_ext_config e;
+    ext_defaults(ext, &e);
+    xyz_ctx* c = (xyz_ctx*)malloc(sizeof(xyz_ctx));
+    if (!c) return NMC_ERR_NOMEM;
+    int st = xyz_setup(rp, tp, v, &e, c);
+    if (st != NMC_OK) { free(c); return st; }
+    int64_t nv = (int64_t)c->Nx * c->Ny * c->Nz, n2d = (int64_t)c->Nx * c->Ny;
+    int nt = n_threads > 0 ? n_threads : nmco_max_threads();
+    if ((uint64_t)nt > n) nt = n > 0 ? (int)n : 1;
+    xyz_acc* acc = (xyz_acc*)calloc((size_t)nt, sizeof(xyz_acc));
+    int fail = 0;
+    for (int i = 0; i < nt; ++i) {
+        if (out->fluence && !(e.flags & NMC_FLAG_NO_FLUENCE)) {
+            acc[i].F = i == 0 ? out->fluence : (double*)calloc((size_t)nv, sizeof(double));
+            fail |= acc[i].F == NULL;
+        }
+        if (out->ryx) { acc[i].ryx = i == 0 ? out->ryx : (double*)calloc((size_t)n2d, sizeof(double)); fail |= acc[i].ryx == NULL; }
+        if (out->det2d) { acc[i].det = i == 0 ? out->det2d : (uint32_t*)calloc((size_t)n2d, sizeof(uint32_t)); fail |= acc[i].det == NULL; }
+    }
+    double t0 = now_s();
+    if (!fail) {
+#pragma omp parallel for num_threads(nt) schedule(static)
+        for (int i = 0; i < nt; ++i) {
+            uint64_t lo = first + n * (uint64_t)i / (uint64_t)nt;
+            uint64_t hi = first + n * (uint64_t)(i + 1) / (uint64_t)nt;
+            for (uint64_t p = lo; p < hi; ++p) xyz_photon(c, p, &acc[i]);
+        }
+    }
+    g_last_seconds = now_s() - t0;
+    for (int i = 0; i < nt; ++i) {
+        if (i > 0) {
+            if (acc[i].F) { for (int64_t j = 0; j < nv; ++j) out->fluence[j] += acc[i].F[j]; free(acc[i].F); }
+            if (acc[i].ryx) { for (int64_t j = 0; j < n2d; ++j) out->ryx[j] += acc[i].ryx[j]; free(acc[i].ryx); }
+            if (acc[i].det) { for (int64_t j = 0; j < n2d; ++j) out->det2d[j] += acc[i].det[j]; free(acc[i].det); }
+        }
+        totals_add(&out->totals, &acc[i].t);
+    }
+    free(acc);
+    free(c);
+    return fail ? NMC_ERR_NOMEM : NMC_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* mcsub (kernels.metal:59-319).                                             */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    float mua, mus, g, n1, n2, mut, one_m_albedo, rsp0;
+    int mcflag, bmode, nr, nz, na;
+    float dr, dz, radius, waist, zfocus, xs, ys, zs;
+    float thr, chance;
+    uint32_t max_events;
+    uint32_t key[2];
+    float cos_tab[64]; /* cos of the upper edge of exit-angle bin j */
+} sub_ctx;
+
+typedef struct { double* rd_ra; double* a_rz; nmc_totals t; } sub_acc;
+
+static inline int rbin(float q, int n) { return q >= (float)n ? n : (int)q; }
+
+static void sub_photon(const sub_ctx* c, uint64_t pid, sub_acc* a)
+{
+    nmc_totals* t = &a->t;
+    uint32_t b[4];
+    float x, y, z, ux, uy, uz, rsp;
+    t->photons++;
+    block(c->key, pid, 0u, 0u, b);
+    if (c->mcflag == 0) {
+        /* uniform collimated beam at the surface (kernels.metal:106-120) */
+        x = c->radius * sqrtf(u01(b[0])); y = 0.0f; z = 0.0f;
+        ux = 0.0f; uy = 0.0f; uz = 1.0f;
+        rsp = c->rsp0;
+    } else if (c->mcflag == 1) {
+        /* Gaussian beam (kernels.metal:121-145) */
+        x = c->radius * sqrtf(-nmco_logf(u01nz(b[0]))); y = 0.0f; z = 0.0f;
+        float xf = c->waist * sqrtf(-nmco_logf(u01nz(b[1])));
+        float tmp = sqrtf((x - xf) * (x - xf) + c->zfocus * c->zfocus);
+        float st = -(x - xf) / tmp, ct = c->zfocus / tmp;
+        float uzn;
+        rsp = nmco_fresnel(c->n2, c->n1, ct, &uzn);
+        (void)st;
+        uz = uzn;
+        ux = sqrtf(1.0f - uz * uz);
+        uy = 0.0f;
+    } else {
+        /* isotropic point source (kernels.metal:146-160) */
+        float ct = 1.0f - 2.0f * u01(b[0]);
+        float st = sqrtf(1.0f - ct * ct);
+        float sp, cp;
+        nmco_sincos2pi(u01(b[1]), &sp, &cp);
+        x = c->xs; y = c->ys; z = c->zs;
+        ux = st * cp; uy = st * sp; uz = ct;
+        rsp = 0.0f;
+    }
+    float W = 1.0f - rsp;
+    t->specular += (double)rsp;
+
+    uint32_t ev = 1, events = 0;
+    for (;;) {
+        block(c->key, pid, ev, 0u, b);
+        /* HOP (kernels.metal:183-187) */
+        float s = -nmco_logf(u01nz(b[0])) / c->mut;
+        float x0 = x, y0 = y, z0 = z;
+        x = fmaf(s, ux, x);
+        y = fmaf(s, uy, y);
+        z = fmaf(s, uz, z);
+        events++;
+        /* ESCAPE at z <= 0 (kernels.metal:190-214) */
+        if (c->bmode == 1 && z <= 0.0f) {
+            uint32_t aux[4];
+            block(c->key, pid, ev, 1u, aux);
+            t->surface_hits++;
+            float uz1;
+            float rf = nmco_fresnel(c->n1, c->n2, -uz, &uz1);
+            if (u01(aux[0]) > rf) {
+                float s2 = z0 / -uz;
+                x = fmaf(s2, ux, x0);
+                y = fmaf(s2, uy, y0);
+                float r = sqrtf(x * x + y * y);
+                int ir = rbin(r / c->dr, c->nr);
+                int ia = 0;
+                while (ia < c->na - 1 && uz1 < c->cos_tab[ia]) ia++;
+                if (a->rd_ra) a->rd_ra[(int64_t)ir * c->na + ia] += (double)W;
+                t->rd += (double)W;
+                t->detections++;
+                t->substeps_cross++;
+                return;
+            }
+            z = -z;
+            uz = -uz;
+        }
+        t->substeps_same++;
+        /* DROP (kernels.metal:224-234) */
+        float absorb = W * c->one_m_albedo;
+        W = W - absorb;
+        float r = sqrtf(x * x + y * y);
+        int ir = rbin(r / c->dr, c->nr);
+        int iz = rbin(fabsf(z) / c->dz, c->nz);
+        if (a->a_rz) a->a_rz[(int64_t)ir * (c->nz + 1) + iz] += (double)absorb;
+        t->absorbed += (double)absorb;
+        /* SPIN (kernels.metal:243-279, with g honoured) */
+        float ct = nmco_hg_cos(c->g, u01(b[1]));
+        nmco_spin(ct, u01(b[2]), &ux, &uy, &uz);
+        t->interactions++;
+        if (!roulette(&W, u01(b[3]), c->thr, c->chance, t)) return;
+        if (events >= c->max_events) {
+            t->capped += (double)W;
+            t->capped_photons++;
+            return;
+        }
+        ev++;
+    }
+}
+
+int nmco_mcsub(const RunParams* rp, const TissueParams* tp, const nmc_ext_config* ext,
+               uint64_t first, uint64_t n, int n_threads, nmc_results* out)
+{
+    nmc_ext_config e;
+    ext_defaults(ext, &e);
+    if (!rp || !tp || e.nr <= 0 || e.nz <= 0 || e.na <= 0 || e.na > 64 || !(e.dr > 0) || !(e.dz > 0))
+        return NMC_ERR_ARG;
+    sub_ctx c;
+    memset(&c, 0, sizeof(c));
+    c.mua = tp->muav[1]; c.mus = tp->musv[1]; c.g = tp->gv[1];
+    c.n1 = rp->n1; c.n2 = rp->n2;
+    if (!(c.mua >= 0) || !(c.mus >= 0) || !(c.mua + c.mus > 0)) return NMC_ERR_ARG;
+    c.mut = c.mua + c.mus;
+    c.one_m_albedo = 1.0f - c.mus / c.mut;
+    float tr = c.n1 / c.n2;
+    tr = (1.0f - tr) / (1.0f + tr);
+    c.rsp0 = tr * tr;
+    c.mcflag = rp->mcflag; c.bmode = rp->boundaryflag;
+    c.nr = e.nr; c.nz = e.nz; c.na = e.na; c.dr = e.dr; c.dz = e.dz;
+    c.radius = rp->radius; c.waist = rp->waist; c.zfocus = rp->zfocus;
+    c.xs = rp->xs; c.ys = rp->ys; c.zs = rp->zs;
+    c.thr = e.threshold; c.chance = e.chance; c.max_events = e.max_events;
+    c.key[0] = (uint32_t)e.seed;
+    c.key[1] = (uint32_t)(e.seed >> 32) + e.stream_id;
+    for (int j = 0; j < c.na; ++j)
+        c.cos_tab[j] = (float)cos((double)(j + 1) * (3.14159265358979323846 / 2.0) / (double)c.na);
+
+    int64_t nra = (int64_t)(c.nr + 1) * c.na, nrz = (int64_t)(c.nr + 1) * (c.nz + 1);
+    int nt = n_threads > 0 ? n_threads : nmco_max_threads();
+    if ((uint64_t)nt > n) nt = n > 0 ? (int)n : 1;
+    sub_acc* acc = (sub_acc*)calloc((size_t)nt, sizeof(sub_acc));
+    int fail = 0;
+    for (int i = 0; i < nt; ++i) {
+        if (out->rd_ra) { acc[i].rd_ra = i == 0 ? out->rd_ra : (double*)calloc((size_t)nra, sizeof(double)); fail |= !acc[i].rd_ra; }
+        if (out->a_rz && !(e.flags & NMC_FLAG_NO_FLUENCE)) { acc[i].a_rz = i == 0 ? out->a_rz : (double*)calloc((size_t)nrz, sizeof(double)); fail |= !acc[i].a_rz; }
+    }
+    double t0 = now_s();
+    if (!fail) {
+#pragma omp parallel for num_threads(nt) schedule(static)
+        for (int i = 0; i < nt; ++i) {
+            uint64_t lo = first + n * (uint64_t)i / (uint64_t)nt;
+            uint64_t hi = first + n * (uint64_t)(i + 1) / (uint64_t)nt;
+            for (uint64_t p = lo; p < hi; ++p) sub_photon(&c, p, &acc[i]);
+        }
+    }
+    g_last_seconds = now_s() - t0;
+    for (int i = 0; i < nt; ++i) {
+        if (i > 0) {
+            if (acc[i].rd_ra) { for (int64_t j = 0; j < nra; ++j) out->rd_ra[j] += acc[i].rd_ra[j]; free(acc[i].rd_ra); }
+            if (acc[i].a_rz) { for (int64_t j = 0; j < nrz; ++j) out->a_rz[j] += acc[i].a_rz[j]; free(acc[i].a_rz); }
+        }
+        totals_add(&out->totals, &acc[i].t);
+    }
+    free(acc);
+    return fail ? NMC_ERR_NOMEM : NMC_OK;
+}

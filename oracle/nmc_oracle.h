@@ -1,0 +1,55 @@
+/*
+ * nmc_oracle.h - CPU oracle for the photon-transport hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the checker: tests/, the smoke entry
+ * point and the CPU-baseline leg of the benchmark may load it; the product
+ * library (nmc_amd/) never does.  See nmc_oracle.c for the algorithm and the
+ * reference lines each function restates.
+ */
+#ifndef NMC_ORACLE_H
+#define NMC_ORACLE_H
+
+#include <stdint.h>
+#include "../include/nmc.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* primitives (exported for the unit tests) */
+void nmco_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+float nmco_logf(float x);
+float nmco_expf(float x);
+float nmco_expm1f(float x);
+void nmco_sincos2pi(float u, float* s, float* c);
+float nmco_fresnel(float n1, float n2, float ca1, float* ca2);
+float nmco_hg_cos(float g, float u);
+float nmco_tt_cos(float gf, float af, float gb, float ab, float cc, float u);
+void nmco_spin(float cost, float u_psi, float* ux, float* uy, float* uz);
+
+/* vectorised helpers for the tests (n elements each) */
+void nmco_logf_n(const float* x, float* y, int64_t n);
+void nmco_expf_n(const float* x, float* y, int64_t n);
+void nmco_expm1f_n(const float* x, float* y, int64_t n);
+void nmco_sincos2pi_n(const float* u, float* s, float* c, int64_t n);
+
+/*
+ * Transport photons [first, first + n) of the configuration and accumulate
+ * into `out` (arrays as documented for nmc_results in include/nmc.h; arrays
+ * must be zeroed by the caller before the first call).  n_threads <= 0 uses
+ * OpenMP's default.  Returns NMC_OK or NMC_ERR_ARG.
+ */
+int nmco_mcxyz(const RunParams* rp, const TissueParams* tp, const uint8_t* v,
+               const nmc_ext_config* ext, uint64_t first, uint64_t n, int n_threads,
+               nmc_results* out);
+int nmco_mcsub(const RunParams* rp, const TissueParams* tp, const nmc_ext_config* ext,
+               uint64_t first, uint64_t n, int n_threads, nmc_results* out);
+
+/* Seconds of the last nmco_mcxyz / nmco_mcsub transport loop (CLOCK_MONOTONIC). */
+double nmco_last_seconds(void);
+int nmco_max_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
