@@ -193,6 +193,56 @@ int nmc_reset_tallies(nmc_handle* h);
 
 void nmc_destroy(nmc_handle* h);
 
+/*
+ * Launch tuning (no reference counterpart: NMC dispatches a fixed 10 000-thread
+ * grid, MetalMC.m:483-495).  Zero fields mean "library default".
+ */
+typedef struct nmc_tuning {
+    int32_t block;          /* threads per workgroup: 256, 512 or 1024          */
+    int32_t blocks_per_cu;  /* workgroups per CU (<= occupancy limit)           */
+    int32_t refill_min;     /* idle lanes per wave before a refill (1..64)      */
+    int32_t chunk;          /* photon ids a wave takes from the queue at once   */
+    int32_t tile[6];        /* mcxyz LDS fluence tile x0,y0,z0,nx,ny,nz;
+                               nx == 0: automatic box, nx < 0: no tile          */
+    int32_t reserved[6];
+} nmc_tuning;
+
+int nmc_set_tuning(nmc_handle* h, const nmc_tuning* t);
+/* Effective values after defaults and clamping (tile resolved to a box). */
+int nmc_get_tuning(nmc_handle* h, nmc_tuning* t);
+
+/*
+ * Batched mcsub: many media (e.g. wavelengths), each with its own optical
+ * properties, Philox stream and photon budget, transported by ONE persistent
+ * launch from a device photon queue.  No counterpart in the reference, whose
+ * notebooks run one `nmc` process per configuration (example3.ipynb:862-902).
+ * Item j's photons are those of a plain run with the same medium and
+ * ext->stream_id = items[j].stream_id.
+ */
+typedef struct nmc_work_item {
+    float mua, mus, g;      /* medium (tissue type 1 of a plain mcsub run)     */
+    float n1, n2;           /* medium / outside refractive index               */
+    uint32_t stream_id;
+    uint64_t n_photons;     /* budget per nmc_run_batch                        */
+} nmc_work_item;
+
+/* Beam and bins from rp/ext (as nmc_prepare on an NMC_SIM_MCSUB handle);
+ * per-item tallies are (nr+1)*na Rd(r,alpha) bins plus, unless
+ * NMC_FLAG_NO_FLUENCE, (nr+1)*(nz+1) A(r,z) bins. */
+int nmc_prepare_batch(nmc_handle* h, const RunParams* rp, const nmc_ext_config* ext,
+                      const nmc_work_item* items, int n_items);
+/* Transports photons [first, first + n_photons_j) of every item j in one
+ * launch; first then advances by max_j n_photons_j. */
+int nmc_run_batch(nmc_handle* h);
+int nmc_run_batch_async(nmc_handle* h);
+/* tally: n_items x per-item bins (NULL to skip); totals: n_items entries. */
+int nmc_get_batch_results(nmc_handle* h, double* tally, nmc_totals* totals);
+
+/* enable != 0: the launch sequence of nmc_run / nmc_run_batch (and their
+ * async forms) is captured once as a hipGraph and replayed on later calls
+ * with the same photon count and seed. */
+int nmc_set_graph_mode(nmc_handle* h, int enable);
+
 /* Message of the last failing call on this host thread ("" if none). */
 const char* nmc_last_error(void);
 

@@ -1,12 +1,22 @@
 // mcsub.hip - semi-infinite homogeneous medium with cylindrical tallies
-// (reference MCSubKernel, NMC/NMCCode/kernels.metal:59-319).
+// (reference MCSubKernel, NMC/NMCCode/kernels.metal:59-319), plain and
+// batched.
+//
+// One kernel serves both: a launch transports the photons of `n_items` work
+// items (media with their own optical properties, Philox stream and photon
+// budget) from one global photon queue.  Queue index q in [0, sum budgets)
+// maps to item j by binary search over the budget prefix sums and to photon id
+// first + (q - prefix[j]) within that item; a plain run is a batch of one.
+// Since an item's photons depend only on (seed, item stream, photon id), item j
+// of a batch reproduces, photon for photon, a plain run of that medium.
 //
 // Every loop iteration is one event of every live lane: hop, surface test
 // (Fresnel all-or-none escape / mirror reflection), drop of W(1-albedo) into
-// A(r,z), HG spin and roulette; lanes are refilled from a wave-private pool
-// of photon ids as in mcxyz.hip.  Rd(r,alpha) and A(r,z) are privatised per
-// workgroup in LDS as FP64 and written out once as block partials, which a
-// second kernel sums over blocks in a fixed order into the FP64 tallies.
+// A(r,z), HG spin and roulette.  Lanes are refilled from a wave-private pool of
+// queue indices as in mcxyz.hip.  Per-item Rd(r,alpha), A(r,z) and totals are
+// privatised per workgroup in LDS (FP64) when they fit, written out once as
+// block partials and summed over blocks in a fixed order by a second kernel;
+// otherwise they are accumulated with FP64 global atomics.
 #include "nmc_internal.h"
 
 namespace nmc {
@@ -24,52 +34,71 @@ __device__ __forceinline__ void addi(nmc_totals* t, int k, unsigned long long v)
 
 template <bool LDS>
 __global__ __launch_bounds__(kSubBlock) void mcsub_kernel(SubArgs a) {
-  extern __shared__ double tal[];  // [(nr+1)*na] Rd(r,a) then [(nr+1)*(nz+1)] A(r,z)
-  __shared__ nmc_totals st;
+  // dynamic LDS: [items] media | [items+1] prefix | [items] totals | tallies
+  extern __shared__ double smem[];
   __shared__ float cos_tab[64];
+  const int ni = a.n_items;
   const int n_ra = (a.nr + 1) * a.na;
   const int n_rz = a.tally_rz ? (a.nr + 1) * (a.nz + 1) : 0;
-  double* t_ra = LDS ? tal : a.rd_ra;
-  double* t_rz = LDS ? tal + n_ra : a.a_rz;
-  if (LDS)
-    for (int i = threadIdx.x; i < n_ra + n_rz; i += blockDim.x) tal[i] = 0.0;
+  const int per_item = n_ra + n_rz;
+  SubMedium* med = (SubMedium*)smem;
+  uint64_t* prefix = (uint64_t*)(med + ni);
+  nmc_totals* st = (nmc_totals*)(prefix + ni + 1);
+  double* tal = (double*)(st + ni);
+  double* tally = LDS ? tal : a.tally;
+  nmc_totals* tot = LDS ? st : a.tot;
+  for (int i = threadIdx.x; i < ni; i += blockDim.x) med[i] = a.items[i];
+  for (int i = threadIdx.x; i <= ni; i += blockDim.x) prefix[i] = a.prefix[i];
   for (int i = threadIdx.x; i < a.na - 1; i += blockDim.x) cos_tab[i] = a.cos_tab[i];
-  if (threadIdx.x < TF_COUNT) ((double*)&st)[threadIdx.x] = 0.0;
-  if (threadIdx.x < TI_COUNT) (&st.photons)[threadIdx.x] = 0ull;
+  if (LDS) {
+    for (int i = threadIdx.x; i < ni * (TF_COUNT + TI_COUNT); i += blockDim.x) ((double*)st)[i] = 0.0;
+    for (int i = threadIdx.x; i < ni * per_item; i += blockDim.x) tal[i] = 0.0;
+  }
   __syncthreads();
+  const uint64_t first = a.state->first;
+  const uint64_t total = prefix[ni];
 
   const int lane = lane_id();
   float x = 0.0f, y = 0.0f, z = 0.0f, ux = 0.0f, uy = 0.0f, uz = 1.0f, W = 0.0f;
+  SubMedium m{};
+  int item = 0;
   uint64_t pid = 0;
-  uint32_t ev = 0, events = 0;
+  uint32_t ev = 0, events = 0, c_int = 0;
   bool alive = false, finished = false;
-  uint64_t pool_next = 0, pool_end = 0;
-  double absorbed = 0.0, specular = 0.0;
-  uint32_t c_int = 0, c_phot = 0;
+  uint64_t pool_next = 0, pool_end = 0;  // wave-uniform
+  double absorbed = 0.0;
 
   for (;;) {
     uint64_t need = ballot(!alive && !finished);
     while (need) {
       if (pool_next >= pool_end) {
         unsigned long long base = 0;
-        if (lane == 0) base = atomicAdd(a.counter, (unsigned long long)a.chunk);
+        if (lane == 0) base = atomicAdd(&a.state->counter, (unsigned long long)a.chunk);
         base = __shfl(base, 0);
-        if (base >= a.n) {
+        if (base >= total) {
           finished = finished || !alive;
           break;
         }
         pool_next = base;
-        pool_end = base + a.chunk < a.n ? base + a.chunk : a.n;
+        pool_end = base + a.chunk < total ? base + a.chunk : total;
       }
       uint64_t avail = pool_end - pool_next;
       uint64_t cnt = (uint64_t)__popcll(need);
       uint32_t take = (uint32_t)(cnt < avail ? cnt : avail);
       uint32_t rank = (uint32_t)__popcll(need & lanemask_lt());
       if (((need >> lane) & 1ull) && rank < take) {
-        pid = a.first + pool_next + rank;
-        c_phot++;
+        uint64_t q = pool_next + rank;
+        int lo = 0, hi = ni - 1;  // largest j with prefix[j] <= q
+        while (lo < hi) {
+          int mid = (lo + hi + 1) >> 1;
+          if (prefix[mid] <= q) lo = mid;
+          else hi = mid - 1;
+        }
+        item = lo;
+        m = med[item];
+        pid = first + (q - prefix[item]);
         // LAUNCH (kernels.metal:106-166)
-        U4 b = philox(pid, 0u, 0u, a.key0, a.key1);
+        U4 b = philox(pid, 0u, 0u, a.key0, m.key1);
         float rsp;
         if (a.mcflag == 0) {
           x = a.radius * sqrtf(u01(b.x));
@@ -78,7 +107,7 @@ __global__ __launch_bounds__(kSubBlock) void mcsub_kernel(SubArgs a) {
           ux = 0.0f;
           uy = 0.0f;
           uz = 1.0f;
-          rsp = a.rsp0;
+          rsp = m.rsp0;
         } else if (a.mcflag == 1) {
           x = a.radius * sqrtf(-log_pos(u01nz(b.x)));
           y = 0.0f;
@@ -87,7 +116,7 @@ __global__ __launch_bounds__(kSubBlock) void mcsub_kernel(SubArgs a) {
           float tmp = sqrtf((x - xf) * (x - xf) + a.zfocus * a.zfocus);
           float ct = a.zfocus / tmp;
           float uzn;
-          rsp = fresnel(a.n2, a.n1, ct, &uzn);
+          rsp = fresnel(m.n2, m.n1, ct, &uzn);
           uz = uzn;
           ux = sqrtf(1.0f - uz * uz);
           uy = 0.0f;
@@ -105,9 +134,12 @@ __global__ __launch_bounds__(kSubBlock) void mcsub_kernel(SubArgs a) {
           rsp = 0.0f;
         }
         W = 1.0f - rsp;
-        specular += (double)rsp;
+        addf(&tot[item], TF_SPECULAR, (double)rsp);
+        addi(&tot[item], TI_PHOTONS, 1);
         ev = 1;
         events = 0;
+        c_int = 0;
+        absorbed = 0.0;
         alive = true;
       }
       pool_next += take;
@@ -116,21 +148,21 @@ __global__ __launch_bounds__(kSubBlock) void mcsub_kernel(SubArgs a) {
     if (ballot(alive) == 0ull) break;
     if (!alive) continue;
 
-    U4 b = philox(pid, ev, 0u, a.key0, a.key1);
+    U4 b = philox(pid, ev, 0u, a.key0, m.key1);
     // HOP (kernels.metal:183-187)
-    float s = -log_pos(u01nz(b.x)) / a.mut;
+    float s = -log_pos(u01nz(b.x)) / m.mut;
     float x0 = x, y0 = y, z0 = z;
     x = fmaf(s, ux, x);
     y = fmaf(s, uy, y);
     z = fmaf(s, uz, z);
     events++;
-    bool escaped = false;
+    bool ended = false;
     // ESCAPE at z <= 0 (kernels.metal:190-214)
     if (a.bmode == 1 && z <= 0.0f) {
-      U4 aux = philox(pid, ev, 1u, a.key0, a.key1);
-      addi(&st, TI_SURFACE_HITS, 1);
+      U4 aux = philox(pid, ev, 1u, a.key0, m.key1);
+      addi(&tot[item], TI_SURFACE_HITS, 1);
       float uz1;
-      float rf = fresnel(a.n1, a.n2, -uz, &uz1);
+      float rf = fresnel(m.n1, m.n2, -uz, &uz1);
       if (u01(aux.x) > rf) {
         float s2 = z0 / -uz;
         float xe = fmaf(s2, ux, x0);
@@ -139,106 +171,139 @@ __global__ __launch_bounds__(kSubBlock) void mcsub_kernel(SubArgs a) {
         int ir = rbin(r / a.dr, a.nr);
         int ia = 0;
         while (ia < a.na - 1 && uz1 < cos_tab[ia]) ia++;
-        unsafeAtomicAdd(&t_ra[ir * a.na + ia], (double)W);
-        addf(&st, TF_RD, (double)W);
-        addi(&st, TI_DETECTIONS, 1);
-        addi(&st, TI_SUB_CROSS, 1);
-        escaped = true;
-        alive = false;
+        unsafeAtomicAdd(&tally[item * per_item + ir * a.na + ia], (double)W);
+        addf(&tot[item], TF_RD, (double)W);
+        addi(&tot[item], TI_DETECTIONS, 1);
+        addi(&tot[item], TI_SUB_CROSS, 1);
+        ended = true;
       } else {
         z = -z;
         uz = -uz;
       }
     }
-    if (escaped) continue;
-    // DROP (kernels.metal:224-234)
-    float absorb = W * a.one_m_albedo;
-    W = W - absorb;
-    if (a.tally_rz) {
-      float r = sqrtf(x * x + y * y);
-      int ir = rbin(r / a.dr, a.nr);
-      int iz = rbin(fabsf(z) / a.dz, a.nz);
-      unsafeAtomicAdd(&t_rz[ir * (a.nz + 1) + iz], (double)absorb);
-    }
-    absorbed += (double)absorb;
-    // SPIN (kernels.metal:243-279, HG honoured)
-    float ct = hg_cos(a.g, u01(b.y));
-    spin(ct, u01(b.z), ux, uy, uz);
-    c_int++;
-    // ROULETTE (kernels.metal:289-294)
-    if (W < a.thr) {
-      if (u01(b.w) <= a.chance) {
-        float nw = W / a.chance;
-        addf(&st, TF_ROULETTE_GAIN, (double)nw - (double)W);
-        addi(&st, TI_ROULETTE_SURVIVALS, 1);
-        W = nw;
-      } else {
-        addf(&st, TF_ROULETTE_LOSS, (double)W);
-        addi(&st, TI_ROULETTE_KILLS, 1);
-        alive = false;
-        continue;
+    if (!ended) {
+      // DROP (kernels.metal:224-234)
+      float absorb = W * m.one_m_albedo;
+      W = W - absorb;
+      if (a.tally_rz) {
+        float r = sqrtf(x * x + y * y);
+        int ir = rbin(r / a.dr, a.nr);
+        int iz = rbin(fabsf(z) / a.dz, a.nz);
+        unsafeAtomicAdd(&tally[item * per_item + n_ra + ir * (a.nz + 1) + iz], (double)absorb);
+      }
+      absorbed += (double)absorb;
+      // SPIN (kernels.metal:243-279, HG honoured)
+      float ct = hg_cos(m.g, u01(b.y));
+      spin(ct, u01(b.z), ux, uy, uz);
+      c_int++;
+      // ROULETTE (kernels.metal:289-294)
+      if (W < a.thr) {
+        if (u01(b.w) <= a.chance) {
+          float nw = W / a.chance;
+          addf(&tot[item], TF_ROULETTE_GAIN, (double)nw - (double)W);
+          addi(&tot[item], TI_ROULETTE_SURVIVALS, 1);
+          W = nw;
+        } else {
+          addf(&tot[item], TF_ROULETTE_LOSS, (double)W);
+          addi(&tot[item], TI_ROULETTE_KILLS, 1);
+          ended = true;
+        }
+      }
+      if (!ended && events >= a.max_events) {
+        addf(&tot[item], TF_CAPPED, (double)W);
+        addi(&tot[item], TI_CAPPED, 1);
+        ended = true;
       }
     }
-    if (events >= a.max_events) {
-      addf(&st, TF_CAPPED, (double)W);
-      addi(&st, TI_CAPPED, 1);
+    if (ended) {
+      addf(&tot[item], TF_ABSORBED, absorbed);
+      addi(&tot[item], TI_SUB_SAME, c_int);
+      addi(&tot[item], TI_INTERACTIONS, c_int);
       alive = false;
-      continue;
+    } else {
+      ev++;
     }
-    ev++;
   }
 
-  addf(&st, TF_ABSORBED, absorbed);
-  addf(&st, TF_SPECULAR, specular);
-  addi(&st, TI_SUB_SAME, c_int);
-  addi(&st, TI_INTERACTIONS, c_int);
-  addi(&st, TI_PHOTONS, c_phot);
   __syncthreads();
   if (LDS) {
-    double* part = a.part + (int64_t)blockIdx.x * (n_ra + n_rz);
-    for (int i = threadIdx.x; i < n_ra + n_rz; i += blockDim.x) part[i] = tal[i];
+    double* part = a.part + (int64_t)blockIdx.x * ni * (per_item + TF_COUNT + TI_COUNT);
+    for (int i = threadIdx.x; i < ni * per_item; i += blockDim.x) part[i] = tal[i];
+    double* pt = part + ni * per_item;
+    for (int i = threadIdx.x; i < ni * (TF_COUNT + TI_COUNT); i += blockDim.x) pt[i] = ((double*)st)[i];
   }
-  if (threadIdx.x < TF_COUNT) unsafeAtomicAdd(&((double*)a.tot)[threadIdx.x], ((double*)&st)[threadIdx.x]);
-  if (threadIdx.x < TI_COUNT)
-    atomicAdd(&(&a.tot->photons)[threadIdx.x], (unsigned long long)(&st.photons)[threadIdx.x]);
 }
 
-// out[j] += sum_b part[b * stride + j], b in ascending order (deterministic).
+// out[j] += sum_b part[b * stride + j] for j < len, blocks in ascending order
+// (deterministic).  Integer totals are stored as u64 bit patterns in the
+// partials and are summed as integers.
 __global__ void reduce_partials_kernel(const double* __restrict__ part, int grid, int64_t stride,
-                                       int64_t len, double* __restrict__ out) {
+                                       int64_t len, double* __restrict__ out, int ni, int per_item) {
   int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= len) return;
-  double s = 0.0;
-  for (int b = 0; b < grid; ++b) s += part[(int64_t)b * stride + j];
-  out[j] += s;
+  int64_t tal_len = (int64_t)ni * per_item;
+  bool is_int = false;
+  if (j >= tal_len) {
+    int k = (int)((j - tal_len) % (TF_COUNT + TI_COUNT));
+    is_int = k >= TF_COUNT;
+  }
+  if (is_int) {
+    unsigned long long s = 0;
+    for (int b = 0; b < grid; ++b) s += __double_as_longlong(part[(int64_t)b * stride + j]);
+    unsigned long long* o = (unsigned long long*)out;
+    o[j] += s;
+  } else {
+    double s = 0.0;
+    for (int b = 0; b < grid; ++b) s += part[(int64_t)b * stride + j];
+    out[j] += s;
+  }
 }
+
+__global__ void state_begin_kernel(RunState* st) { st->counter = 0ull; }
+__global__ void state_advance_kernel(RunState* st, unsigned long long adv) { st->first += adv; }
 
 }  // namespace
 
-hipError_t launch_mcsub(const SubArgs& a, int grid, int lds_bytes, hipStream_t s) {
-  if (a.lds_tallies) {
-    hipFuncSetAttribute((const void*)mcsub_kernel<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                        lds_bytes);
-    hipLaunchKernelGGL(mcsub_kernel<true>, dim3(grid), dim3(kSubBlock), lds_bytes, s, a);
-  } else {
-    hipLaunchKernelGGL(mcsub_kernel<false>, dim3(grid), dim3(kSubBlock), 0, s, a);
-  }
+size_t mcsub_lds_bytes(int n_items, int64_t per_item) {
+  return (size_t)n_items * sizeof(SubMedium) + (size_t)(n_items + 1) * sizeof(uint64_t) +
+         (size_t)n_items * sizeof(nmc_totals) + (size_t)n_items * per_item * sizeof(double);
+}
+
+hipError_t launch_mcsub(const SubArgs& a, int grid, size_t lds_bytes, bool lds_tallies,
+                        hipStream_t s) {
+  if (lds_tallies) hipLaunchKernelGGL(mcsub_kernel<true>, dim3(grid), dim3(kSubBlock), lds_bytes, s, a);
+  else hipLaunchKernelGGL(mcsub_kernel<false>, dim3(grid), dim3(kSubBlock), lds_bytes, s, a);
   return hipGetLastError();
 }
 
-int mcsub_blocks_per_cu(int lds_bytes) {
+hipError_t mcsub_configure(size_t lds_bytes, bool lds_tallies) {
+  const void* f = lds_tallies ? (const void*)mcsub_kernel<true> : (const void*)mcsub_kernel<false>;
+  return hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+}
+
+int mcsub_blocks_per_cu(size_t lds_bytes, bool lds_tallies) {
   int nb = 0;
-  hipError_t e = lds_bytes > 0
+  hipError_t e = lds_tallies
       ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, mcsub_kernel<true>, kSubBlock, lds_bytes)
-      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, mcsub_kernel<false>, kSubBlock, 0);
+      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, mcsub_kernel<false>, kSubBlock, lds_bytes);
   return (e == hipSuccess && nb > 0) ? nb : 1;
 }
 
-hipError_t launch_reduce_partials(const double* part, int grid, int64_t stride, int64_t len,
-                                  double* out, hipStream_t s) {
-  if (len <= 0) return hipSuccess;
-  hipLaunchKernelGGL(reduce_partials_kernel, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s,
-                     part, grid, stride, len, out);
+hipError_t launch_reduce_partials(const double* part, int grid, int64_t stride, double* out,
+                                  int ni, int per_item, hipStream_t s) {
+  if (stride <= 0) return hipSuccess;
+  hipLaunchKernelGGL(reduce_partials_kernel, dim3((unsigned)((stride + 255) / 256)), dim3(256), 0,
+                     s, part, grid, stride, stride, out, ni, per_item);
+  return hipGetLastError();
+}
+
+hipError_t launch_state_begin(RunState* st, hipStream_t s) {
+  hipLaunchKernelGGL(state_begin_kernel, dim3(1), dim3(1), 0, s, st);
+  return hipGetLastError();
+}
+
+hipError_t launch_state_advance(RunState* st, uint64_t adv, hipStream_t s) {
+  hipLaunchKernelGGL(state_advance_kernel, dim3(1), dim3(1), 0, s, st, (unsigned long long)adv);
   return hipGetLastError();
 }
 

@@ -4,17 +4,20 @@
 //
 // Launch model: persistent wave64 workgroups.  Each lane carries one photon;
 // the loop body advances every live lane by one piece of its current step
-// (to the next interaction, voxel face or detector plane) and, for lanes
-// whose step is used up, performs the interaction (spin + roulette + next
-// step length).  Lanes whose photon ended are refilled at the top of the
-// loop from a wave-private pool of photon ids that is replenished, `chunk`
-// ids at a time, from one global counter.  A photon's random numbers depend
-// only on its id, never on the lane or the order of execution.
+// (to the next interaction, voxel face or detector plane) and, for lanes whose
+// step is used up, performs the interaction (spin + roulette + next step
+// length).  A wave refills its idle lanes only once at least `refill_min` of
+// them are idle (or none is live), so the launch code runs for many lanes at
+// once; refills draw from a wave-private pool of photon ids replenished
+// `chunk` ids at a time from one device counter.  A photon's random numbers
+// depend only on its id, never on the lane or the order of execution.
 //
-// Tallies: fluence is summed per voxel visit in a register (double) and added
-// with one FP64 global atomic when the photon leaves the voxel; rare events
-// (detections, escapes, roulette) go to LDS totals; frequent counters stay in
-// registers until the workgroup ends.
+// Fluence: deposits are summed per voxel visit in a register (FP64) and added
+// when the photon leaves the voxel - into the workgroup's LDS tile (FP64) if
+// the voxel lies inside the tile box, else with an FP64 global atomic.  The
+// tiles are written out as block partials and summed over blocks in a fixed
+// order.  Rare events (detections, escapes, roulette) go to LDS totals;
+// frequent counters stay in registers until the workgroup ends.
 #include "nmc_internal.h"
 
 namespace nmc {
@@ -36,12 +39,24 @@ __device__ __forceinline__ int64_t vidx(const XyzArgs& a, int ix, int iy, int iz
   return (int64_t)iz * a.Ny * a.Nx + (int64_t)ix * a.Ny + iy;
 }
 
-__device__ __forceinline__ void lds_addf(double* p, double v) { unsafeAtomicAdd(p, v); }
 __device__ __forceinline__ void lds_addi(nmc_totals* t, int k, unsigned long long v) {
   atomicAdd(&((unsigned long long*)&t->photons)[k], v);
 }
 __device__ __forceinline__ void lds_addf(nmc_totals* t, int k, double v) {
   unsafeAtomicAdd(&((double*)t)[k], v);
+}
+
+// Adds the deposits of the current voxel visit to the tile or the global grid.
+__device__ __forceinline__ void flush_visit(Photon& p, const XyzArgs& a, double* tile) {
+  if (p.inside && a.F && p.vdep != 0.0) {
+    unsigned lx = (unsigned)(p.ix - a.tile_x0), ly = (unsigned)(p.iy - a.tile_y0),
+             lz = (unsigned)(p.iz - a.tile_z0);
+    if (lx < (unsigned)a.tile_nx && ly < (unsigned)a.tile_ny && lz < (unsigned)a.tile_nz)
+      unsafeAtomicAdd(&tile[((int)lz * a.tile_nx + (int)lx) * a.tile_ny + (int)ly], p.vdep);
+    else
+      unsafeAtomicAdd(&a.F[vidx(a, p.ix, p.iy, p.iz)], p.vdep);
+  }
+  p.vdep = 0.0;
 }
 
 // Escape through a grid face under boundaryflag 1 (or the top under 2).
@@ -134,14 +149,18 @@ __device__ __forceinline__ void launch(Photon& p, const XyzArgs& a, const XyzTis
   p.alive = true;
 }
 
-template <bool TWO_TERM>
-__global__ __launch_bounds__(kXyzBlock) void mcxyz_kernel(XyzArgs a) {
+template <bool TWO_TERM, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
+  extern __shared__ double tile[];  // [tile_nz][tile_nx][tile_ny]
   __shared__ XyzTissue T[NMC_NTISS];
   __shared__ nmc_totals st;
-  for (int i = threadIdx.x; i < NMC_NTISS; i += blockDim.x) T[i] = a.tissue[i];
+  const int tile_n = a.tile_nx * a.tile_ny * a.tile_nz;
+  for (int i = threadIdx.x; i < NMC_NTISS; i += BLOCK) T[i] = a.tissue[i];
+  for (int i = threadIdx.x; i < tile_n; i += BLOCK) tile[i] = 0.0;
   if (threadIdx.x < TF_COUNT) ((double*)&st)[threadIdx.x] = 0.0;
   if (threadIdx.x < TI_COUNT) (&st.photons)[threadIdx.x] = 0ull;
   __syncthreads();
+  const uint64_t first = a.state->first;
 
   const int lane = lane_id();
   Photon p;
@@ -151,37 +170,41 @@ __global__ __launch_bounds__(kXyzBlock) void mcxyz_kernel(XyzArgs a) {
   p.uz = 1.0f;
   p.W = 0.0f;
   p.vdep = 0.0;
+  p.ix = p.iy = p.iz = 0;
+  p.inside = false;
   bool finished = false;
   uint64_t pool_next = 0, pool_end = 0;  // wave-uniform
   double absorbed = 0.0, absorbed_out = 0.0;
   uint32_t c_same = 0, c_cross = 0, c_int = 0, c_phot = 0;
 
   for (;;) {
-    // ---- refill lanes whose photon ended ----
-    uint64_t need = ballot(!p.alive && !finished);
-    while (need) {
-      if (pool_next >= pool_end) {
-        unsigned long long base = 0;
-        if (lane == 0) base = atomicAdd(a.counter, (unsigned long long)a.chunk);
-        base = __shfl(base, 0);
-        if (base >= a.n) {
-          finished = finished || !p.alive;
-          break;
+    // ---- refill idle lanes, in batches of at least refill_min ----
+    uint64_t idle = ballot(!p.alive && !finished);
+    if (idle && ((uint32_t)__popcll(idle) >= a.refill_min || ballot(p.alive) == 0ull)) {
+      while (idle) {
+        if (pool_next >= pool_end) {
+          unsigned long long base = 0;
+          if (lane == 0) base = atomicAdd(&a.state->counter, (unsigned long long)a.chunk);
+          base = __shfl(base, 0);
+          if (base >= a.n) {
+            finished = finished || !p.alive;
+            break;
+          }
+          pool_next = base;
+          pool_end = base + a.chunk < a.n ? base + a.chunk : a.n;
         }
-        pool_next = base;
-        pool_end = base + a.chunk < a.n ? base + a.chunk : a.n;
+        uint64_t avail = pool_end - pool_next;
+        uint64_t cnt = (uint64_t)__popcll(idle);
+        uint32_t take = (uint32_t)(cnt < avail ? cnt : avail);
+        uint32_t rank = (uint32_t)__popcll(idle & lanemask_lt());
+        if (((idle >> lane) & 1ull) && rank < take) {
+          p.pid = first + pool_next + rank;
+          c_phot++;
+          launch(p, a, T, &st);
+        }
+        pool_next += take;
+        idle = ballot(!p.alive && !finished);
       }
-      uint64_t avail = pool_end - pool_next;
-      uint64_t cnt = (uint64_t)__popcll(need);
-      uint32_t take = (uint32_t)(cnt < avail ? cnt : avail);
-      uint32_t rank = (uint32_t)__popcll(need & lanemask_lt());
-      if (((need >> lane) & 1ull) && rank < take) {
-        p.pid = a.first + pool_next + rank;
-        c_phot++;
-        launch(p, a, T, &st);
-      }
-      pool_next += take;
-      need = ballot(!p.alive && !finished);
     }
     if (ballot(p.alive) == 0ull) break;
     if (!p.alive) continue;
@@ -275,8 +298,7 @@ __global__ __launch_bounds__(kXyzBlock) void mcxyz_kernel(XyzArgs a) {
         if (!(p.W > 0.0f)) alive = false;
       } else {
         // voxel face (kernels.metal:641-679)
-        if (p.inside && a.F && p.vdep != 0.0) unsafeAtomicAdd(&a.F[vidx(a, p.ix, p.iy, p.iz)], p.vdep);
-        p.vdep = 0.0;
+        flush_visit(p, a, tile);
         if (axis == 0) p.ix += p.ux > 0.0f ? 1 : -1;
         else if (axis == 1) p.iy += p.uy > 0.0f ? 1 : -1;
         else p.iz += p.uz > 0.0f ? 1 : -1;
@@ -327,13 +349,12 @@ __global__ __launch_bounds__(kXyzBlock) void mcxyz_kernel(XyzArgs a) {
       }
     }
     if (!alive) {
-      if (p.inside && a.F && p.vdep != 0.0) unsafeAtomicAdd(&a.F[vidx(a, p.ix, p.iy, p.iz)], p.vdep);
-      p.vdep = 0.0;
+      flush_visit(p, a, tile);
       p.alive = false;
     }
   }
 
-  // ---- workgroup totals -> global ----
+  // ---- workgroup totals and tile -> global ----
   lds_addf(&st, TF_ABSORBED, absorbed);
   lds_addf(&st, TF_ABSORBED_OUTSIDE, absorbed_out);
   lds_addi(&st, TI_SUB_SAME, c_same);
@@ -341,34 +362,85 @@ __global__ __launch_bounds__(kXyzBlock) void mcxyz_kernel(XyzArgs a) {
   lds_addi(&st, TI_INTERACTIONS, c_int);
   lds_addi(&st, TI_PHOTONS, c_phot);
   __syncthreads();
+  if (tile_n) {
+    double* part = a.tile_part + (int64_t)blockIdx.x * tile_n;
+    for (int i = threadIdx.x; i < tile_n; i += BLOCK) part[i] = tile[i];
+  }
   if (threadIdx.x < TF_COUNT) unsafeAtomicAdd(&((double*)a.tot)[threadIdx.x], ((double*)&st)[threadIdx.x]);
   if (threadIdx.x < TI_COUNT)
     atomicAdd(&(&a.tot->photons)[threadIdx.x], (unsigned long long)(&st.photons)[threadIdx.x]);
 }
 
-__global__ void zero_u64_kernel(unsigned long long* p, int n) {
+// F[tile voxel i] += sum_b part[b][i], blocks in ascending order.
+__global__ void tile_reduce_kernel(const double* __restrict__ part, int grid, XyzArgs a) {
+  int n = a.tile_nx * a.tile_ny * a.tile_nz;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0ull;
+  if (i >= n) return;
+  double s = 0.0;
+  for (int b = 0; b < grid; ++b) s += part[(int64_t)b * n + i];
+  if (s != 0.0) {
+    int ly = i % a.tile_ny, t = i / a.tile_ny;
+    int lx = t % a.tile_nx, lz = t / a.tile_nx;
+    a.F[vidx(a, a.tile_x0 + lx, a.tile_y0 + ly, a.tile_z0 + lz)] += s;
+  }
+}
+
+template <bool TT>
+hipError_t launch_block(const XyzArgs& a, int block, int grid, size_t lds, hipStream_t s) {
+  switch (block) {
+    case 256:
+      hipLaunchKernelGGL((mcxyz_kernel<TT, 256>), dim3(grid), dim3(256), lds, s, a);
+      break;
+    case 512:
+      hipLaunchKernelGGL((mcxyz_kernel<TT, 512>), dim3(grid), dim3(512), lds, s, a);
+      break;
+    default:
+      hipLaunchKernelGGL((mcxyz_kernel<TT, 1024>), dim3(grid), dim3(1024), lds, s, a);
+      break;
+  }
+  return hipGetLastError();
+}
+
+template <bool TT>
+hipError_t configure(int block, size_t lds) {
+  const void* f = block == 256   ? (const void*)mcxyz_kernel<TT, 256>
+                  : block == 512 ? (const void*)mcxyz_kernel<TT, 512>
+                                 : (const void*)mcxyz_kernel<TT, 1024>;
+  return hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+}
+
+template <bool TT>
+int occupancy(int block, size_t lds) {
+  int nb = 0;
+  hipError_t e;
+  switch (block) {
+    case 256: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, mcxyz_kernel<TT, 256>, 256, lds); break;
+    case 512: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, mcxyz_kernel<TT, 512>, 512, lds); break;
+    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, mcxyz_kernel<TT, 1024>, 1024, lds); break;
+  }
+  return (e == hipSuccess) ? nb : 0;
 }
 
 }  // namespace
 
-hipError_t launch_mcxyz(const XyzArgs& a, bool two_term, int grid, hipStream_t s) {
-  if (two_term) hipLaunchKernelGGL(mcxyz_kernel<true>, dim3(grid), dim3(kXyzBlock), 0, s, a);
-  else hipLaunchKernelGGL(mcxyz_kernel<false>, dim3(grid), dim3(kXyzBlock), 0, s, a);
-  return hipGetLastError();
+hipError_t launch_mcxyz(const XyzArgs& a, bool two_term, int block, int grid, size_t lds_bytes,
+                        hipStream_t s) {
+  return two_term ? launch_block<true>(a, block, grid, lds_bytes, s)
+                  : launch_block<false>(a, block, grid, lds_bytes, s);
 }
 
-int mcxyz_blocks_per_cu(bool two_term) {
-  int nb = 0;
-  hipError_t e = two_term
-      ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, mcxyz_kernel<true>, kXyzBlock, 0)
-      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, mcxyz_kernel<false>, kXyzBlock, 0);
-  return (e == hipSuccess && nb > 0) ? nb : 4;
+hipError_t mcxyz_configure(bool two_term, int block, size_t lds_bytes) {
+  return two_term ? configure<true>(block, lds_bytes) : configure<false>(block, lds_bytes);
 }
 
-hipError_t launch_zero_u64(unsigned long long* p, int n, hipStream_t s) {
-  hipLaunchKernelGGL(zero_u64_kernel, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
+int mcxyz_blocks_per_cu(bool two_term, int block, size_t lds_bytes) {
+  return two_term ? occupancy<true>(block, lds_bytes) : occupancy<false>(block, lds_bytes);
+}
+
+hipError_t launch_tile_reduce(const double* part, int grid, const XyzArgs& a, hipStream_t s) {
+  int n = a.tile_nx * a.tile_ny * a.tile_nz;
+  if (n <= 0 || !a.F) return hipSuccess;
+  hipLaunchKernelGGL(tile_reduce_kernel, dim3((n + 255) / 256), dim3(256), 0, s, part, grid, a);
   return hipGetLastError();
 }
 

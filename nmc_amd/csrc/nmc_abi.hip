@@ -2,7 +2,11 @@
 //
 // Plays the role of the reference's MetalMC object (NMC/NMCCode/MetalMC.m):
 // owns the device buffers, validates and uploads the configuration, launches
-// the transport kernels on its own HIP stream and returns the tallies.
+// the transport kernels on its own HIP stream and returns the tallies.  A
+// run is the sequence [reset queue counter, transport kernel, reduction of
+// block partials, advance the first photon id]; all four steps are kernels,
+// so the sequence can be captured once as a hipGraph and replayed.
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -39,6 +43,12 @@ int ok() {
     if (e_ != hipSuccess)                                                             \
       return fail(e_ == hipErrorOutOfMemory ? NMC_ERR_NOMEM : NMC_ERR_DEVICE,         \
                   "%s failed: %s", #expr, hipGetErrorString(e_));                     \
+  } while (0)
+
+#define NMC_TRY(expr)              \
+  do {                             \
+    int s_ = (expr);               \
+    if (s_ != NMC_OK) return s_;   \
   } while (0)
 
 bool finite_nonneg(float v) { return std::isfinite(v) && v >= 0.0f; }
@@ -84,51 +94,382 @@ void tt_prepare(float gf, float af, float gb, float ab, float cc, TTCoef* t) {
   }
 }
 
+// One medium of an mcsub run (mut, albedo, specular as the reference computes
+// them, kernels.metal:99-119).
+SubMedium make_medium(float mua, float mus, float g, float n1, float n2, uint32_t key1) {
+  SubMedium m{};
+  m.mut = mua + mus;
+  m.one_m_albedo = 1.0f - mus / m.mut;
+  m.g = g;
+  m.n1 = n1;
+  m.n2 = n2;
+  float tr = n1 / n2;
+  tr = (1.0f - tr) / (1.0f + tr);
+  m.rsp0 = tr * tr;
+  m.key1 = key1;
+  return m;
+}
+
+constexpr int kMaxBatchItems = 2048;
+
 }  // namespace
 
 struct nmc_handle {
   int sim = 0, device = 0, num_cu = 0;
+  size_t lds_max = 0;
   hipStream_t stream = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  bool prepared = false, timed = false;
+  bool prepared = false, timed = false, batch = false;
   RunParams rp{};
   TissueParams tp{};
   nmc_ext_config ext{};
+  nmc_tuning tune_req{};   // as requested by the caller
+  nmc_tuning tune{};       // resolved
   uint64_t next_photon = 0;
-  // device buffers
+  // device state
+  RunState* dState = nullptr;
+  nmc_totals* dTot = nullptr;
+  // mcxyz
   uint8_t* dV = nullptr;
   XyzTissue* dTis = nullptr;
   double* dF = nullptr;
+  double* dTilePart = nullptr;
   double* dRyx = nullptr;
   uint32_t* dDet = nullptr;
-  double* dRa = nullptr;
-  double* dRz = nullptr;
-  double* dPart = nullptr;
+  size_t xyz_lds = 0;
+  int xyz_grid = 0;
+  // mcsub (plain = batch of one)
+  int n_items = 0;
+  std::vector<nmc_work_item> items;
+  std::vector<uint64_t> prefix;
+  SubMedium* dItems = nullptr;
+  uint64_t* dPrefix = nullptr;
   float* dCos = nullptr;
-  nmc_totals* dTot = nullptr;
-  unsigned long long* dCounter = nullptr;
-  int sub_grid = 0, lds_bytes = 0;
+  double* dOut = nullptr;   // [n_items * per_item tallies | n_items nmc_totals]
+  double* dPart = nullptr;
   int64_t n_ra = 0, n_rz = 0;
+  bool sub_lds = false;
+  size_t sub_lds_bytes = 0;
+  int sub_grid = 0;
+  // graph mode
+  bool graph_mode = false;
+  hipGraphExec_t graph_exec = nullptr;
+  uint64_t graph_n = 0;
+  uint32_t graph_key = 0;
 };
 
 namespace {
 
+template <class T>
+void free_ptr(T*& p) {
+  if (p) (void)hipFree((void*)p);
+  p = nullptr;
+}
+
+void drop_graph(nmc_handle* h) {
+  if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
+  h->graph_exec = nullptr;
+  h->graph_n = 0;
+}
+
 void free_buffers(nmc_handle* h) {
-  void* bufs[] = {h->dV, h->dTis, h->dF, h->dRyx, h->dDet, h->dRa, h->dRz, h->dPart, h->dCos,
-                  h->dTot, h->dCounter};
-  for (void* b : bufs)
-    if (b) (void)hipFree(b);
-  h->dV = nullptr;
-  h->dTis = nullptr;
-  h->dF = h->dRyx = h->dRa = h->dRz = h->dPart = nullptr;
-  h->dDet = nullptr;
-  h->dCos = nullptr;
-  h->dTot = nullptr;
-  h->dCounter = nullptr;
+  drop_graph(h);
+  free_ptr(h->dV);
+  free_ptr(h->dTis);
+  free_ptr(h->dF);
+  free_ptr(h->dTilePart);
+  free_ptr(h->dRyx);
+  free_ptr(h->dDet);
+  free_ptr(h->dItems);
+  free_ptr(h->dPrefix);
+  free_ptr(h->dCos);
+  free_ptr(h->dOut);
+  free_ptr(h->dPart);
   h->prepared = false;
 }
 
 int64_t nvox(const RunParams* rp) { return (int64_t)rp->Nx * rp->Ny * rp->Nz; }
+
+int64_t per_item(const nmc_handle* h) { return h->n_ra + h->n_rz; }
+
+// ---- mcxyz launch configuration -------------------------------------------
+int resolve_xyz_tuning(nmc_handle* h) {
+  const nmc_tuning& r = h->tune_req;
+  nmc_tuning t{};
+  t.block = r.block ? r.block : 1024;
+  if (t.block != 256 && t.block != 512 && t.block != 1024)
+    return fail(NMC_ERR_ARG, "tuning.block %d not 256, 512 or 1024", t.block);
+  t.refill_min = r.refill_min ? r.refill_min : 16;
+  if (t.refill_min < 1 || t.refill_min > 64) return fail(NMC_ERR_ARG, "tuning.refill_min not in 1..64");
+  t.chunk = r.chunk;
+  bool tt = h->sim == NMC_SIM_MCXYZ_TT;
+  const RunParams& rp = h->rp;
+  const size_t static_lds = sizeof(XyzTissue) * NMC_NTISS + sizeof(nmc_totals) + 256;
+  int64_t budget = h->lds_max > static_lds ? (int64_t)((h->lds_max - static_lds) / sizeof(double)) : 0;
+  if (!h->dF || r.tile[3] < 0) {
+    for (int i = 0; i < 6; ++i) t.tile[i] = 0;
+  } else if (r.tile[3] > 0) {
+    for (int i = 0; i < 6; ++i) t.tile[i] = r.tile[i];
+  } else {
+    // automatic box: centred on the source in x,y, from the source depth down
+    // (centred on it for an interior point source), about 2/3 as deep as wide
+    int nz = (int)std::lround(0.66 * std::cbrt((double)budget));
+    nz = std::max(1, std::min(nz, rp.Nz));
+    int nxy = (int)std::floor(std::sqrt((double)budget / nz));
+    int nx = std::max(1, std::min(nxy, rp.Nx)), ny = std::max(1, std::min(nxy, rp.Ny));
+    float sx = rp.launchflag == 1 || rp.mcflag == 2 ? rp.xs : (rp.mcflag == 4 ? 0.0f : rp.xs);
+    float sy = rp.launchflag == 1 || rp.mcflag == 2 ? rp.ys : (rp.mcflag == 4 ? 0.0f : rp.ys);
+    int cx = (int)std::floor(0.5f * rp.Nx + sx / rp.dx), cy = (int)std::floor(0.5f * rp.Ny + sy / rp.dy);
+    int cz = (int)std::floor(rp.zs / rp.dz);
+    int z0 = (rp.mcflag == 2 || rp.launchflag == 1) ? cz - nz / 2 : cz;
+    t.tile[0] = std::max(0, std::min(cx - nx / 2, rp.Nx - nx));
+    t.tile[1] = std::max(0, std::min(cy - ny / 2, rp.Ny - ny));
+    t.tile[2] = std::max(0, std::min(z0, rp.Nz - nz));
+    t.tile[3] = nx;
+    t.tile[4] = ny;
+    t.tile[5] = nz;
+  }
+  int64_t tile_n = (int64_t)t.tile[3] * t.tile[4] * t.tile[5];
+  if (tile_n > 0) {
+    if (t.tile[0] < 0 || t.tile[1] < 0 || t.tile[2] < 0 || t.tile[0] + t.tile[3] > rp.Nx ||
+        t.tile[1] + t.tile[4] > rp.Ny || t.tile[2] + t.tile[5] > rp.Nz)
+      return fail(NMC_ERR_ARG, "tuning.tile box outside the grid");
+    if (tile_n > budget) return fail(NMC_ERR_ARG, "tuning.tile needs %lld voxels, LDS holds %lld",
+                                     (long long)tile_n, (long long)budget);
+  }
+  h->xyz_lds = (size_t)tile_n * sizeof(double);
+  HIP_TRY(mcxyz_configure(tt, t.block, h->xyz_lds));
+  int occ = mcxyz_blocks_per_cu(tt, t.block, h->xyz_lds);
+  if (occ <= 0) return fail(NMC_ERR_ARG, "kernel does not fit on a CU with block %d, %zu B LDS", t.block, h->xyz_lds);
+  t.blocks_per_cu = r.blocks_per_cu > 0 ? std::min(r.blocks_per_cu, occ) : occ;
+  h->xyz_grid = h->num_cu * t.blocks_per_cu;
+  h->tune = t;
+  free_ptr(h->dTilePart);
+  if (tile_n > 0) HIP_TRY(hipMalloc(&h->dTilePart, (size_t)h->xyz_grid * tile_n * sizeof(double)));
+  drop_graph(h);
+  return ok();
+}
+
+// ---- mcsub launch configuration -------------------------------------------
+int resolve_sub_tuning(nmc_handle* h) {
+  const nmc_tuning& r = h->tune_req;
+  nmc_tuning t{};
+  t.block = kSubBlock;
+  if (r.block && r.block != kSubBlock) return fail(NMC_ERR_ARG, "mcsub runs with block %d", kSubBlock);
+  t.refill_min = 1;
+  t.chunk = r.chunk;
+  size_t lds_full = mcsub_lds_bytes(h->n_items, per_item(h)) + 256;
+  h->sub_lds = lds_full <= h->lds_max;
+  h->sub_lds_bytes = h->sub_lds ? mcsub_lds_bytes(h->n_items, per_item(h))
+                                : (size_t)h->n_items * sizeof(SubMedium) + (h->n_items + 1) * sizeof(uint64_t);
+  HIP_TRY(mcsub_configure(h->sub_lds_bytes, h->sub_lds));
+  int occ = mcsub_blocks_per_cu(h->sub_lds_bytes, h->sub_lds);
+  if (occ <= 0) return fail(NMC_ERR_ARG, "mcsub kernel does not fit on a CU");
+  t.blocks_per_cu = r.blocks_per_cu > 0 ? std::min(r.blocks_per_cu, occ) : occ;
+  h->sub_grid = h->num_cu * t.blocks_per_cu;
+  h->tune = t;
+  free_ptr(h->dPart);
+  if (h->sub_lds) {
+    size_t stride = (size_t)h->n_items * (per_item(h) + TF_COUNT + TI_COUNT);
+    HIP_TRY(hipMalloc(&h->dPart, (size_t)h->sub_grid * stride * sizeof(double)));
+  }
+  drop_graph(h);
+  return ok();
+}
+
+uint32_t pick_chunk(uint64_t n, int waves, int req) {
+  if (req > 0) return (uint32_t)req;
+  uint64_t c = n / ((uint64_t)waves * 8ull);
+  return (uint32_t)std::max<uint64_t>(64, std::min<uint64_t>(1024, c));
+}
+
+uint32_t key0_of(const nmc_handle* h) { return (uint32_t)h->ext.seed; }
+uint32_t key1_of(const nmc_handle* h, uint32_t stream) { return (uint32_t)(h->ext.seed >> 32) + stream; }
+
+// Enqueues one run (no events, no capture handling).
+int enqueue_run(nmc_handle* h, uint64_t n) {
+  hipStream_t s = h->stream;
+  HIP_TRY(launch_state_begin(h->dState, s));
+  if (h->sim == NMC_SIM_MCSUB) {
+    const RunParams& rp = h->rp;
+    const nmc_ext_config& e = h->ext;
+    SubArgs a{};
+    a.mcflag = rp.mcflag;
+    a.bmode = rp.boundaryflag;
+    a.nr = e.nr;
+    a.nz = e.nz;
+    a.na = e.na;
+    a.dr = e.dr;
+    a.dz = e.dz;
+    a.radius = rp.radius;
+    a.waist = rp.waist;
+    a.zfocus = rp.zfocus;
+    a.xs = rp.xs;
+    a.ys = rp.ys;
+    a.zs = rp.zs;
+    a.thr = e.threshold;
+    a.chance = e.chance;
+    a.max_events = e.max_events;
+    a.key0 = key0_of(h);
+    uint64_t total = h->prefix[h->n_items];
+    a.chunk = pick_chunk(total, h->sub_grid * (kSubBlock / 64), h->tune.chunk);
+    a.n_items = h->n_items;
+    a.items = h->dItems;
+    a.prefix = h->dPrefix;
+    a.cos_tab = h->dCos;
+    a.tally_rz = h->n_rz > 0;
+    a.tally = h->dOut;
+    a.tot = (nmc_totals*)(h->dOut + (int64_t)h->n_items * per_item(h));
+    a.part = h->dPart;
+    a.state = h->dState;
+    HIP_TRY(launch_mcsub(a, h->sub_grid, h->sub_lds_bytes, h->sub_lds, s));
+    if (h->sub_lds) {
+      int64_t stride = (int64_t)h->n_items * (per_item(h) + TF_COUNT + TI_COUNT);
+      HIP_TRY(launch_reduce_partials(h->dPart, h->sub_grid, stride, h->dOut, h->n_items,
+                                     (int)per_item(h), s));
+    }
+    uint64_t adv = 0;
+    for (int j = 0; j < h->n_items; ++j) adv = std::max<uint64_t>(adv, h->prefix[j + 1] - h->prefix[j]);
+    HIP_TRY(launch_state_advance(h->dState, adv, s));
+  } else {
+    const RunParams& rp = h->rp;
+    const nmc_ext_config& e = h->ext;
+    XyzArgs a{};
+    a.Nx = rp.Nx;
+    a.Ny = rp.Ny;
+    a.Nz = rp.Nz;
+    a.dx = rp.dx;
+    a.dy = rp.dy;
+    a.dz = rp.dz;
+    a.hx = 0.5f * (float)rp.Nx;
+    a.hy = 0.5f * (float)rp.Ny;
+    a.bmode = rp.boundaryflag;
+    a.det_state = rp.det_state;
+    a.mcflag = rp.mcflag;
+    a.launchflag = rp.launchflag;
+    a.zsurf = rp.zsurf;
+    a.n1 = rp.n1;
+    a.n2 = rp.n2;
+    if (rp.xd == 0.0f) {
+      a.rmin = 0.0f;
+      a.rmax = rp.det_radius;
+    } else {
+      a.rmin = rp.xd - rp.det_radius;
+      a.rmax = a.rmin + 2.0f * rp.det_radius;
+    }
+    a.cos_na = rp.na >= 90.0f ? -1.0f : (float)std::cos((double)rp.na * 3.14159265358979323846 / 180.0);
+    a.xs = rp.xs;
+    a.ys = rp.ys;
+    a.zs = rp.zs;
+    a.ux0 = rp.ux0;
+    a.uy0 = rp.uy0;
+    a.uz0 = rp.uz0;
+    a.radius = rp.radius;
+    a.waist = rp.waist;
+    a.zfocus = rp.zfocus;
+    a.thr = e.threshold;
+    a.chance = e.chance;
+    a.max_events = e.max_events;
+    a.key0 = key0_of(h);
+    a.key1 = key1_of(h, e.stream_id);
+    a.n = n;
+    a.chunk = pick_chunk(n, h->xyz_grid * (h->tune.block / 64), h->tune.chunk);
+    a.refill_min = (uint32_t)h->tune.refill_min;
+    a.tile_x0 = h->tune.tile[0];
+    a.tile_y0 = h->tune.tile[1];
+    a.tile_z0 = h->tune.tile[2];
+    a.tile_nx = h->tune.tile[3];
+    a.tile_ny = h->tune.tile[4];
+    a.tile_nz = h->tune.tile[5];
+    a.V = h->dV;
+    a.tissue = h->dTis;
+    a.F = h->dF;
+    a.tile_part = h->dTilePart;
+    a.ryx = h->dRyx;
+    a.det2d = h->dDet;
+    a.tot = h->dTot;
+    a.state = h->dState;
+    bool tt = h->sim == NMC_SIM_MCXYZ_TT;
+    HIP_TRY(launch_mcxyz(a, tt, h->tune.block, h->xyz_grid, h->xyz_lds, s));
+    HIP_TRY(launch_tile_reduce(h->dTilePart, h->xyz_grid, a, s));
+    HIP_TRY(launch_state_advance(h->dState, n, s));
+  }
+  return ok();
+}
+
+int write_first(nmc_handle* h) {
+  RunState st{0ull, (unsigned long long)h->next_photon};
+  HIP_TRY(hipMemcpyAsync(h->dState, &st, sizeof st, hipMemcpyHostToDevice, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return ok();
+}
+
+// Runs (n photons for mcxyz; every item's budget for mcsub), via the graph in
+// graph mode.
+int run_async(nmc_handle* h, uint64_t n) {
+  HIP_TRY(hipSetDevice(h->device));
+  HIP_TRY(hipEventRecord(h->e0, h->stream));
+  if (h->graph_mode) {
+    uint32_t key = key0_of(h) ^ (key1_of(h, h->ext.stream_id) * 0x9E3779B9u);
+    if (!h->graph_exec || h->graph_n != n || h->graph_key != key) {
+      drop_graph(h);
+      hipGraph_t g = nullptr;
+      HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+      int st = enqueue_run(h, n);
+      hipError_t ce = hipStreamEndCapture(h->stream, &g);
+      if (st != NMC_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+      }
+      HIP_TRY(ce);
+      hipError_t ie = hipGraphInstantiate(&h->graph_exec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      HIP_TRY(ie);
+      h->graph_n = n;
+      h->graph_key = key;
+    }
+    HIP_TRY(hipGraphLaunch(h->graph_exec, h->stream));
+  } else {
+    NMC_TRY(enqueue_run(h, n));
+  }
+  HIP_TRY(hipEventRecord(h->e1, h->stream));
+  h->timed = true;
+  return ok();
+}
+
+int alloc_sub_outputs(nmc_handle* h) {
+  free_ptr(h->dOut);
+  size_t n = (size_t)h->n_items * (per_item(h) + TF_COUNT + TI_COUNT);
+  HIP_TRY(hipMalloc(&h->dOut, n * sizeof(double)));
+  HIP_TRY(hipMalloc(&h->dItems, h->n_items * sizeof(SubMedium)));
+  HIP_TRY(hipMalloc(&h->dPrefix, (h->n_items + 1) * sizeof(uint64_t)));
+  std::vector<SubMedium> med(h->n_items);
+  h->prefix.assign(h->n_items + 1, 0);
+  for (int j = 0; j < h->n_items; ++j) {
+    const nmc_work_item& it = h->items[j];
+    med[j] = make_medium(it.mua, it.mus, it.g, it.n1, it.n2, key1_of(h, it.stream_id));
+    h->prefix[j + 1] = h->prefix[j] + it.n_photons;
+  }
+  HIP_TRY(hipMemcpy(h->dItems, med.data(), med.size() * sizeof(SubMedium), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(h->dPrefix, h->prefix.data(), h->prefix.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+  std::vector<float> cos_tab(64, 0.0f);
+  for (int j = 0; j < h->ext.na; ++j)
+    cos_tab[j] = (float)std::cos((double)(j + 1) * (3.14159265358979323846 / 2.0) / (double)h->ext.na);
+  HIP_TRY(hipMalloc(&h->dCos, 64 * sizeof(float)));
+  HIP_TRY(hipMemcpy(h->dCos, cos_tab.data(), 64 * sizeof(float), hipMemcpyHostToDevice));
+  return ok();
+}
+
+// Re-derives the per-item media (their Philox key depends on the seed).
+int upload_media(nmc_handle* h) {
+  std::vector<SubMedium> med(h->n_items);
+  for (int j = 0; j < h->n_items; ++j) {
+    const nmc_work_item& it = h->items[j];
+    med[j] = make_medium(it.mua, it.mus, it.g, it.n1, it.n2, key1_of(h, it.stream_id));
+  }
+  HIP_TRY(hipMemcpy(h->dItems, med.data(), med.size() * sizeof(SubMedium), hipMemcpyHostToDevice));
+  return ok();
+}
 
 }  // namespace
 
@@ -141,7 +482,7 @@ void nmc_ext_config_init(nmc_ext_config* cfg) {
 const char* nmc_last_error(void) { return g_err.c_str(); }
 
 const char* nmc_version(void) {
-  return "nmc_hip 1 gfx950 -O3 -ffp-contract=off wave64 philox4x32-10";
+  return "nmc_hip 2 gfx950 -O3 -ffp-contract=off wave64 philox4x32-10";
 }
 
 int nmc_validate(int sim, const RunParams* rp, const TissueParams* tp, const uint8_t* v,
@@ -222,12 +563,15 @@ int nmc_create(int sim_type, int device, nmc_handle** out) {
   nmc_handle* h = new nmc_handle();
   h->sim = sim_type;
   h->device = device;
+  int lds = 0;
   hipError_t e = hipDeviceGetAttribute(&h->num_cu, hipDeviceAttributeMultiprocessorCount, device);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device);
+  h->lds_max = (size_t)lds;
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreate(&h->e0);
   if (e == hipSuccess) e = hipEventCreate(&h->e1);
   if (e == hipSuccess) e = hipMalloc(&h->dTot, sizeof(nmc_totals));
-  if (e == hipSuccess) e = hipMalloc(&h->dCounter, sizeof(unsigned long long));
+  if (e == hipSuccess) e = hipMalloc(&h->dState, sizeof(RunState));
   if (e != hipSuccess) {
     nmc_destroy(h);
     return fail(NMC_ERR_DEVICE, "device setup failed: %s", hipGetErrorString(e));
@@ -236,70 +580,97 @@ int nmc_create(int sim_type, int device, nmc_handle** out) {
   return ok();
 }
 
+static int prepare_sub(nmc_handle* h, const RunParams* rp, const nmc_ext_config* ext,
+                       const nmc_work_item* items, int n_items, bool batch) {
+  h->rp = *rp;
+  ext_defaults(&h->ext);
+  if (ext) h->ext = *ext;
+  h->batch = batch;
+  h->n_items = n_items;
+  h->items.assign(items, items + n_items);
+  h->n_ra = (int64_t)(h->ext.nr + 1) * h->ext.na;
+  h->n_rz = (h->ext.flags & NMC_FLAG_NO_FLUENCE) ? 0 : (int64_t)(h->ext.nr + 1) * (h->ext.nz + 1);
+  NMC_TRY(alloc_sub_outputs(h));
+  NMC_TRY(resolve_sub_tuning(h));
+  h->prepared = true;
+  return nmc_reset_tallies(h);
+}
+
 int nmc_prepare(nmc_handle* h, const RunParams* rp, const TissueParams* tp, const uint8_t* v,
                 const nmc_ext_config* ext) {
   if (!h) return fail(NMC_ERR_ARG, "null handle");
-  int st = nmc_validate(h->sim, rp, tp, v, ext);
-  if (st != NMC_OK) return st;
+  NMC_TRY(nmc_validate(h->sim, rp, tp, v, ext));
   HIP_TRY(hipSetDevice(h->device));
   HIP_TRY(hipStreamSynchronize(h->stream));
-  // keep dTot / dCounter across prepares
-  nmc_totals* tot = h->dTot;
-  unsigned long long* ctr = h->dCounter;
-  h->dTot = nullptr;
-  h->dCounter = nullptr;
   free_buffers(h);
-  h->dTot = tot;
-  h->dCounter = ctr;
-  h->rp = *rp;
   h->tp = *tp;
-  ext_defaults(&h->ext);
-  if (ext) h->ext = *ext;
   h->next_photon = 0;
   h->timed = false;
-
   if (h->sim == NMC_SIM_MCSUB) {
-    const nmc_ext_config& e = h->ext;
-    h->n_ra = (int64_t)(e.nr + 1) * e.na;
-    h->n_rz = (e.flags & NMC_FLAG_NO_FLUENCE) ? 0 : (int64_t)(e.nr + 1) * (e.nz + 1);
-    HIP_TRY(hipMalloc(&h->dRa, h->n_ra * sizeof(double)));
-    if (h->n_rz) HIP_TRY(hipMalloc(&h->dRz, h->n_rz * sizeof(double)));
-    std::vector<float> cos_tab(64, 0.0f);
-    for (int j = 0; j < e.na; ++j)
-      cos_tab[j] = (float)std::cos((double)(j + 1) * (3.14159265358979323846 / 2.0) / (double)e.na);
-    HIP_TRY(hipMalloc(&h->dCos, 64 * sizeof(float)));
-    HIP_TRY(hipMemcpy(h->dCos, cos_tab.data(), 64 * sizeof(float), hipMemcpyHostToDevice));
-    int lds_max = 0;
-    HIP_TRY(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, h->device));
-    int64_t bytes = (h->n_ra + h->n_rz) * (int64_t)sizeof(double);
-    // keep room for the static LDS (totals, cosine table)
-    h->lds_bytes = bytes + 2048 <= lds_max ? (int)bytes : 0;
-    if (h->lds_bytes) {
-      h->sub_grid = h->num_cu * mcsub_blocks_per_cu(h->lds_bytes);
-      HIP_TRY(hipMalloc(&h->dPart, (size_t)h->sub_grid * (h->n_ra + h->n_rz) * sizeof(double)));
-    } else {
-      h->sub_grid = h->num_cu * mcsub_blocks_per_cu(0);
-    }
-  } else {
-    int64_t n = nvox(rp), n2 = (int64_t)rp->Nx * rp->Ny;
-    HIP_TRY(hipMalloc(&h->dV, n));
-    HIP_TRY(hipMemcpy(h->dV, v, n, hipMemcpyHostToDevice));
-    if (!(h->ext.flags & NMC_FLAG_NO_FLUENCE)) HIP_TRY(hipMalloc(&h->dF, n * sizeof(double)));
-    HIP_TRY(hipMalloc(&h->dRyx, n2 * sizeof(double)));
-    HIP_TRY(hipMalloc(&h->dDet, n2 * sizeof(uint32_t)));
-    std::vector<XyzTissue> tis(NMC_NTISS);
-    for (int t = 0; t < NMC_NTISS; ++t) {
-      std::memset(&tis[t], 0, sizeof(XyzTissue));
-      tis[t].mua = tp->muav[t];
-      tis[t].mus = tp->musv[t];
-      tis[t].g = tp->gv[t];
-      if (h->sim == NMC_SIM_MCXYZ_TT) tt_prepare(tp->gf[t], tp->af[t], tp->gb[t], tp->ab[t], tp->CC[t], &tis[t].tt);
-    }
-    HIP_TRY(hipMalloc(&h->dTis, NMC_NTISS * sizeof(XyzTissue)));
-    HIP_TRY(hipMemcpy(h->dTis, tis.data(), NMC_NTISS * sizeof(XyzTissue), hipMemcpyHostToDevice));
+    nmc_ext_config e;
+    ext_defaults(&e);
+    if (ext) e = *ext;
+    nmc_work_item it{};
+    it.mua = tp->muav[1];
+    it.mus = tp->musv[1];
+    it.g = tp->gv[1];
+    it.n1 = rp->n1;
+    it.n2 = rp->n2;
+    it.stream_id = e.stream_id;
+    it.n_photons = 0;
+    NMC_TRY(prepare_sub(h, rp, ext, &it, 1, false));
+    return write_first(h);
   }
+  h->rp = *rp;
+  ext_defaults(&h->ext);
+  if (ext) h->ext = *ext;
+  int64_t n = nvox(rp), n2 = (int64_t)rp->Nx * rp->Ny;
+  HIP_TRY(hipMalloc(&h->dV, n));
+  HIP_TRY(hipMemcpy(h->dV, v, n, hipMemcpyHostToDevice));
+  if (!(h->ext.flags & NMC_FLAG_NO_FLUENCE)) HIP_TRY(hipMalloc(&h->dF, n * sizeof(double)));
+  HIP_TRY(hipMalloc(&h->dRyx, n2 * sizeof(double)));
+  HIP_TRY(hipMalloc(&h->dDet, n2 * sizeof(uint32_t)));
+  std::vector<XyzTissue> tis(NMC_NTISS);
+  for (int t = 0; t < NMC_NTISS; ++t) {
+    std::memset(&tis[t], 0, sizeof(XyzTissue));
+    tis[t].mua = tp->muav[t];
+    tis[t].mus = tp->musv[t];
+    tis[t].g = tp->gv[t];
+    if (h->sim == NMC_SIM_MCXYZ_TT) tt_prepare(tp->gf[t], tp->af[t], tp->gb[t], tp->ab[t], tp->CC[t], &tis[t].tt);
+  }
+  HIP_TRY(hipMalloc(&h->dTis, NMC_NTISS * sizeof(XyzTissue)));
+  HIP_TRY(hipMemcpy(h->dTis, tis.data(), NMC_NTISS * sizeof(XyzTissue), hipMemcpyHostToDevice));
+  NMC_TRY(resolve_xyz_tuning(h));
   h->prepared = true;
-  return nmc_reset_tallies(h);
+  NMC_TRY(nmc_reset_tallies(h));
+  return write_first(h);
+}
+
+int nmc_prepare_batch(nmc_handle* h, const RunParams* rp, const nmc_ext_config* ext,
+                      const nmc_work_item* items, int n_items) {
+  if (!h) return fail(NMC_ERR_ARG, "null handle");
+  if (h->sim != NMC_SIM_MCSUB) return fail(NMC_ERR_UNSUPPORTED, "batched runs need an NMC_SIM_MCSUB handle");
+  if (!items || n_items <= 0 || n_items > kMaxBatchItems)
+    return fail(NMC_ERR_ARG, "n_items must be in 1..%d", kMaxBatchItems);
+  TissueParams tp{};
+  for (int j = 0; j < n_items; ++j) {
+    const nmc_work_item& it = items[j];
+    tp.muav[1] = it.mua;
+    tp.musv[1] = it.mus;
+    tp.gv[1] = it.g;
+    RunParams r = *rp;
+    r.n1 = it.n1;
+    r.n2 = it.n2;
+    int st = nmc_validate(NMC_SIM_MCSUB, &r, &tp, nullptr, ext);
+    if (st != NMC_OK) return fail(st, "work item %d: %s", j, g_err.c_str());
+  }
+  HIP_TRY(hipSetDevice(h->device));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  free_buffers(h);
+  h->next_photon = 0;
+  h->timed = false;
+  NMC_TRY(prepare_sub(h, rp, ext, items, n_items, true));
+  return write_first(h);
 }
 
 int nmc_reset_tallies(nmc_handle* h) {
@@ -308,8 +679,8 @@ int nmc_reset_tallies(nmc_handle* h) {
   HIP_TRY(hipSetDevice(h->device));
   HIP_TRY(hipMemsetAsync(h->dTot, 0, sizeof(nmc_totals), h->stream));
   if (h->sim == NMC_SIM_MCSUB) {
-    HIP_TRY(hipMemsetAsync(h->dRa, 0, h->n_ra * sizeof(double), h->stream));
-    if (h->dRz) HIP_TRY(hipMemsetAsync(h->dRz, 0, h->n_rz * sizeof(double), h->stream));
+    size_t n = (size_t)h->n_items * (per_item(h) + TF_COUNT + TI_COUNT);
+    HIP_TRY(hipMemsetAsync(h->dOut, 0, n * sizeof(double), h->stream));
   } else {
     int64_t n = nvox(&h->rp), n2 = (int64_t)h->rp.Nx * h->rp.Ny;
     if (h->dF) HIP_TRY(hipMemsetAsync(h->dF, 0, n * sizeof(double), h->stream));
@@ -323,132 +694,52 @@ int nmc_reset_tallies(nmc_handle* h) {
 int nmc_set_seed(nmc_handle* h, uint64_t seed, uint64_t first_photon) {
   if (!h) return fail(NMC_ERR_ARG, "null handle");
   if (!h->prepared) return fail(NMC_ERR_STATE, "nmc_prepare has not succeeded");
+  HIP_TRY(hipSetDevice(h->device));
+  HIP_TRY(hipStreamSynchronize(h->stream));
   h->ext.seed = seed;
   h->next_photon = first_photon;
+  if (h->sim == NMC_SIM_MCSUB) NMC_TRY(upload_media(h));
+  return write_first(h);
+}
+
+int nmc_set_tuning(nmc_handle* h, const nmc_tuning* t) {
+  if (!h || !t) return fail(NMC_ERR_ARG, "null argument");
+  h->tune_req = *t;
+  if (!h->prepared) return ok();
+  HIP_TRY(hipSetDevice(h->device));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return h->sim == NMC_SIM_MCSUB ? resolve_sub_tuning(h) : resolve_xyz_tuning(h);
+}
+
+int nmc_get_tuning(nmc_handle* h, nmc_tuning* t) {
+  if (!h || !t) return fail(NMC_ERR_ARG, "null argument");
+  if (!h->prepared) return fail(NMC_ERR_STATE, "nmc_prepare has not succeeded");
+  *t = h->tune;
   return ok();
 }
 
-static uint32_t pick_chunk(uint64_t n, int waves) {
-  uint64_t c = n / ((uint64_t)waves * 8ull);
-  if (c < 64) c = 64;
-  if (c > 1024) c = 1024;
-  return (uint32_t)c;
+int nmc_set_graph_mode(nmc_handle* h, int enable) {
+  if (!h) return fail(NMC_ERR_ARG, "null handle");
+  HIP_TRY(hipSetDevice(h->device));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  h->graph_mode = enable != 0;
+  drop_graph(h);
+  return ok();
 }
 
 int nmc_run_async(nmc_handle* h, uint64_t n_photons) {
   if (!h) return fail(NMC_ERR_ARG, "null handle");
   if (!h->prepared) return fail(NMC_ERR_STATE, "nmc_prepare has not succeeded");
   if (n_photons == 0) return ok();
-  HIP_TRY(hipSetDevice(h->device));
-  const nmc_ext_config& e = h->ext;
-  uint32_t key0 = (uint32_t)e.seed, key1 = (uint32_t)(e.seed >> 32) + e.stream_id;
-  HIP_TRY(launch_zero_u64(h->dCounter, 1, h->stream));
-  HIP_TRY(hipEventRecord(h->e0, h->stream));
   if (h->sim == NMC_SIM_MCSUB) {
-    const RunParams& rp = h->rp;
-    SubArgs a{};
-    float mua = h->tp.muav[1], mus = h->tp.musv[1];
-    a.mut = mua + mus;
-    a.one_m_albedo = 1.0f - mus / a.mut;
-    a.g = h->tp.gv[1];
-    a.n1 = rp.n1;
-    a.n2 = rp.n2;
-    float tr = rp.n1 / rp.n2;
-    tr = (1.0f - tr) / (1.0f + tr);
-    a.rsp0 = tr * tr;
-    a.mcflag = rp.mcflag;
-    a.bmode = rp.boundaryflag;
-    a.nr = e.nr;
-    a.nz = e.nz;
-    a.na = e.na;
-    a.dr = e.dr;
-    a.dz = e.dz;
-    a.radius = rp.radius;
-    a.waist = rp.waist;
-    a.zfocus = rp.zfocus;
-    a.xs = rp.xs;
-    a.ys = rp.ys;
-    a.zs = rp.zs;
-    a.thr = e.threshold;
-    a.chance = e.chance;
-    a.max_events = e.max_events;
-    a.key0 = key0;
-    a.key1 = key1;
-    a.first = h->next_photon;
-    a.n = n_photons;
-    a.chunk = pick_chunk(n_photons, h->sub_grid * (kSubBlock / 64));
-    a.cos_tab = h->dCos;
-    a.tally_rz = h->dRz != nullptr;
-    a.lds_tallies = h->lds_bytes > 0;
-    a.part = h->dPart;
-    a.rd_ra = h->dRa;
-    a.a_rz = h->dRz;
-    a.tot = h->dTot;
-    a.counter = h->dCounter;
-    HIP_TRY(launch_mcsub(a, h->sub_grid, h->lds_bytes, h->stream));
-    if (a.lds_tallies) {
-      int64_t stride = h->n_ra + h->n_rz;  // block partial = [Rd(r,a) | A(r,z)]
-      HIP_TRY(launch_reduce_partials(h->dPart, h->sub_grid, stride, h->n_ra, h->dRa, h->stream));
-      if (h->n_rz)
-        HIP_TRY(launch_reduce_partials(h->dPart + h->n_ra, h->sub_grid, stride, h->n_rz, h->dRz,
-                                       h->stream));
+    if (h->batch) return fail(NMC_ERR_STATE, "handle holds a batch; use nmc_run_batch");
+    if (h->items[0].n_photons != n_photons) {
+      h->items[0].n_photons = n_photons;
+      h->prefix[1] = n_photons;
+      HIP_TRY(hipMemcpy(h->dPrefix, h->prefix.data(), 2 * sizeof(uint64_t), hipMemcpyHostToDevice));
     }
-  } else {
-    const RunParams& rp = h->rp;
-    XyzArgs a{};
-    a.Nx = rp.Nx;
-    a.Ny = rp.Ny;
-    a.Nz = rp.Nz;
-    a.dx = rp.dx;
-    a.dy = rp.dy;
-    a.dz = rp.dz;
-    a.hx = 0.5f * (float)rp.Nx;
-    a.hy = 0.5f * (float)rp.Ny;
-    a.bmode = rp.boundaryflag;
-    a.det_state = rp.det_state;
-    a.mcflag = rp.mcflag;
-    a.launchflag = rp.launchflag;
-    a.zsurf = rp.zsurf;
-    a.n1 = rp.n1;
-    a.n2 = rp.n2;
-    if (rp.xd == 0.0f) {
-      a.rmin = 0.0f;
-      a.rmax = rp.det_radius;
-    } else {
-      a.rmin = rp.xd - rp.det_radius;
-      a.rmax = a.rmin + 2.0f * rp.det_radius;
-    }
-    a.cos_na = rp.na >= 90.0f ? -1.0f : (float)std::cos((double)rp.na * 3.14159265358979323846 / 180.0);
-    a.xs = rp.xs;
-    a.ys = rp.ys;
-    a.zs = rp.zs;
-    a.ux0 = rp.ux0;
-    a.uy0 = rp.uy0;
-    a.uz0 = rp.uz0;
-    a.radius = rp.radius;
-    a.waist = rp.waist;
-    a.zfocus = rp.zfocus;
-    a.thr = e.threshold;
-    a.chance = e.chance;
-    a.max_events = e.max_events;
-    a.key0 = key0;
-    a.key1 = key1;
-    a.first = h->next_photon;
-    a.n = n_photons;
-    bool tt = h->sim == NMC_SIM_MCXYZ_TT;
-    int grid = h->num_cu * mcxyz_blocks_per_cu(tt);
-    a.chunk = pick_chunk(n_photons, grid * (kXyzBlock / 64));
-    a.V = h->dV;
-    a.tissue = h->dTis;
-    a.F = h->dF;
-    a.ryx = h->dRyx;
-    a.det2d = h->dDet;
-    a.tot = h->dTot;
-    a.counter = h->dCounter;
-    HIP_TRY(launch_mcxyz(a, tt, grid, h->stream));
   }
-  HIP_TRY(hipEventRecord(h->e1, h->stream));
-  h->timed = true;
+  NMC_TRY(run_async(h, n_photons));
   h->next_photon += n_photons;
   return ok();
 }
@@ -461,8 +752,22 @@ int nmc_synchronize(nmc_handle* h) {
 }
 
 int nmc_run(nmc_handle* h, uint64_t n_photons) {
-  int st = nmc_run_async(h, n_photons);
-  if (st != NMC_OK) return st;
+  NMC_TRY(nmc_run_async(h, n_photons));
+  return nmc_synchronize(h);
+}
+
+int nmc_run_batch_async(nmc_handle* h) {
+  if (!h) return fail(NMC_ERR_ARG, "null handle");
+  if (!h->prepared || !h->batch) return fail(NMC_ERR_STATE, "nmc_prepare_batch has not succeeded");
+  uint64_t adv = 0;
+  for (int j = 0; j < h->n_items; ++j) adv = std::max<uint64_t>(adv, h->items[j].n_photons);
+  NMC_TRY(run_async(h, h->prefix[h->n_items]));
+  h->next_photon += adv;
+  return ok();
+}
+
+int nmc_run_batch(nmc_handle* h) {
+  NMC_TRY(nmc_run_batch_async(h));
   return nmc_synchronize(h);
 }
 
@@ -481,21 +786,35 @@ int nmc_get_results(nmc_handle* h, nmc_results* out) {
   HIP_TRY(hipSetDevice(h->device));
   HIP_TRY(hipStreamSynchronize(h->stream));
   if (h->sim == NMC_SIM_MCSUB) {
-    if (out->rd_ra) HIP_TRY(hipMemcpy(out->rd_ra, h->dRa, h->n_ra * sizeof(double), hipMemcpyDeviceToHost));
+    if (h->batch) return fail(NMC_ERR_STATE, "handle holds a batch; use nmc_get_batch_results");
+    if (out->rd_ra) HIP_TRY(hipMemcpy(out->rd_ra, h->dOut, h->n_ra * sizeof(double), hipMemcpyDeviceToHost));
     if (out->a_rz) {
-      if (!h->dRz) return fail(NMC_ERR_STATE, "A(r,z) tally disabled by NMC_FLAG_NO_FLUENCE");
-      HIP_TRY(hipMemcpy(out->a_rz, h->dRz, h->n_rz * sizeof(double), hipMemcpyDeviceToHost));
+      if (!h->n_rz) return fail(NMC_ERR_STATE, "A(r,z) tally disabled by NMC_FLAG_NO_FLUENCE");
+      HIP_TRY(hipMemcpy(out->a_rz, h->dOut + h->n_ra, h->n_rz * sizeof(double), hipMemcpyDeviceToHost));
     }
-  } else {
-    int64_t n = nvox(&h->rp), n2 = (int64_t)h->rp.Nx * h->rp.Ny;
-    if (out->fluence) {
-      if (!h->dF) return fail(NMC_ERR_STATE, "fluence tally disabled by NMC_FLAG_NO_FLUENCE");
-      HIP_TRY(hipMemcpy(out->fluence, h->dF, n * sizeof(double), hipMemcpyDeviceToHost));
-    }
-    if (out->ryx) HIP_TRY(hipMemcpy(out->ryx, h->dRyx, n2 * sizeof(double), hipMemcpyDeviceToHost));
-    if (out->det2d) HIP_TRY(hipMemcpy(out->det2d, h->dDet, n2 * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(&out->totals, h->dOut + per_item(h), sizeof(nmc_totals), hipMemcpyDeviceToHost));
+    return ok();
   }
+  int64_t n = nvox(&h->rp), n2 = (int64_t)h->rp.Nx * h->rp.Ny;
+  if (out->fluence) {
+    if (!h->dF) return fail(NMC_ERR_STATE, "fluence tally disabled by NMC_FLAG_NO_FLUENCE");
+    HIP_TRY(hipMemcpy(out->fluence, h->dF, n * sizeof(double), hipMemcpyDeviceToHost));
+  }
+  if (out->ryx) HIP_TRY(hipMemcpy(out->ryx, h->dRyx, n2 * sizeof(double), hipMemcpyDeviceToHost));
+  if (out->det2d) HIP_TRY(hipMemcpy(out->det2d, h->dDet, n2 * sizeof(uint32_t), hipMemcpyDeviceToHost));
   HIP_TRY(hipMemcpy(&out->totals, h->dTot, sizeof(nmc_totals), hipMemcpyDeviceToHost));
+  return ok();
+}
+
+int nmc_get_batch_results(nmc_handle* h, double* tally, nmc_totals* totals) {
+  if (!h) return fail(NMC_ERR_ARG, "null handle");
+  if (!h->prepared || !h->batch) return fail(NMC_ERR_STATE, "nmc_prepare_batch has not succeeded");
+  HIP_TRY(hipSetDevice(h->device));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  int64_t nt = (int64_t)h->n_items * per_item(h);
+  if (tally) HIP_TRY(hipMemcpy(tally, h->dOut, nt * sizeof(double), hipMemcpyDeviceToHost));
+  if (totals)
+    HIP_TRY(hipMemcpy(totals, h->dOut + nt, h->n_items * sizeof(nmc_totals), hipMemcpyDeviceToHost));
   return ok();
 }
 
@@ -504,8 +823,8 @@ void nmc_destroy(nmc_handle* h) {
   (void)hipSetDevice(h->device);
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   free_buffers(h);
-  if (h->dTot) (void)hipFree(h->dTot);
-  if (h->dCounter) (void)hipFree(h->dCounter);
+  free_ptr(h->dTot);
+  free_ptr(h->dState);
   if (h->e0) (void)hipEventDestroy(h->e0);
   if (h->e1) (void)hipEventDestroy(h->e1);
   if (h->stream) (void)hipStreamDestroy(h->stream);

@@ -28,6 +28,13 @@ struct XyzTissue {
   TTCoef tt;
 };
 
+// Photon-queue state read by the kernels from device memory, so that a
+// captured graph can advance `first` between replays.
+struct RunState {
+  unsigned long long counter;       // next queue index (zeroed before each launch)
+  unsigned long long first;         // photon id of queue index 0 (per item)
+};
+
 struct XyzArgs {
   int Nx, Ny, Nz;
   float dx, dy, dz, hx, hy;
@@ -37,45 +44,61 @@ struct XyzArgs {
   float thr, chance;
   uint32_t max_events;
   uint32_t key0, key1;
-  uint64_t first, n;
-  uint32_t chunk;
+  uint64_t n;                       // photons in this launch
+  uint32_t chunk;                   // queue indices taken per wave refill
+  uint32_t refill_min;              // refill once this many lanes of a wave are idle
+  int tile_x0, tile_y0, tile_z0;    // LDS fluence tile origin (voxel indices)
+  int tile_nx, tile_ny, tile_nz;    // tile extent; 0 = no tile
   const uint8_t* V;
   const XyzTissue* tissue;
   double* F;                        // may be null (NMC_FLAG_NO_FLUENCE)
+  double* tile_part;                // [grid][tile voxels] block partials of the tile
   double* ryx;
   uint32_t* det2d;
   nmc_totals* tot;
-  unsigned long long* counter;      // photon-id dispenser, zeroed before launch
+  RunState* state;
+};
+
+// One medium of a (batched) mcsub launch.
+struct SubMedium {
+  float mut, one_m_albedo, g, n1, n2, rsp0;
+  uint32_t key1;
+  int pad;
 };
 
 struct SubArgs {
-  float mut, one_m_albedo, g, n1, n2, rsp0;
   int mcflag, bmode, nr, nz, na;
   float dr, dz, radius, waist, zfocus, xs, ys, zs;
   float thr, chance;
   uint32_t max_events;
-  uint32_t key0, key1;
-  uint64_t first, n;
+  uint32_t key0;
   uint32_t chunk;
+  int n_items;
+  const SubMedium* items;           // [n_items]
+  const uint64_t* prefix;           // [n_items+1] queue index where item j starts
   const float* cos_tab;             // na-1 upper bin-edge cosines
   int tally_rz;                     // 0: skip the A(r,z) tally
-  int lds_tallies;                  // 1: tallies privatised in LDS, flushed to `part`
-  double* part;                     // [grid][(nr+1)*na + (nr+1)*(nz+1)] block partials
-  double* rd_ra;                    // direct global tallies when lds_tallies == 0
-  double* a_rz;
-  nmc_totals* tot;
-  unsigned long long* counter;
+  double* tally;                    // [n_items][(nr+1)*na + (nr+1)*(nz+1)] (global mode)
+  nmc_totals* tot;                  // [n_items] (global mode)
+  double* part;                     // block partials (LDS mode)
+  RunState* state;
 };
 
-constexpr int kXyzBlock = 256;
 constexpr int kSubBlock = 1024;
 
-hipError_t launch_mcxyz(const XyzArgs& a, bool two_term, int grid, hipStream_t s);
-hipError_t launch_mcsub(const SubArgs& a, int grid, int lds_bytes, hipStream_t s);
-hipError_t launch_reduce_partials(const double* part, int grid, int64_t stride, int64_t len,
-                                  double* out, hipStream_t s);
-hipError_t launch_zero_u64(unsigned long long* p, int n, hipStream_t s);
-int mcxyz_blocks_per_cu(bool two_term);
-int mcsub_blocks_per_cu(int lds_bytes);
+hipError_t launch_mcxyz(const XyzArgs& a, bool two_term, int block, int grid, size_t lds_bytes,
+                        hipStream_t s);
+hipError_t mcxyz_configure(bool two_term, int block, size_t lds_bytes);
+int mcxyz_blocks_per_cu(bool two_term, int block, size_t lds_bytes);
+hipError_t launch_tile_reduce(const double* part, int grid, const XyzArgs& a, hipStream_t s);
+hipError_t launch_mcsub(const SubArgs& a, int grid, size_t lds_bytes, bool lds_tallies,
+                        hipStream_t s);
+hipError_t mcsub_configure(size_t lds_bytes, bool lds_tallies);
+int mcsub_blocks_per_cu(size_t lds_bytes, bool lds_tallies);
+size_t mcsub_lds_bytes(int n_items, int64_t per_item);
+hipError_t launch_reduce_partials(const double* part, int grid, int64_t stride, double* out,
+                                  int ni, int per_item, hipStream_t s);
+hipError_t launch_state_begin(RunState* st, hipStream_t s);
+hipError_t launch_state_advance(RunState* st, uint64_t adv, hipStream_t s);
 
 }  // namespace nmc

@@ -73,6 +73,50 @@ class MetalMC:
     def reset_tallies(self) -> None:
         check(lib().nmc_reset_tallies(self._h))
 
+    # -- launch tuning / graphs ---------------------------------------------
+    def set_tuning(self, tuning: P.Tuning) -> None:
+        check(lib().nmc_set_tuning(self._h, C.byref(tuning)))
+
+    def get_tuning(self) -> P.Tuning:
+        t = P.Tuning()
+        check(lib().nmc_get_tuning(self._h, C.byref(t)))
+        return t
+
+    def set_graph_mode(self, enable: bool) -> None:
+        check(lib().nmc_set_graph_mode(self._h, int(bool(enable))))
+
+    # -- batched mcsub (many media, one launch) -------------------------------
+    def prepare_batch(self, run_params: P.RunParams, items, ext: P.ExtConfig | None = None) -> None:
+        """items: sequence of WorkItem (or dicts with its fields)."""
+        self.rp = run_params
+        self.ext = ext if ext is not None else P.ExtConfig.default()
+        arr = (P.WorkItem * len(items))()
+        for i, it in enumerate(items):
+            if isinstance(it, dict):
+                for k, v in it.items():
+                    setattr(arr[i], k, v)
+            else:
+                C.memmove(C.byref(arr[i]), C.byref(it), C.sizeof(P.WorkItem))
+        self._items = arr
+        check(lib().nmc_prepare_batch(self._h, C.byref(run_params), C.byref(self.ext), arr, len(items)))
+
+    def run_batch(self) -> None:
+        check(lib().nmc_run_batch(self._h))
+
+    def run_batch_async(self) -> None:
+        check(lib().nmc_run_batch_async(self._h))
+
+    def get_batch_results(self):
+        """(tally[n_items, per_item] float64, list of totals dicts)."""
+        n = len(self._items)
+        per = (self.ext.nr + 1) * self.ext.na
+        if not self.ext.flags & P.FLAG_NO_FLUENCE:
+            per += (self.ext.nr + 1) * (self.ext.nz + 1)
+        tally = np.zeros((n, per), np.float64)
+        tots = (P.Totals * n)()
+        check(lib().nmc_get_batch_results(self._h, tally.ctypes.data_as(C.POINTER(C.c_double)), tots))
+        return tally, [t.as_dict() for t in tots]
+
     def close(self) -> None:
         if self._h:
             lib().nmc_destroy(self._h)

@@ -120,8 +120,36 @@ class Results(C.Structure):
     ]
 
 
+class Tuning(C.Structure):
+    """nmc_tuning: launch shape (block, blocks/CU, refill batch, queue chunk) and
+    the mcxyz LDS fluence tile box (x0, y0, z0, nx, ny, nz)."""
+
+    _fields_ = [("block", C.c_int32), ("blocks_per_cu", C.c_int32), ("refill_min", C.c_int32),
+                ("chunk", C.c_int32), ("tile", C.c_int32 * 6), ("reserved", C.c_int32 * 6)]
+
+    def as_dict(self) -> dict:
+        return {"block": self.block, "blocks_per_cu": self.blocks_per_cu,
+                "refill_min": self.refill_min, "chunk": self.chunk, "tile": list(self.tile)}
+
+    @classmethod
+    def make(cls, block=0, blocks_per_cu=0, refill_min=0, chunk=0, tile=None) -> "Tuning":
+        t = cls()
+        t.block, t.blocks_per_cu, t.refill_min, t.chunk = block, blocks_per_cu, refill_min, chunk
+        for i, v in enumerate(tile or (0,) * 6):
+            t.tile[i] = v
+        return t
+
+
+class WorkItem(C.Structure):
+    """nmc_work_item: one medium of a batched mcsub launch."""
+
+    _fields_ = [("mua", C.c_float), ("mus", C.c_float), ("g", C.c_float), ("n1", C.c_float),
+                ("n2", C.c_float), ("stream_id", C.c_uint32), ("n_photons", C.c_uint64)]
+
+
 assert C.sizeof(RunParams) == 172
 assert C.sizeof(TissueParams) == 608
+assert C.sizeof(WorkItem) == 32
 
 
 def dptr(a: np.ndarray | None):

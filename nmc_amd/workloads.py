@@ -102,6 +102,37 @@ def mcsub_semi_infinite(n_photons: int = 1_000_000, seed: int = 1) -> Workload:
     return Workload("mcsub_semi_infinite", P.SIM_MCSUB, rp, tp, None, ext, n_photons)
 
 
+def spectral_sweep(n_wavelengths: int = 128, seed: int = 3, budget_scale: float = 1.0):
+    """128-wavelength sweep on the semi-infinite mcsub geometry.
+
+    lambda_i = 400 + 500 i/127 nm.  From numpy default_rng(seed): mus(lambda) =
+    a (lambda/500)^-b with a ~ U(50, 300) /cm and b ~ U(0.5, 2); mua(lambda) is
+    a sum of three Gaussian peaks (centres U(400, 900) nm, widths U(10, 60) nm,
+    heights U(50, 300) /cm) scaled so its maximum is at most 300 /cm; budgets
+    floor(10^U(5, 7)) photons.  g = 0.9, n = 1.33 against air, pencil beam.
+    Returns (Workload of the geometry, list of WorkItem dicts, wavelengths).
+    """
+    rng = np.random.default_rng(seed)
+    lam = 400.0 + 500.0 * np.arange(n_wavelengths) / max(1, n_wavelengths - 1)
+    a = rng.uniform(50, 300)
+    b = rng.uniform(0.5, 2.0)
+    mus = a * (lam / 500.0) ** (-b)
+    c = rng.uniform(400, 900, 3)
+    wd = rng.uniform(10, 60, 3)
+    amp = rng.uniform(50, 300, 3)
+    mua = (amp[None, :] * np.exp(-(lam[:, None] - c[None, :]) ** 2 / (2 * wd[None, :] ** 2))).sum(1)
+    if mua.max() > 300.0:
+        mua *= 300.0 / mua.max()
+    budgets = np.floor(10 ** rng.uniform(5, 7, n_wavelengths) * budget_scale).astype(np.int64)
+    w = mcsub_semi_infinite()
+    w.ext.na = 1
+    w.ext.flags = P.FLAG_NO_FLUENCE
+    items = [dict(mua=float(np.float32(mua[i])), mus=float(np.float32(mus[i])), g=0.9, n1=1.33,
+                  n2=1.0, stream_id=i, n_photons=int(max(1, budgets[i])))
+             for i in range(n_wavelengths)]
+    return w, items, lam
+
+
 def tiny_voxel_case(kind: str = "mismatch", n_photons: int = 2000, seed: int = 7,
                     n: int = 16) -> Workload:
     """Small mcxyz cases for parity: every launch mode / boundary mode can be set

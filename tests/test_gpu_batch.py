@@ -1,0 +1,134 @@
+"""GPU: batched mcsub launches, graph replays and launch tuning.
+
+A batch item must reproduce, photon for photon, the plain run of its medium
+(integer tallies bit-exact against the oracle with the item's stream); graph
+replays must equal consecutive plain runs; and no launch-tuning choice (block
+size, refill batch, queue chunk, LDS tile box) may change any result.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import sys
+
+import numpy as np
+import pytest
+
+from nmc_amd import params as P
+from nmc_amd import workloads as W
+from nmc_amd.metal_mc import MetalMC, simulate
+from oracle import oracle as O
+
+from parity_util import RTOL, assert_same_tallies
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _oracle_item(w, it, n):
+    """Oracle run of one work item as a plain mcsub configuration."""
+    w.rp.n1, w.rp.n2 = it["n1"], it["n2"]
+    w.tp.muav[1], w.tp.musv[1], w.tp.gv[1] = it["mua"], it["mus"], it["g"]
+    w.ext.stream_id = it["stream_id"]
+    return O.run(w, n, threads=4)
+
+
+def test_batch_items_match_oracle(gpu):
+    w, items, _ = W.spectral_sweep(n_wavelengths=24, budget_scale=0.003)
+    items[3]["g"] = 0.0
+    items[5]["n1"] = 1.0
+    w.ext.flags = 0                      # keep A(r,z) too
+    w.ext.na = 4
+    _say(f"[batch] 24 items, {sum(i['n_photons'] for i in items)} photons")
+    with MetalMC(P.SIM_MCSUB) as mc:
+        mc.prepare_batch(w.rp, items, w.ext)
+        mc.run_batch()
+        tally, tots = mc.get_batch_results()
+    n_ra = (w.ext.nr + 1) * w.ext.na
+    for j in (0, 3, 5, 11, 23):
+        it = items[j]
+        ww = W.mcsub_semi_infinite()
+        ww.ext.na = 4
+        ref = _oracle_item(ww, it, it["n_photons"])
+        for k in P.TOTAL_INT_FIELDS:
+            assert tots[j][k] == ref.totals[k], (j, k, tots[j][k], ref.totals[k])
+        for k in P.TOTAL_FLOAT_FIELDS:
+            assert np.isclose(tots[j][k], ref.totals[k], rtol=RTOL, atol=1e-9), (j, k)
+        np.testing.assert_allclose(tally[j, :n_ra], ref.rd_ra, rtol=RTOL, atol=1e-12)
+        np.testing.assert_allclose(tally[j, n_ra:], ref.a_rz, rtol=RTOL, atol=1e-12)
+
+
+def test_batch_uneven_budgets_all_consumed(gpu):
+    w, items, _ = W.spectral_sweep(n_wavelengths=128, budget_scale=0.002)
+    _say(f"[batch] 128 items, {sum(i['n_photons'] for i in items)} photons")
+    with MetalMC(P.SIM_MCSUB) as mc:
+        mc.prepare_batch(w.rp, items, w.ext)
+        mc.run_batch()
+        _, tots = mc.get_batch_results()
+    for j, it in enumerate(items):
+        assert tots[j]["photons"] == it["n_photons"], j
+        bal = (tots[j]["specular"] + tots[j]["rd"] + tots[j]["absorbed"] + tots[j]["roulette_loss"]
+               + tots[j]["capped"] - tots[j]["roulette_gain"]) / it["n_photons"]
+        assert abs(bal - 1.0) < 1e-5, j
+
+
+def test_plain_mcsub_graph_replay(gpu):
+    w = W.mcsub_semi_infinite()
+    _say("[graph] mcsub 3 replays x 20000")
+    with MetalMC(w.sim) as mc:
+        mc.prepare_data(w.rp, w.tp, None, w.ext)
+        mc.set_graph_mode(True)
+        for _ in range(3):
+            mc.send_compute_command(20000)
+        g = mc.get_compute_results()
+    ref = O.run(w, 60000, threads=8)
+    assert_same_tallies(g, ref, "graph")
+
+
+def test_mcxyz_graph_replay(gpu):
+    w = W.skin_vessel()
+    _say("[graph] mcxyz 2 replays x 10000")
+    with MetalMC(w.sim) as mc:
+        mc.prepare_data(w.rp, w.tp, w.v, w.ext)
+        mc.set_graph_mode(True)
+        mc.send_compute_command(10000)
+        mc.send_compute_command(10000)
+        g = mc.get_compute_results()
+    ref = O.run(w, 20000, threads=8)
+    assert_same_tallies(g, ref, "graph-xyz")
+
+
+TUNINGS = [
+    P.Tuning.make(block=256, refill_min=1, tile=(0, 0, 0, -1, 0, 0)),
+    P.Tuning.make(block=512, refill_min=8, chunk=64),
+    P.Tuning.make(block=1024, refill_min=32, chunk=1000, tile=(90, 95, 1, 20, 10, 30)),
+    P.Tuning.make(block=1024, refill_min=64, blocks_per_cu=1),
+]
+
+
+@pytest.mark.parametrize("ti", range(len(TUNINGS)))
+def test_mcxyz_tuning_invariance(gpu, ti):
+    w = W.skin_vessel()
+    _say(f"[tuning] {TUNINGS[ti].as_dict()}")
+    ref = O.run(w, 15000, threads=8)
+    with MetalMC(w.sim) as mc:
+        mc.prepare_data(w.rp, w.tp, w.v, w.ext)
+        mc.set_tuning(TUNINGS[ti])
+        mc.set_num_runs_seed(1, 0)
+        mc.send_compute_command(15000)
+        got = mc.get_compute_results()
+        eff = mc.get_tuning()
+    assert eff.block == TUNINGS[ti].block
+    assert_same_tallies(got, ref, f"tuning {ti}")
+
+
+def test_default_tile_is_used(gpu):
+    w = W.skin_vessel(n_photons=1000)
+    with MetalMC(w.sim) as mc:
+        mc.prepare_data(w.rp, w.tp, w.v, w.ext)
+        t = mc.get_tuning()
+    _say(f"[tuning] default {t.as_dict()}")
+    assert t.tile[3] * t.tile[4] * t.tile[5] > 1000
