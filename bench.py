@@ -93,7 +93,7 @@ def main() -> int:
     ops = OPS_SAME * tot["substeps_same"] + OPS_CROSS * tot["substeps_cross"] \
         + OPS_INT * tot["interactions"]
     achieved = ops / (sum(kern_ms) * 1e-3) / 1e12
-    atomics = tot["substeps_cross"] + tot["photons"] + 2 * tot["detections"]
+    visits = tot["substeps_cross"] + tot["photons"]   # voxel-visit flushes (LDS tile or global)
     rd = tot["rd"] / tot["photons"]
 
     out = {
@@ -111,7 +111,8 @@ def main() -> int:
                      "events_per_photon": {"same": tot["substeps_same"] / tot["photons"],
                                            "cross": tot["substeps_cross"] / tot["photons"],
                                            "interactions": tot["interactions"] / tot["photons"]},
-                     "fp64_atomics_per_s": atomics / (sum(kern_ms) * 1e-3)},
+                     "voxel_flushes_per_s": visits / (sum(kern_ms) * 1e-3),
+                     "tuning": mc.get_tuning().as_dict()},
         "rd": rd,
         "energy_balance": (tot["rd"] + tot["rd_outside"] + tot["absorbed"] + tot["tt"]
                            + tot["lateral"] + tot["top"] + tot["roulette_loss"] + tot["capped"]
