@@ -243,6 +243,70 @@ int nmc_get_batch_results(nmc_handle* h, double* tally, nmc_totals* totals);
  * with the same photon count and seed. */
 int nmc_set_graph_mode(nmc_handle* h, int enable);
 
+/*
+ * Layered media with per-layer refractive index and gradients (no reference
+ * counterpart: NMC has a single global n1/n2 used only at zsurf,
+ * definitions.h:196, kernels.metal:579).  Semantics follow the mcxyz loop
+ * (continuous absorption, W *= exp(-mua s)) with all-or-none Fresnel
+ * reflection / Snell refraction at every interface.  Light enters at the
+ * origin along +z (pencil); Rd(r) counts weight leaving through the top.
+ */
+#define NMC_SIM_LAYERED 3
+#define NMC_MAX_LAYERS 8
+
+typedef struct nmc_layer {
+    float thickness;              /* cm; last layer: <= 0 means semi-infinite */
+    float mua, mus, n;            /* /cm, /cm, refractive index               */
+    float g;                      /* HG anisotropy (phase == 0)               */
+    float gf, af, gb, ab, cc;     /* two-term parameters (phase == 1), as the
+                                     gf/af/gb/ab/CC of TissueParams           */
+    int32_t phase;                /* 0 HG, 1 two-term                          */
+    int32_t reserved;
+} nmc_layer;
+
+typedef struct nmc_stack {
+    int32_t n_layers;             /* 1..NMC_MAX_LAYERS                         */
+    float n_above, n_below;       /* ambient indices above / below the stack  */
+    uint32_t stream_id;           /* Philox stream of this stack              */
+    uint64_t n_photons;           /* budget per nmc_layered_run               */
+    nmc_layer layer[NMC_MAX_LAYERS];
+} nmc_stack;
+
+typedef struct nmc_layered_config {
+    uint32_t struct_size;         /* sizeof(nmc_layered_config)               */
+    uint32_t version;             /* NMC_EXT_VERSION                          */
+    uint64_t seed;
+    int32_t nr;                   /* Rd(r) bins; bin nr is the overflow bin   */
+    float dr;
+    float threshold, chance;      /* roulette                                 */
+    uint32_t max_events;
+    int32_t records;              /* 1: keep per-detected-photon path records */
+    uint64_t record_capacity;     /* 0: sum of the budgets                    */
+    int32_t reserved[8];
+} nmc_layered_config;
+
+/* Allocates tallies (and the record buffer) for n_stacks stacks and uploads
+ * them on an NMC_SIM_LAYERED handle. */
+int nmc_layered_prepare(nmc_handle* h, const nmc_layered_config* cfg, const nmc_stack* stacks,
+                        int n_stacks);
+/* New optical properties / budgets for the same stack shapes (optimisers). */
+int nmc_layered_update(nmc_handle* h, const nmc_stack* stacks, int n_stacks);
+/* Zeroes tallies and records, then transports photons [first, first + n_j)
+ * of every stack j in one launch (first from nmc_set_seed, default 0; it is
+ * not advanced, so repeated runs reuse the same photons). */
+int nmc_layered_run(nmc_handle* h);
+/* rd: n_stacks x (nr+1) raw escaping weight; absorbed: n_stacks x
+ * NMC_MAX_LAYERS; totals: n_stacks; n_records: records stored (any may be NULL). */
+int nmc_layered_results(nmc_handle* h, double* rd, double* absorbed, nmc_totals* totals,
+                        uint64_t* n_records);
+/* Perturbation-MC backward pass over the stored records: given
+ * grad_rd[j][ir] = dLoss/d(rd[j][ir] / n_photons_j), returns
+ * grad_mua[j][l] = sum over records of grad_rd * w * (-L_l) / n_j and
+ * grad_mus[j][l] = sum of grad_rd * w * (K_l / mus_l - L_l) / n_j, where L_l and
+ * K_l are the photon's path length and scattering count in layer l. */
+int nmc_layered_backward(nmc_handle* h, const double* grad_rd, double* grad_mua,
+                         double* grad_mus);
+
 /* Message of the last failing call on this host thread ("" if none). */
 const char* nmc_last_error(void);
 

@@ -55,6 +55,12 @@ def lib():
         "nmc_run_batch_async": (C.c_int, [H]),
         "nmc_get_batch_results": (C.c_int, [H, C.POINTER(C.c_double), C.POINTER(P.Totals)]),
         "nmc_set_graph_mode": (C.c_int, [H, C.c_int]),
+        "nmc_layered_prepare": (C.c_int, [H, C.POINTER(P.LayeredConfig), C.POINTER(P.Stack), C.c_int]),
+        "nmc_layered_update": (C.c_int, [H, C.POINTER(P.Stack), C.c_int]),
+        "nmc_layered_run": (C.c_int, [H]),
+        "nmc_layered_results": (C.c_int, [H, C.c_void_p, C.c_void_p, C.POINTER(P.Totals),
+                                          C.POINTER(C.c_uint64)]),
+        "nmc_layered_backward": (C.c_int, [H, C.c_void_p, C.c_void_p, C.c_void_p]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(L, name)
@@ -69,7 +75,8 @@ EXPORTED = ("nmc_ext_config_init", "nmc_validate", "nmc_create", "nmc_prepare", 
             "nmc_get_results", "nmc_reset_tallies", "nmc_destroy", "nmc_last_error",
             "nmc_version", "nmc_set_tuning", "nmc_get_tuning", "nmc_prepare_batch",
             "nmc_run_batch", "nmc_run_batch_async", "nmc_get_batch_results",
-            "nmc_set_graph_mode")
+            "nmc_set_graph_mode", "nmc_layered_prepare", "nmc_layered_update", "nmc_layered_run",
+            "nmc_layered_results", "nmc_layered_backward")
 
 
 def check(status: int) -> None:

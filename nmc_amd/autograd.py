@@ -1,0 +1,43 @@
+"""torch.autograd wrapper of the layered transport (PyTorch is used only here).
+
+    rd = LayeredRd.apply(mua, mus, runner)
+
+mua, mus: float64 tensors [n_stacks, n_layers]; rd: float64 tensor
+[n_stacks, nr+1], Rd(r) per launched photon.  forward() uploads the
+properties and runs the transport (keeping the per-photon path records in
+HBM); backward() runs the perturbation-MC backward kernel over those records.
+The same photons are used by every forward call unless the runner's seed or
+first photon id is changed (common random numbers across optimisation steps).
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from .layered import LayeredMC
+
+
+class LayeredRd(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, mua: torch.Tensor, mus: torch.Tensor, runner: LayeredMC):
+        nl = mua.shape[1]
+        full_mua = np.zeros((runner.n_stacks, 8))
+        full_mus = np.zeros((runner.n_stacks, 8))
+        full_mua[:, :nl] = mua.detach().cpu().double().numpy()
+        full_mus[:, :nl] = mus.detach().cpu().double().numpy()
+        runner.set_properties(full_mua, full_mus)
+        runner.run()
+        ctx.runner = runner
+        ctx.nl = nl
+        return torch.from_numpy(runner.rd_per_photon()).to(mua.device)
+
+    @staticmethod
+    def backward(ctx, grad_rd: torch.Tensor):
+        gmua, gmus = ctx.runner.backward(grad_rd.detach().cpu().double().numpy())
+        nl = ctx.nl
+        dev = grad_rd.device
+        return (torch.from_numpy(gmua[:, :nl]).to(dev), torch.from_numpy(gmus[:, :nl]).to(dev), None)
+
+
+def layered_rd(mua: torch.Tensor, mus: torch.Tensor, runner: LayeredMC) -> torch.Tensor:
+    return LayeredRd.apply(mua, mus, runner)

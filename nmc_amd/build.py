@@ -16,7 +16,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libnmc_hip.so")
-SOURCES = ["nmc_abi.hip", "mcxyz.hip", "mcsub.hip"]
+SOURCES = ["nmc_abi.hip", "mcxyz.hip", "mcsub.hip", "layered.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("NMC_OFFLOAD_ARCH", "gfx950")
 

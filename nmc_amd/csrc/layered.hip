@@ -1,0 +1,392 @@
+// layered.hip - layered media with per-layer refractive index, batched over
+// stacks, with perturbation-MC path records and the backward kernel.
+//
+// The walk is the mcxyz loop (kernels.metal:493-750: continuous absorption
+// W *= exp(-mua s), HG or two-term spin, roulette) with the voxel faces
+// replaced by layer interfaces, where the photon is reflected or refracted
+// all-or-none with Fresnel probability (the scheme of the reference's mcsub
+// surface, kernels.metal:190-214).  Each photon keeps its path length L_l
+// and scattering count K_l per layer; when it leaves through the top its
+// (stack, Rd bin, weight, L, K) record is appended to an HBM buffer, from
+// which the backward kernel forms dLoss/dmua_l = -sum g w L_l and
+// dLoss/dmus_l = sum g w (K_l/mus_l - L_l).
+//
+// Launch model as mcxyz.hip: persistent wave64 workgroups, one photon per
+// lane, batched lane refills from a device queue spanning every stack's
+// budget (queue index -> stack by binary search over budget prefix sums).
+#include <algorithm>
+
+#include "nmc_internal.h"
+
+namespace nmc {
+
+namespace {
+
+__device__ __forceinline__ int rbin(float q, int n) { return q >= (float)n ? n : (int)q; }
+
+__device__ __forceinline__ void addf(nmc_totals* t, int k, double v) {
+  unsafeAtomicAdd(&((double*)t)[k], v);
+}
+__device__ __forceinline__ void addi(nmc_totals* t, int k, unsigned long long v) {
+  atomicAdd(&(&t->photons)[k], v);
+}
+
+struct LPhoton {
+  float x, y, z, ux, uy, uz, ruz, W, sleft;
+  float zt, zb, mua, mus;
+  float Lcur;
+  int item, l, nl, row0;
+  uint32_t Kcur;
+  uint64_t pid;
+  uint32_t ev, events, hits, b1, b2, b3;
+  uint32_t n_same, n_cross, n_int;
+  double acur;
+  bool alive;
+  float L[NMC_MAX_LAYERS];
+  uint32_t K[NMC_MAX_LAYERS];
+};
+
+__device__ __forceinline__ void load_layer(LPhoton& p, const LayRow* rows) {
+  const LayRow& r = rows[p.row0 + p.l];
+  p.zt = r.zt;
+  p.zb = r.zb;
+  p.mua = r.mua;
+  p.mus = r.mus;
+}
+
+// Folds the current layer's path length, scatter count and absorption into
+// the per-layer records / LDS absorption tally.
+__device__ __forceinline__ void fold_layer(LPhoton& p, double* absorbed) {
+#pragma unroll
+  for (int i = 0; i < NMC_MAX_LAYERS; ++i) {
+    if (i == p.l) {
+      p.L[i] = p.L[i] + p.Lcur;
+      p.K[i] += p.Kcur;
+    }
+  }
+  if (p.acur != 0.0) unsafeAtomicAdd(&absorbed[p.item * NMC_MAX_LAYERS + p.l], p.acur);
+  p.Lcur = 0.0f;
+  p.Kcur = 0u;
+  p.acur = 0.0;
+}
+
+__global__ __launch_bounds__(kLayBlock) void layered_kernel(LayArgs a) {
+  extern __shared__ double smem[];
+  // [items] absorbed[MAXL] | [items] totals | [items] headers | [items+1] prefix | [rows]
+  const int ni = a.n_items;
+  double* absorbed = smem;
+  nmc_totals* st = (nmc_totals*)(absorbed + ni * NMC_MAX_LAYERS);
+  LayStackDev* hdr = (LayStackDev*)(st + ni);
+  uint64_t* prefix = (uint64_t*)(hdr + ni);
+  LayRow* rows = (LayRow*)(prefix + ni + 1);
+  for (int i = threadIdx.x; i < ni * (NMC_MAX_LAYERS + TF_COUNT + TI_COUNT); i += kLayBlock) smem[i] = 0.0;
+  for (int i = threadIdx.x; i < ni; i += kLayBlock) hdr[i] = a.stacks[i];
+  for (int i = threadIdx.x; i <= ni; i += kLayBlock) prefix[i] = a.prefix[i];
+  for (int i = threadIdx.x; i < a.n_rows; i += kLayBlock) rows[i] = a.rows[i];
+  __syncthreads();
+  const uint64_t first = a.state->first;
+  const uint64_t total = prefix[ni];
+
+  const int lane = lane_id();
+  LPhoton p;
+  p.alive = false;
+  p.item = 0;
+  p.l = 0;
+  p.acur = 0.0;
+  bool finished = false;
+  uint64_t pool_next = 0, pool_end = 0;
+
+  for (;;) {
+    uint64_t idle = ballot(!p.alive && !finished);
+    if (idle && ((uint32_t)__popcll(idle) >= a.refill_min || ballot(p.alive) == 0ull)) {
+      while (idle) {
+        if (pool_next >= pool_end) {
+          unsigned long long base = 0;
+          if (lane == 0) base = atomicAdd(&a.state->counter, (unsigned long long)a.chunk);
+          base = __shfl(base, 0);
+          if (base >= total) {
+            finished = finished || !p.alive;
+            break;
+          }
+          pool_next = base;
+          pool_end = base + a.chunk < total ? base + a.chunk : total;
+        }
+        uint64_t avail = pool_end - pool_next;
+        uint64_t cnt = (uint64_t)__popcll(idle);
+        uint32_t take = (uint32_t)(cnt < avail ? cnt : avail);
+        uint32_t rank = (uint32_t)__popcll(idle & lanemask_lt());
+        if (((idle >> lane) & 1ull) && rank < take) {
+          uint64_t q = pool_next + rank;
+          int lo = 0, hi = ni - 1;
+          while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (prefix[mid] <= q) lo = mid;
+            else hi = mid - 1;
+          }
+          const LayStackDev& H = hdr[lo];
+          p.item = lo;
+          p.nl = H.nl;
+          p.row0 = H.row0;
+          p.pid = first + (q - prefix[lo]);
+          // LAUNCH: pencil beam at the origin along +z, specular removed
+          addi(&st[lo], TI_PHOTONS, 1);
+          addf(&st[lo], TF_SPECULAR, (double)H.rsp);
+          p.W = 1.0f - H.rsp;
+          p.x = p.y = p.z = 0.0f;
+          p.ux = p.uy = 0.0f;
+          p.uz = 1.0f;
+          p.ruz = 1.0f / p.uz;
+          p.l = 0;
+          load_layer(p, rows);
+          p.hits = 0;
+          p.n_same = p.n_cross = p.n_int = 0;
+          p.ev = 1;
+          p.events = 0;
+#pragma unroll
+          for (int i = 0; i < NMC_MAX_LAYERS; ++i) {
+            p.L[i] = 0.0f;
+            p.K[i] = 0u;
+          }
+          p.Lcur = 0.0f;
+          p.Kcur = 0u;
+          p.acur = 0.0;
+          U4 b = philox(p.pid, 1u, 0u, a.key0, H.key1);
+          p.b1 = b.y;
+          p.b2 = b.z;
+          p.b3 = b.w;
+          p.sleft = -log_pos(u01nz(b.x));
+          p.alive = true;
+        }
+        pool_next += take;
+        idle = ballot(!p.alive && !finished);
+      }
+    }
+    if (ballot(p.alive) == 0ull) break;
+    if (!p.alive) continue;
+
+    nmc_totals* T = &st[p.item];
+    // ---- HOP to the next interaction or interface ----
+    float s_int = p.sleft / p.mus;
+    float s_b = kInf;
+    if (p.uz > 0.0f) s_b = fmaxf((p.zb - p.z) * p.ruz, 0.0f);
+    else if (p.uz < 0.0f) s_b = fmaxf((p.zt - p.z) * p.ruz, 0.0f);
+    bool inter = s_int <= s_b;
+    float s = inter ? s_int : s_b;
+    p.x = fmaf(s, p.ux, p.x);
+    p.y = fmaf(s, p.uy, p.y);
+    p.z = fmaf(s, p.uz, p.z);
+    float em = expm1_f(-(p.mua * s));
+    float dep = -(p.W * em);
+    p.W = p.W - dep;
+    p.acur += (double)dep;
+    p.Lcur = p.Lcur + s;
+    p.events++;
+    bool alive = true;
+    if (inter) {
+      p.sleft = 0.0f;
+      p.n_same++;
+    } else {
+      p.n_cross++;
+      p.sleft = p.sleft - s * p.mus;
+      if (!(p.sleft > 0.0f)) p.sleft = 0.0f;
+      bool up = p.uz < 0.0f;
+      p.z = up ? p.zt : p.zb;
+      const LayStackDev& H = hdr[p.item];
+      float ni_ = rows[p.row0 + p.l].n;
+      float nt = up ? (p.l == 0 ? H.n_above : rows[p.row0 + p.l - 1].n)
+                    : (p.l == p.nl - 1 ? H.n_below : rows[p.row0 + p.l + 1].n);
+      float ca2;
+      float r = fresnel(ni_, nt, fabsf(p.uz), &ca2);
+      U4 aux = philox(p.pid, p.hits, 2u, a.key0, H.key1);
+      p.hits++;
+      if (u01(aux.x) < r) {
+        p.uz = -p.uz;
+        p.ruz = -p.ruz;
+      } else if (up && p.l == 0) {
+        // leaves through the top: Rd(r) and the path record
+        fold_layer(p, absorbed);
+        float re = sqrtf(p.x * p.x + p.y * p.y);
+        int ir = rbin(re / a.dr, a.nr);
+        unsafeAtomicAdd(&a.rd[(int64_t)p.item * (a.nr + 1) + ir], (double)p.W);
+        addf(T, TF_RD, (double)p.W);
+        addi(T, TI_DETECTIONS, 1);
+        if (a.records) {
+          unsigned long long k = atomicAdd(a.rec_count, 1ull);
+          if (k < a.cap) {
+            a.rec_item[k] = (uint32_t)p.item;
+            a.rec_bin[k] = (uint32_t)ir;
+            a.rec_w[k] = p.W;
+#pragma unroll
+            for (int i = 0; i < NMC_MAX_LAYERS; ++i) {
+              a.rec_L[(uint64_t)i * a.cap + k] = p.L[i];
+              a.rec_K[(uint64_t)i * a.cap + k] = p.K[i];
+            }
+          }
+        }
+        alive = false;
+      } else if (!up && p.l == p.nl - 1) {
+        fold_layer(p, absorbed);
+        addf(T, TF_TT, (double)p.W);
+        addi(T, TI_ESC_BOTTOM, 1);
+        alive = false;
+      } else {
+        fold_layer(p, absorbed);
+        if (ni_ != nt) {
+          p.ux = p.ux * ni_ / nt;
+          p.uy = p.uy * ni_ / nt;
+          p.uz = up ? -ca2 : ca2;
+          p.ruz = 1.0f / p.uz;
+        }
+        p.l += up ? -1 : 1;
+        load_layer(p, rows);
+      }
+    }
+    if (alive && p.events >= a.max_events) {
+      addf(T, TF_CAPPED, (double)p.W);
+      addi(T, TI_CAPPED, 1);
+      alive = false;
+    }
+    if (alive && !(p.sleft > 0.0f)) {
+      const LayRow& R = rows[p.row0 + p.l];
+      float ct = R.phase ? tt_cos(R.tt, u01(p.b1)) : hg_cos(R.g, u01(p.b1));
+      spin(ct, u01(p.b2), p.ux, p.uy, p.uz);
+      p.ruz = 1.0f / p.uz;
+      p.Kcur++;
+      p.n_int++;
+      if (p.W < a.thr) {
+        if (u01(p.b3) <= a.chance) {
+          float nw = p.W / a.chance;
+          addf(T, TF_ROULETTE_GAIN, (double)nw - (double)p.W);
+          addi(T, TI_ROULETTE_SURVIVALS, 1);
+          p.W = nw;
+        } else {
+          addf(T, TF_ROULETTE_LOSS, (double)p.W);
+          addi(T, TI_ROULETTE_KILLS, 1);
+          alive = false;
+        }
+      }
+      if (alive) {
+        p.ev++;
+        U4 b = philox(p.pid, p.ev, 0u, a.key0, hdr[p.item].key1);
+        p.b1 = b.y;
+        p.b2 = b.z;
+        p.b3 = b.w;
+        p.sleft = -log_pos(u01nz(b.x));
+      }
+    }
+    if (!alive) {
+      if (p.acur != 0.0) fold_layer(p, absorbed);
+      addi(T, TI_SUB_SAME, p.n_same);
+      addi(T, TI_SUB_CROSS, p.n_cross);
+      addi(T, TI_INTERACTIONS, p.n_int);
+      addi(T, TI_SURFACE_HITS, p.hits);
+      p.alive = false;
+    }
+  }
+
+  __syncthreads();
+  // absorbed per layer: FP64 LDS -> partials; totals: absorbed sum + the rest
+  const int per = ni * (NMC_MAX_LAYERS + TF_COUNT + TI_COUNT);
+  for (int j = threadIdx.x; j < ni; j += kLayBlock) {
+    double s = 0.0;
+    for (int l = 0; l < NMC_MAX_LAYERS; ++l) s += absorbed[j * NMC_MAX_LAYERS + l];
+    ((double*)&st[j])[TF_ABSORBED] = s;
+  }
+  __syncthreads();
+  double* part = a.part + (int64_t)blockIdx.x * per;
+  for (int i = threadIdx.x; i < per; i += kLayBlock) part[i] = smem[i];
+}
+
+// Gradient accumulation over the records.  Each thread keeps the running sums
+// of the stack of its last record in registers and flushes them to LDS when
+// the stack changes; blocks then add their LDS sums to the outputs.
+__global__ __launch_bounds__(256) void layered_backward_kernel(LayBackArgs b) {
+  extern __shared__ double acc[];   // [n_items][2][MAXL]
+  const int ni = b.n_items;
+  for (int i = threadIdx.x; i < ni * 2 * NMC_MAX_LAYERS; i += blockDim.x) acc[i] = 0.0;
+  __syncthreads();
+  uint64_t m = b.n_rec < b.cap ? b.n_rec : b.cap;
+  int cur = -1;
+  double ga[NMC_MAX_LAYERS], gs[NMC_MAX_LAYERS];
+#pragma unroll
+  for (int l = 0; l < NMC_MAX_LAYERS; ++l) ga[l] = gs[l] = 0.0;
+  for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < m;
+       k += (uint64_t)gridDim.x * blockDim.x) {
+    int j = (int)b.rec_item[k];
+    if (j >= ni) continue;
+    if (j != cur) {
+      if (cur >= 0) {
+#pragma unroll
+        for (int l = 0; l < NMC_MAX_LAYERS; ++l) {
+          if (ga[l] != 0.0) unsafeAtomicAdd(&acc[(cur * 2) * NMC_MAX_LAYERS + l], ga[l]);
+          if (gs[l] != 0.0) unsafeAtomicAdd(&acc[(cur * 2 + 1) * NMC_MAX_LAYERS + l], gs[l]);
+          ga[l] = gs[l] = 0.0;
+        }
+      }
+      cur = j;
+    }
+    double g = b.grad_rd[(int64_t)j * (b.nr + 1) + b.rec_bin[k]] * (double)b.rec_w[k] * b.inv_n[j];
+    int nl = b.nl[j];
+#pragma unroll
+    for (int l = 0; l < NMC_MAX_LAYERS; ++l) {
+      if (l < nl) {
+        double L = (double)b.rec_L[(uint64_t)l * b.cap + k];
+        double K = (double)b.rec_K[(uint64_t)l * b.cap + k];
+        ga[l] += -g * L;
+        gs[l] += g * (K / (double)b.mus[j * NMC_MAX_LAYERS + l] - L);
+      }
+    }
+  }
+  if (cur >= 0) {
+#pragma unroll
+    for (int l = 0; l < NMC_MAX_LAYERS; ++l) {
+      if (ga[l] != 0.0) unsafeAtomicAdd(&acc[(cur * 2) * NMC_MAX_LAYERS + l], ga[l]);
+      if (gs[l] != 0.0) unsafeAtomicAdd(&acc[(cur * 2 + 1) * NMC_MAX_LAYERS + l], gs[l]);
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < ni * 2 * NMC_MAX_LAYERS; i += blockDim.x) {
+    double v = acc[i];
+    if (v == 0.0) continue;
+    int j = i / (2 * NMC_MAX_LAYERS), w = (i / NMC_MAX_LAYERS) & 1, l = i % NMC_MAX_LAYERS;
+    unsafeAtomicAdd(&(w ? b.grad_mus : b.grad_mua)[j * NMC_MAX_LAYERS + l], v);
+  }
+}
+
+}  // namespace
+
+size_t layered_lds_bytes(int n_items, int n_rows) {
+  return (size_t)n_items * (NMC_MAX_LAYERS + TF_COUNT + TI_COUNT) * sizeof(double) +
+         (size_t)n_items * sizeof(LayStackDev) + (size_t)(n_items + 1) * sizeof(uint64_t) +
+         (size_t)n_rows * sizeof(LayRow);
+}
+
+hipError_t layered_configure(size_t lds_bytes) {
+  return hipFuncSetAttribute((const void*)layered_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)lds_bytes);
+}
+
+int layered_blocks_per_cu(size_t lds_bytes) {
+  int nb = 0;
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, layered_kernel, kLayBlock, lds_bytes);
+  return e == hipSuccess ? nb : 0;
+}
+
+hipError_t launch_layered(const LayArgs& a, int grid, size_t lds_bytes, hipStream_t s) {
+  hipLaunchKernelGGL(layered_kernel, dim3(grid), dim3(kLayBlock), lds_bytes, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_layered_backward(const LayBackArgs& b, hipStream_t s) {
+  size_t lds = (size_t)b.n_items * 2 * NMC_MAX_LAYERS * sizeof(double);
+  uint64_t m = b.n_rec < b.cap ? b.n_rec : b.cap;
+  int grid = (int)std::min<uint64_t>(2048, (m + 255) / 256);
+  if (grid < 1) grid = 1;
+  hipError_t e = hipFuncSetAttribute((const void*)layered_backward_kernel,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(layered_backward_kernel, dim3(grid), dim3(256), lds, s, b);
+  return hipGetLastError();
+}
+
+}  // namespace nmc

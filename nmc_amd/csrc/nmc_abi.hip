@@ -11,6 +11,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <string>
 #include <vector>
 
@@ -112,6 +113,14 @@ SubMedium make_medium(float mua, float mus, float g, float n1, float n2, uint32_
 
 constexpr int kMaxBatchItems = 2048;
 
+// A semi-infinite last layer that neither scatters nor absorbs would let a
+// photon travel forever.
+bool last_layer_semi_infinite_needs_scatter(const nmc_stack& s) {
+  const nmc_layer& L = s.layer[s.n_layers - 1];
+  bool semi = !(L.thickness > 0.0f && std::isfinite(L.thickness));
+  return semi && !(L.mua > 0.0f) && !(L.mus > 0.0f);
+}
+
 }  // namespace
 
 struct nmc_handle {
@@ -151,6 +160,27 @@ struct nmc_handle {
   bool sub_lds = false;
   size_t sub_lds_bytes = 0;
   int sub_grid = 0;
+  // layered media
+  nmc_layered_config lcfg{};
+  std::vector<nmc_stack> stacks;
+  LayStackDev* dHdr = nullptr;
+  LayRow* dRows = nullptr;
+  uint64_t* dLPrefix = nullptr;
+  double* dLRd = nullptr;      // [n][nr+1]
+  double* dLOut = nullptr;     // [n][MAXL] absorbed | [n] totals
+  double* dLPart = nullptr;
+  unsigned long long* dRecCount = nullptr;
+  uint32_t* dRecItem = nullptr;
+  uint32_t* dRecBin = nullptr;
+  float* dRecW = nullptr;
+  float* dRecL = nullptr;
+  uint32_t* dRecK = nullptr;
+  uint64_t rec_cap = 0;
+  double* dGrad = nullptr;     // grad_rd [n][nr+1] | inv_n [n] | grad_mua, grad_mus [n][MAXL]
+  float* dMus = nullptr;
+  int* dNl = nullptr;
+  int lay_rows = 0, lay_grid = 0;
+  size_t lay_lds = 0;
   // graph mode
   bool graph_mode = false;
   hipGraphExec_t graph_exec = nullptr;
@@ -185,6 +215,21 @@ void free_buffers(nmc_handle* h) {
   free_ptr(h->dCos);
   free_ptr(h->dOut);
   free_ptr(h->dPart);
+  free_ptr(h->dHdr);
+  free_ptr(h->dRows);
+  free_ptr(h->dLPrefix);
+  free_ptr(h->dLRd);
+  free_ptr(h->dLOut);
+  free_ptr(h->dLPart);
+  free_ptr(h->dRecCount);
+  free_ptr(h->dRecItem);
+  free_ptr(h->dRecBin);
+  free_ptr(h->dRecW);
+  free_ptr(h->dRecL);
+  free_ptr(h->dRecK);
+  free_ptr(h->dGrad);
+  free_ptr(h->dMus);
+  free_ptr(h->dNl);
   h->prepared = false;
 }
 
@@ -554,7 +599,8 @@ int nmc_validate(int sim, const RunParams* rp, const TissueParams* tp, const uin
 int nmc_create(int sim_type, int device, nmc_handle** out) {
   if (!out) return fail(NMC_ERR_ARG, "out is null");
   *out = nullptr;
-  if (sim_type != NMC_SIM_MCSUB && sim_type != NMC_SIM_MCXYZ && sim_type != NMC_SIM_MCXYZ_TT)
+  if (sim_type != NMC_SIM_MCSUB && sim_type != NMC_SIM_MCXYZ && sim_type != NMC_SIM_MCXYZ_TT &&
+      sim_type != NMC_SIM_LAYERED)
     return fail(NMC_ERR_UNSUPPORTED, "unsupported simulation kind %d", sim_type);
   int count = 0;
   HIP_TRY(hipGetDeviceCount(&count));
@@ -815,6 +861,301 @@ int nmc_get_batch_results(nmc_handle* h, double* tally, nmc_totals* totals) {
   if (tally) HIP_TRY(hipMemcpy(tally, h->dOut, nt * sizeof(double), hipMemcpyDeviceToHost));
   if (totals)
     HIP_TRY(hipMemcpy(totals, h->dOut + nt, h->n_items * sizeof(nmc_totals), hipMemcpyDeviceToHost));
+  return ok();
+}
+
+// ---------------------------------------------------------------------------
+// Layered media
+// ---------------------------------------------------------------------------
+static int validate_stacks(const nmc_layered_config* cfg, const nmc_stack* st, int n) {
+  if (!cfg || !st) return fail(NMC_ERR_ARG, "config and stacks are required");
+  if (cfg->struct_size != sizeof(nmc_layered_config) || cfg->version != NMC_EXT_VERSION)
+    return fail(NMC_ERR_ARG, "nmc_layered_config size/version mismatch");
+  if (n <= 0 || n > kMaxBatchItems) return fail(NMC_ERR_ARG, "n_stacks must be in 1..%d", kMaxBatchItems);
+  if (cfg->nr <= 0 || !(cfg->dr > 0.0f)) return fail(NMC_ERR_ARG, "Rd bins invalid (nr >= 1, dr > 0)");
+  if (!(cfg->threshold >= 0.0f) || !(cfg->chance > 0.0f && cfg->chance <= 1.0f) || cfg->max_events == 0)
+    return fail(NMC_ERR_ARG, "roulette / max_events settings invalid");
+  for (int j = 0; j < n; ++j) {
+    const nmc_stack& s = st[j];
+    if (s.n_layers < 1 || s.n_layers > NMC_MAX_LAYERS)
+      return fail(NMC_ERR_ARG, "stack %d: n_layers %d not in 1..%d", j, s.n_layers, NMC_MAX_LAYERS);
+    if (!(s.n_above > 0.0f) || !(s.n_below > 0.0f)) return fail(NMC_ERR_ARG, "stack %d: ambient index must be > 0", j);
+    for (int l = 0; l < s.n_layers; ++l) {
+      const nmc_layer& L = s.layer[l];
+      bool last = l == s.n_layers - 1;
+      if (!last && !(L.thickness > 0.0f && std::isfinite(L.thickness)))
+        return fail(NMC_ERR_ARG, "stack %d layer %d: thickness must be > 0 (only the last may be semi-infinite)", j, l);
+      if (!finite_nonneg(L.mua) || !finite_nonneg(L.mus) || !(L.n > 0.0f) || !std::isfinite(L.n))
+        return fail(NMC_ERR_ARG, "stack %d layer %d: mua, mus >= 0 and n > 0 required", j, l);
+      if (L.phase == 0) {
+        if (!(L.g >= -1.0f && L.g <= 1.0f)) return fail(NMC_ERR_ARG, "stack %d layer %d: g not in [-1,1]", j, l);
+      } else if (L.phase == 1) {
+        if (!(L.cc >= 0.0f && L.cc <= 1.0f) || !(L.gf >= 0.0f && L.gf < 1.0f) || !(L.gb >= 0.0f && L.gb < 1.0f) ||
+            (L.gf != 0.0f && !(L.af > 0.0f)) || (L.gb != 0.0f && !(L.ab > 0.0f)))
+          return fail(NMC_ERR_ARG, "stack %d layer %d: two-term parameters out of range", j, l);
+      } else {
+        return fail(NMC_ERR_ARG, "stack %d layer %d: phase %d not 0 or 1", j, l, L.phase);
+      }
+    }
+    if (last_layer_semi_infinite_needs_scatter(s))
+      return fail(NMC_ERR_ARG, "stack %d: a semi-infinite last layer needs mua > 0 or mus > 0", j);
+  }
+  return ok();
+}
+
+// Per-stack header and per-layer rows (interface depths summed in double,
+// rounded once to float).
+static void build_tables(nmc_handle* h, std::vector<LayStackDev>& hdr, std::vector<LayRow>& rows,
+                         std::vector<float>& mus, std::vector<int>& nl) {
+  int n = (int)h->stacks.size();
+  hdr.assign(n, LayStackDev{});
+  rows.clear();
+  mus.assign((size_t)n * NMC_MAX_LAYERS, 0.0f);
+  nl.assign(n, 0);
+  for (int j = 0; j < n; ++j) {
+    const nmc_stack& s = h->stacks[j];
+    LayStackDev& H = hdr[j];
+    H.nl = s.n_layers;
+    H.row0 = (int)rows.size();
+    H.n_above = s.n_above;
+    H.n_below = s.n_below;
+    H.key1 = key1_of(h, s.stream_id);
+    // normal-incidence specular reflection at the top (fresnel() ca1 = 1 branch)
+    float n0 = s.layer[0].n;
+    float d = (s.n_above + n0), r = (n0 - s.n_above) / d;
+    float e = n0 * 0x1p-23f * 2.0f;
+    H.rsp = (s.n_above >= n0 - e && s.n_above <= n0 + e) ? 0.0f : r * r;
+    double z = 0.0;
+    for (int l = 0; l < s.n_layers; ++l) {
+      const nmc_layer& L = s.layer[l];
+      LayRow R{};
+      R.zt = (float)z;
+      bool last = l == s.n_layers - 1;
+      if (last && !(L.thickness > 0.0f && std::isfinite(L.thickness))) {
+        R.zb = std::numeric_limits<float>::infinity();
+      } else {
+        z += (double)L.thickness;
+        R.zb = (float)z;
+      }
+      R.mua = L.mua;
+      R.mus = L.mus;
+      R.n = L.n;
+      R.g = L.g;
+      R.phase = L.phase;
+      if (L.phase) tt_prepare(L.gf, L.af, L.gb, L.ab, L.cc, &R.tt);
+      rows.push_back(R);
+      mus[(size_t)j * NMC_MAX_LAYERS + l] = L.mus;
+    }
+    nl[j] = s.n_layers;
+  }
+}
+
+static int upload_stacks(nmc_handle* h) {
+  std::vector<LayStackDev> hdr;
+  std::vector<LayRow> rows;
+  std::vector<float> mus;
+  std::vector<int> nl;
+  build_tables(h, hdr, rows, mus, nl);
+  int n = (int)h->stacks.size();
+  std::vector<uint64_t> prefix(n + 1, 0);
+  std::vector<double> inv_n(n);
+  for (int j = 0; j < n; ++j) {
+    prefix[j + 1] = prefix[j] + h->stacks[j].n_photons;
+    inv_n[j] = h->stacks[j].n_photons ? 1.0 / (double)h->stacks[j].n_photons : 0.0;
+  }
+  h->prefix = prefix;
+  HIP_TRY(hipMemcpy(h->dHdr, hdr.data(), n * sizeof(LayStackDev), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(h->dRows, rows.data(), rows.size() * sizeof(LayRow), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(h->dLPrefix, prefix.data(), (n + 1) * sizeof(uint64_t), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(h->dMus, mus.data(), mus.size() * sizeof(float), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(h->dNl, nl.data(), n * sizeof(int), hipMemcpyHostToDevice));
+  double* inv = h->dGrad + (size_t)n * (h->lcfg.nr + 1);
+  HIP_TRY(hipMemcpy(inv, inv_n.data(), n * sizeof(double), hipMemcpyHostToDevice));
+  return ok();
+}
+
+int nmc_layered_prepare(nmc_handle* h, const nmc_layered_config* cfg, const nmc_stack* stacks,
+                        int n_stacks) {
+  if (!h) return fail(NMC_ERR_ARG, "null handle");
+  if (h->sim != NMC_SIM_LAYERED) return fail(NMC_ERR_UNSUPPORTED, "handle is not NMC_SIM_LAYERED");
+  NMC_TRY(validate_stacks(cfg, stacks, n_stacks));
+  HIP_TRY(hipSetDevice(h->device));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  free_buffers(h);
+  h->lcfg = *cfg;
+  h->stacks.assign(stacks, stacks + n_stacks);
+  h->ext.seed = cfg->seed;
+  h->next_photon = 0;
+  h->timed = false;
+  int n = n_stacks;
+  h->lay_rows = 0;
+  uint64_t total = 0;
+  for (int j = 0; j < n; ++j) {
+    h->lay_rows += stacks[j].n_layers;
+    total += stacks[j].n_photons;
+  }
+  HIP_TRY(hipMalloc(&h->dHdr, n * sizeof(LayStackDev)));
+  HIP_TRY(hipMalloc(&h->dRows, h->lay_rows * sizeof(LayRow)));
+  HIP_TRY(hipMalloc(&h->dLPrefix, (n + 1) * sizeof(uint64_t)));
+  HIP_TRY(hipMalloc(&h->dLRd, (size_t)n * (cfg->nr + 1) * sizeof(double)));
+  HIP_TRY(hipMalloc(&h->dLOut, (size_t)n * (NMC_MAX_LAYERS + TF_COUNT + TI_COUNT) * sizeof(double)));
+  HIP_TRY(hipMalloc(&h->dMus, (size_t)n * NMC_MAX_LAYERS * sizeof(float)));
+  HIP_TRY(hipMalloc(&h->dNl, n * sizeof(int)));
+  HIP_TRY(hipMalloc(&h->dGrad, ((size_t)n * (cfg->nr + 1) + n + 2 * (size_t)n * NMC_MAX_LAYERS) * sizeof(double)));
+  HIP_TRY(hipMalloc(&h->dRecCount, sizeof(unsigned long long)));
+  h->rec_cap = 0;
+  if (cfg->records) {
+    h->rec_cap = cfg->record_capacity ? cfg->record_capacity : total;
+    if (h->rec_cap == 0) h->rec_cap = 1;
+    HIP_TRY(hipMalloc(&h->dRecItem, h->rec_cap * sizeof(uint32_t)));
+    HIP_TRY(hipMalloc(&h->dRecBin, h->rec_cap * sizeof(uint32_t)));
+    HIP_TRY(hipMalloc(&h->dRecW, h->rec_cap * sizeof(float)));
+    HIP_TRY(hipMalloc(&h->dRecL, h->rec_cap * NMC_MAX_LAYERS * sizeof(float)));
+    HIP_TRY(hipMalloc(&h->dRecK, h->rec_cap * NMC_MAX_LAYERS * sizeof(uint32_t)));
+  }
+  h->lay_lds = layered_lds_bytes(n, h->lay_rows);
+  if (h->lay_lds + 1024 > h->lds_max)
+    return fail(NMC_ERR_ARG, "%d stacks / %d layers need %zu B of LDS (%zu available)", n, h->lay_rows,
+                h->lay_lds, h->lds_max);
+  HIP_TRY(layered_configure(h->lay_lds));
+  int occ = layered_blocks_per_cu(h->lay_lds);
+  if (occ <= 0) return fail(NMC_ERR_ARG, "layered kernel does not fit on a CU");
+  if (h->tune_req.blocks_per_cu > 0) occ = std::min(occ, h->tune_req.blocks_per_cu);
+  h->lay_grid = h->num_cu * occ;
+  h->tune = nmc_tuning{};
+  h->tune.block = kLayBlock;
+  h->tune.blocks_per_cu = occ;
+  h->tune.refill_min = h->tune_req.refill_min ? h->tune_req.refill_min : 16;
+  h->tune.chunk = h->tune_req.chunk;
+  HIP_TRY(hipMalloc(&h->dLPart, (size_t)h->lay_grid * n * (NMC_MAX_LAYERS + TF_COUNT + TI_COUNT) * sizeof(double)));
+  NMC_TRY(upload_stacks(h));
+  h->prepared = true;
+  return write_first(h);
+}
+
+int nmc_layered_update(nmc_handle* h, const nmc_stack* stacks, int n_stacks) {
+  if (!h) return fail(NMC_ERR_ARG, "null handle");
+  if (h->sim != NMC_SIM_LAYERED || !h->prepared) return fail(NMC_ERR_STATE, "nmc_layered_prepare has not succeeded");
+  if (n_stacks != (int)h->stacks.size()) return fail(NMC_ERR_ARG, "update must keep the number of stacks");
+  NMC_TRY(validate_stacks(&h->lcfg, stacks, n_stacks));
+  uint64_t total = 0;
+  for (int j = 0; j < n_stacks; ++j) {
+    if (stacks[j].n_layers != h->stacks[j].n_layers) return fail(NMC_ERR_ARG, "update must keep each stack's layer count");
+    total += stacks[j].n_photons;
+  }
+  if (h->lcfg.records && !h->lcfg.record_capacity && total > h->rec_cap)
+    return fail(NMC_ERR_ARG, "budgets exceed the record capacity set at prepare");
+  HIP_TRY(hipSetDevice(h->device));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  h->stacks.assign(stacks, stacks + n_stacks);
+  return upload_stacks(h);
+}
+
+int nmc_layered_run(nmc_handle* h) {
+  if (!h) return fail(NMC_ERR_ARG, "null handle");
+  if (h->sim != NMC_SIM_LAYERED || !h->prepared) return fail(NMC_ERR_STATE, "nmc_layered_prepare has not succeeded");
+  HIP_TRY(hipSetDevice(h->device));
+  int n = (int)h->stacks.size();
+  hipStream_t s = h->stream;
+  HIP_TRY(hipMemsetAsync(h->dLRd, 0, (size_t)n * (h->lcfg.nr + 1) * sizeof(double), s));
+  HIP_TRY(hipMemsetAsync(h->dLOut, 0, (size_t)n * (NMC_MAX_LAYERS + TF_COUNT + TI_COUNT) * sizeof(double), s));
+  HIP_TRY(hipMemsetAsync(h->dRecCount, 0, sizeof(unsigned long long), s));
+  HIP_TRY(hipEventRecord(h->e0, s));
+  HIP_TRY(launch_state_begin(h->dState, s));
+  LayArgs a{};
+  a.n_items = n;
+  a.n_rows = h->lay_rows;
+  a.nr = h->lcfg.nr;
+  a.dr = h->lcfg.dr;
+  a.thr = h->lcfg.threshold;
+  a.chance = h->lcfg.chance;
+  a.max_events = h->lcfg.max_events;
+  a.key0 = key0_of(h);
+  uint64_t total = h->prefix[n];
+  a.chunk = pick_chunk(total, h->lay_grid * (kLayBlock / 64), h->tune.chunk);
+  a.refill_min = (uint32_t)h->tune.refill_min;
+  a.stacks = h->dHdr;
+  a.rows = h->dRows;
+  a.prefix = h->dLPrefix;
+  a.rd = h->dLRd;
+  a.part = h->dLPart;
+  a.records = h->lcfg.records != 0;
+  a.cap = h->rec_cap;
+  a.rec_count = h->dRecCount;
+  a.rec_item = h->dRecItem;
+  a.rec_bin = h->dRecBin;
+  a.rec_w = h->dRecW;
+  a.rec_L = h->dRecL;
+  a.rec_K = h->dRecK;
+  a.state = h->dState;
+  HIP_TRY(launch_layered(a, h->lay_grid, h->lay_lds, s));
+  HIP_TRY(launch_reduce_partials(h->dLPart, h->lay_grid, (int64_t)n * (NMC_MAX_LAYERS + TF_COUNT + TI_COUNT),
+                                 h->dLOut, n, NMC_MAX_LAYERS, s));
+  HIP_TRY(hipEventRecord(h->e1, s));
+  h->timed = true;
+  HIP_TRY(hipStreamSynchronize(s));
+  return ok();
+}
+
+int nmc_layered_results(nmc_handle* h, double* rd, double* absorbed, nmc_totals* totals,
+                        uint64_t* n_records) {
+  if (!h) return fail(NMC_ERR_ARG, "null handle");
+  if (h->sim != NMC_SIM_LAYERED || !h->prepared) return fail(NMC_ERR_STATE, "nmc_layered_prepare has not succeeded");
+  HIP_TRY(hipSetDevice(h->device));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  size_t n = h->stacks.size();
+  if (rd) HIP_TRY(hipMemcpy(rd, h->dLRd, n * (h->lcfg.nr + 1) * sizeof(double), hipMemcpyDeviceToHost));
+  if (absorbed) HIP_TRY(hipMemcpy(absorbed, h->dLOut, n * NMC_MAX_LAYERS * sizeof(double), hipMemcpyDeviceToHost));
+  if (totals)
+    HIP_TRY(hipMemcpy(totals, h->dLOut + n * NMC_MAX_LAYERS, n * sizeof(nmc_totals), hipMemcpyDeviceToHost));
+  if (n_records) {
+    unsigned long long c = 0;
+    HIP_TRY(hipMemcpy(&c, h->dRecCount, sizeof c, hipMemcpyDeviceToHost));
+    *n_records = c;
+  }
+  return ok();
+}
+
+int nmc_layered_backward(nmc_handle* h, const double* grad_rd, double* grad_mua, double* grad_mus) {
+  if (!h || !grad_rd) return fail(NMC_ERR_ARG, "null argument");
+  if (h->sim != NMC_SIM_LAYERED || !h->prepared) return fail(NMC_ERR_STATE, "nmc_layered_prepare has not succeeded");
+  if (!h->lcfg.records) return fail(NMC_ERR_STATE, "records were not enabled at prepare");
+  HIP_TRY(hipSetDevice(h->device));
+  hipStream_t s = h->stream;
+  size_t n = h->stacks.size(), nb = n * (h->lcfg.nr + 1);
+  unsigned long long nrec = 0;
+  HIP_TRY(hipMemcpy(&nrec, h->dRecCount, sizeof nrec, hipMemcpyDeviceToHost));
+  if (nrec > h->rec_cap)
+    return fail(NMC_ERR_NOMEM, "%llu records exceed the capacity %llu; raise record_capacity",
+                (unsigned long long)nrec, (unsigned long long)h->rec_cap);
+  double* d_grad_rd = h->dGrad;
+  double* d_inv = h->dGrad + nb;
+  double* d_gmua = d_inv + n;
+  double* d_gmus = d_gmua + n * NMC_MAX_LAYERS;
+  HIP_TRY(hipMemcpyAsync(d_grad_rd, grad_rd, nb * sizeof(double), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemsetAsync(d_gmua, 0, 2 * n * NMC_MAX_LAYERS * sizeof(double), s));
+  LayBackArgs b{};
+  b.n_items = (int)n;
+  b.nr = h->lcfg.nr;
+  b.n_rec = nrec;
+  b.cap = h->rec_cap;
+  b.rec_item = h->dRecItem;
+  b.rec_bin = h->dRecBin;
+  b.rec_w = h->dRecW;
+  b.rec_L = h->dRecL;
+  b.rec_K = h->dRecK;
+  b.grad_rd = d_grad_rd;
+  b.mus = h->dMus;
+  b.nl = h->dNl;
+  b.inv_n = d_inv;
+  b.grad_mua = d_gmua;
+  b.grad_mus = d_gmus;
+  HIP_TRY(hipEventRecord(h->e0, s));
+  HIP_TRY(launch_layered_backward(b, s));
+  HIP_TRY(hipEventRecord(h->e1, s));
+  if (grad_mua) HIP_TRY(hipMemcpyAsync(grad_mua, d_gmua, n * NMC_MAX_LAYERS * sizeof(double), hipMemcpyDeviceToHost, s));
+  if (grad_mus) HIP_TRY(hipMemcpyAsync(grad_mus, d_gmus, n * NMC_MAX_LAYERS * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
   return ok();
 }
 

@@ -84,6 +84,62 @@ struct SubArgs {
   RunState* state;
 };
 
+// Layered media: one row per (stack, layer), one header per stack.
+struct LayRow {
+  float zt, zb, mua, mus, n, g;
+  int phase, pad;
+  TTCoef tt;
+};
+struct LayStackDev {
+  int nl, row0;
+  float n_above, n_below, rsp;
+  uint32_t key1;
+  int pad[2];
+};
+
+struct LayArgs {
+  int n_items, n_rows, nr;
+  float dr, thr, chance;
+  uint32_t max_events, key0, chunk, refill_min;
+  const LayStackDev* stacks;        // [n_items]
+  const LayRow* rows;               // [n_rows]
+  const uint64_t* prefix;           // [n_items+1]
+  double* rd;                       // [n_items][nr+1] (global atomics)
+  double* part;                     // block partials of [absorbed | totals]
+  int records;
+  uint64_t cap;
+  unsigned long long* rec_count;
+  uint32_t* rec_item;
+  uint32_t* rec_bin;
+  float* rec_w;
+  float* rec_L;                     // [NMC_MAX_LAYERS][cap]
+  uint32_t* rec_K;                  // [NMC_MAX_LAYERS][cap]
+  RunState* state;
+};
+
+struct LayBackArgs {
+  int n_items, nr;
+  uint64_t n_rec, cap;
+  const uint32_t* rec_item;
+  const uint32_t* rec_bin;
+  const float* rec_w;
+  const float* rec_L;
+  const uint32_t* rec_K;
+  const double* grad_rd;            // [n_items][nr+1]
+  const float* mus;                 // [n_items][NMC_MAX_LAYERS]
+  const int* nl;                    // [n_items]
+  const double* inv_n;              // [n_items] 1 / budget
+  double* grad_mua;                 // [n_items][NMC_MAX_LAYERS] (accumulated)
+  double* grad_mus;
+};
+
+constexpr int kLayBlock = 1024;
+size_t layered_lds_bytes(int n_items, int n_rows);
+hipError_t layered_configure(size_t lds_bytes);
+int layered_blocks_per_cu(size_t lds_bytes);
+hipError_t launch_layered(const LayArgs& a, int grid, size_t lds_bytes, hipStream_t s);
+hipError_t launch_layered_backward(const LayBackArgs& b, hipStream_t s);
+
 constexpr int kSubBlock = 1024;
 
 hipError_t launch_mcxyz(const XyzArgs& a, bool two_term, int block, int grid, size_t lds_bytes,

@@ -147,9 +147,57 @@ class WorkItem(C.Structure):
                 ("n2", C.c_float), ("stream_id", C.c_uint32), ("n_photons", C.c_uint64)]
 
 
+SIM_LAYERED = 3
+MAX_LAYERS = 8
+
+
+class Layer(C.Structure):
+    """nmc_layer: one layer of a stack (thickness <= 0 on the last layer:
+    semi-infinite)."""
+
+    _fields_ = [("thickness", C.c_float), ("mua", C.c_float), ("mus", C.c_float), ("n", C.c_float),
+                ("g", C.c_float), ("gf", C.c_float), ("af", C.c_float), ("gb", C.c_float),
+                ("ab", C.c_float), ("cc", C.c_float), ("phase", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
+class Stack(C.Structure):
+    """nmc_stack: a layered medium with its Philox stream and photon budget."""
+
+    _fields_ = [("n_layers", C.c_int32), ("n_above", C.c_float), ("n_below", C.c_float),
+                ("stream_id", C.c_uint32), ("n_photons", C.c_uint64),
+                ("layer", Layer * MAX_LAYERS)]
+
+
+class LayeredConfig(C.Structure):
+    """nmc_layered_config."""
+
+    _fields_ = [("struct_size", C.c_uint32), ("version", C.c_uint32), ("seed", C.c_uint64),
+                ("nr", C.c_int32), ("dr", C.c_float), ("threshold", C.c_float),
+                ("chance", C.c_float), ("max_events", C.c_uint32), ("records", C.c_int32),
+                ("record_capacity", C.c_uint64), ("reserved", C.c_int32 * 8)]
+
+    @classmethod
+    def default(cls, **kw) -> "LayeredConfig":
+        c = cls()
+        c.struct_size = C.sizeof(cls)
+        c.version = EXT_VERSION
+        c.seed = 1
+        c.nr, c.dr = 200, 0.005
+        c.threshold, c.chance = THRESHOLD, CHANCE
+        c.max_events = 100_000_000
+        c.records = 1
+        for k, v in kw.items():
+            if not hasattr(c, k):
+                raise AttributeError(f"nmc_layered_config has no field {k!r}")
+            setattr(c, k, v)
+        return c
+
+
 assert C.sizeof(RunParams) == 172
 assert C.sizeof(TissueParams) == 608
 assert C.sizeof(WorkItem) == 32
+assert C.sizeof(Layer) == 48 and C.sizeof(Stack) == 408 and C.sizeof(LayeredConfig) == 80
 
 
 def dptr(a: np.ndarray | None):

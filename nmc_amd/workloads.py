@@ -133,6 +133,60 @@ def spectral_sweep(n_wavelengths: int = 128, seed: int = 3, budget_scale: float 
     return w, items, lam
 
 
+TWO_TERM_SKIN = dict(gf=0.9, af=0.5, gb=0.3, ab=0.5, cc=0.8)   # HG(0.9)x0.8 + HG(-0.3)x0.2
+
+
+def make_stack(thick, mua, mus, n, n_photons, stream_id=0, phase=None, g=0.9,
+               n_above=1.0, n_below=1.0) -> P.Stack:
+    """A Stack struct; the last layer is semi-infinite if its thickness <= 0."""
+    s = P.Stack()
+    s.n_layers = len(mua)
+    s.n_above, s.n_below = n_above, n_below
+    s.stream_id = stream_id
+    s.n_photons = int(n_photons)
+    for l in range(len(mua)):
+        L = s.layer[l]
+        L.thickness, L.mua, L.mus, L.n, L.g = thick[l], mua[l], mus[l], n[l], g
+        if phase:
+            L.phase = 1
+            L.gf, L.af, L.gb, L.ab, L.cc = (phase[k] for k in ("gf", "af", "gb", "ab", "cc"))
+    return s
+
+
+def multilayer_5(n_photons: int = 10_000_000, seed: int = 4):
+    """Differentiable 5-layer case: from default_rng(seed), thickness
+    U(0.01, 0.2) cm (the 5th layer semi-infinite), mua U(0.1, 10) /cm, mus
+    U(50, 300) /cm, n U(1.33, 1.45); two-term HG (0.9, -0.3, 0.8); pencil beam;
+    Rd(r) in 200 bins of 0.005 cm.  Returns (LayeredConfig, [Stack])."""
+    rng = np.random.default_rng(seed)
+    thick = rng.uniform(0.01, 0.2, 5)
+    thick[-1] = 0.0
+    mua = rng.uniform(0.1, 10.0, 5)
+    mus = rng.uniform(50.0, 300.0, 5)
+    n = rng.uniform(1.33, 1.45, 5)
+    st = make_stack(thick, mua, mus, n, n_photons, phase=TWO_TERM_SKIN)
+    return P.LayeredConfig.default(nr=200, dr=0.005), [st]
+
+
+def inverse_fit_instances(n_inst: int = 256, seed: int = 5, budget_scale: float = 1.0):
+    """Batched inverse-fit case: n_inst independent 3-layer stacks (ranges as
+    multilayer_5, the 3rd layer semi-infinite), budgets floor(10^U(5, log10 5e6)),
+    Rd(r) in 100 bins of 0.01 cm.  Returns (LayeredConfig, [Stack] with the
+    hidden properties)."""
+    rng = np.random.default_rng(seed)
+    stacks = []
+    for i in range(n_inst):
+        thick = rng.uniform(0.01, 0.2, 3)
+        thick[-1] = 0.0
+        mua = rng.uniform(0.1, 10.0, 3)
+        mus = rng.uniform(50.0, 300.0, 3)
+        n = rng.uniform(1.33, 1.45, 3)
+        budget = int(np.floor(10 ** rng.uniform(5, np.log10(5e6)) * budget_scale))
+        stacks.append(make_stack(thick, mua, mus, n, max(1, budget), stream_id=i,
+                                 phase=TWO_TERM_SKIN))
+    return P.LayeredConfig.default(nr=100, dr=0.01), stacks
+
+
 def tiny_voxel_case(kind: str = "mismatch", n_photons: int = 2000, seed: int = 7,
                     n: int = 16) -> Workload:
     """Small mcxyz cases for parity: every launch mode / boundary mode can be set

@@ -856,3 +856,279 @@ int nmco_mcsub(const RunParams* rp, const TissueParams* tp, const nmc_ext_config
     free(acc);
     return fail ? NMC_ERR_NOMEM : NMC_OK;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Layered media with per-layer index and perturbation-MC records.          */
+/* No reference counterpart (NMC has one global n1/n2, definitions.h:196);  */
+/* the step/drop/spin/roulette follow mcxyz above (kernels.metal:493-750),  */
+/* interfaces use Fresnel all-or-none reflection and Snell refraction as    */
+/* mcsub's surface does (kernels.metal:190-214, helpers.metal:243-314).     */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    int nl;
+    float n_above, n_below;
+    float zt[NMC_MAX_LAYERS], zb[NMC_MAX_LAYERS];
+    float mua[NMC_MAX_LAYERS], mus[NMC_MAX_LAYERS], n[NMC_MAX_LAYERS], g[NMC_MAX_LAYERS];
+    int phase[NMC_MAX_LAYERS];
+    tt_coef tt[NMC_MAX_LAYERS];
+    float rsp;
+    uint32_t key[2];
+} lay_stack;
+
+typedef struct {
+    int nr;
+    float dr, thr, chance;
+    uint32_t max_events;
+} lay_cfg;
+
+typedef struct {
+    double* rd;          /* [nr+1] */
+    double* absorbed;    /* [NMC_MAX_LAYERS] */
+    nmc_totals t;
+    /* records (may be NULL) */
+    uint64_t* n_rec;
+    uint64_t cap;
+    uint32_t* rec_item;
+    uint32_t* rec_bin;
+    float* rec_w;
+    float* rec_L;        /* [NMC_MAX_LAYERS][cap] */
+    uint32_t* rec_K;     /* [NMC_MAX_LAYERS][cap] */
+    uint32_t item;
+} lay_acc;
+
+void nmco_layered_prepare_stack(const nmc_stack* s, uint64_t seed, lay_stack* o);
+
+static void lay_photon(const lay_stack* S, const lay_cfg* c, uint64_t pid, lay_acc* a)
+{
+    nmc_totals* t = &a->t;
+    uint32_t b[4], aux[4];
+    t->photons++;
+    float ca2;
+    float rsp = nmco_fresnel(S->n_above, S->n[0], 1.0f, &ca2);
+    t->specular += (double)rsp;
+    float W = 1.0f - rsp;
+    float x = 0.0f, y = 0.0f, z = 0.0f, ux = 0.0f, uy = 0.0f, uz = 1.0f;
+    float ruz = 1.0f / uz;
+    int l = 0;
+    uint32_t hits = 0, ev = 1, events = 0;
+    /* path length is summed per layer visit (Lcur) and added to L[l] when
+     * the photon leaves the layer */
+    float L[NMC_MAX_LAYERS] = {0}, Lcur = 0.0f;
+    uint32_t K[NMC_MAX_LAYERS] = {0};
+    block(S->key, pid, ev, 0u, b);
+    float sleft = -nmco_logf(u01nz(b[0]));
+    for (;;) {
+        float mua = S->mua[l], mus = S->mus[l];
+        float s_int = sleft / mus;
+        float s_b = INF_F;
+        if (uz > 0.0f) s_b = fmaxf((S->zb[l] - z) * ruz, 0.0f);
+        else if (uz < 0.0f) s_b = fmaxf((S->zt[l] - z) * ruz, 0.0f);
+        int inter = s_int <= s_b;
+        float s = inter ? s_int : s_b;
+        x = fmaf(s, ux, x);
+        y = fmaf(s, uy, y);
+        z = fmaf(s, uz, z);
+        float em = nmco_expm1f(-(mua * s));
+        float dep = -(W * em);
+        W = W - dep;
+        a->absorbed[l] += (double)dep;
+        t->absorbed += (double)dep;
+        Lcur = Lcur + s;
+        events++;
+        if (inter) {
+            sleft = 0.0f;
+            t->substeps_same++;
+        } else {
+            t->substeps_cross++;
+            sleft = sleft - s * mus;
+            if (!(sleft > 0.0f)) sleft = 0.0f;
+            int up = uz < 0.0f;
+            z = up ? S->zt[l] : S->zb[l];
+            float ni = S->n[l];
+            float nt = up ? (l == 0 ? S->n_above : S->n[l - 1]) : (l == S->nl - 1 ? S->n_below : S->n[l + 1]);
+            float r = nmco_fresnel(ni, nt, fabsf(uz), &ca2);
+            t->surface_hits++;
+            block(S->key, pid, hits, 2u, aux);
+            hits++;
+            if (u01(aux[0]) < r) {
+                uz = -uz;
+                ruz = -ruz;
+            } else if (up && l == 0) {
+                L[l] = L[l] + Lcur;
+                float re = sqrtf(x * x + y * y);
+                int ir = rbin(re / c->dr, c->nr);
+                a->rd[ir] += (double)W;
+                t->rd += (double)W;
+                t->detections++;
+                if (a->rec_w) {
+                    uint64_t k = (*a->n_rec)++;
+                    if (k < a->cap) {
+                        a->rec_item[k] = a->item;
+                        a->rec_bin[k] = (uint32_t)ir;
+                        a->rec_w[k] = W;
+                        for (int i = 0; i < NMC_MAX_LAYERS; ++i) {
+                            a->rec_L[(uint64_t)i * a->cap + k] = L[i];
+                            a->rec_K[(uint64_t)i * a->cap + k] = K[i];
+                        }
+                    }
+                }
+                return;
+            } else if (!up && l == S->nl - 1) {
+                t->tt += (double)W;
+                t->escapes_bottom++;
+                return;
+            } else {
+                L[l] = L[l] + Lcur;
+                Lcur = 0.0f;
+                if (ni != nt) {
+                    ux = ux * ni / nt;
+                    uy = uy * ni / nt;
+                    uz = up ? -ca2 : ca2;
+                    ruz = 1.0f / uz;
+                }
+                l += up ? -1 : 1;
+            }
+        }
+        if (events >= c->max_events) {
+            t->capped += (double)W;
+            t->capped_photons++;
+            return;
+        }
+        if (sleft > 0.0f) continue;
+        float ct = S->phase[l] ? tt_cos(&S->tt[l], u01(b[1])) : nmco_hg_cos(S->g[l], u01(b[1]));
+        nmco_spin(ct, u01(b[2]), &ux, &uy, &uz);
+        ruz = 1.0f / uz;
+        K[l]++;
+        t->interactions++;
+        if (!roulette(&W, u01(b[3]), c->thr, c->chance, t)) return;
+        ev++;
+        block(S->key, pid, ev, 0u, b);
+        sleft = -nmco_logf(u01nz(b[0]));
+    }
+}
+
+/* Interface depths summed in double, rounded once to float. */
+void nmco_layered_prepare_stack(const nmc_stack* s, uint64_t seed, lay_stack* o)
+{
+    memset(o, 0, sizeof(*o));
+    o->nl = s->n_layers;
+    o->n_above = s->n_above;
+    o->n_below = s->n_below;
+    double z = 0.0;
+    for (int l = 0; l < s->n_layers; ++l) {
+        const nmc_layer* L = &s->layer[l];
+        o->zt[l] = (float)z;
+        int last = l == s->n_layers - 1;
+        if (last && !(L->thickness > 0.0f && L->thickness < INF_F)) o->zb[l] = INF_F;
+        else { z += (double)L->thickness; o->zb[l] = (float)z; }
+        o->mua[l] = L->mua; o->mus[l] = L->mus; o->n[l] = L->n; o->g[l] = L->g;
+        o->phase[l] = L->phase;
+        if (L->phase) tt_prepare(L->gf, L->af, L->gb, L->ab, L->cc, &o->tt[l]);
+    }
+    o->key[0] = (uint32_t)seed;
+    o->key[1] = (uint32_t)(seed >> 32) + s->stream_id;
+}
+
+/*
+ * Runs photons [first, first + n_j) of each stack j.  rd: [n_stacks][nr+1],
+ * absorbed: [n_stacks][NMC_MAX_LAYERS], totals: [n_stacks]; all accumulated
+ * (+=).  Records (optional, rec_w != NULL) are appended stack by stack in
+ * photon order; *n_rec is the count (records beyond cap are counted, not kept).
+ */
+int nmco_layered(const nmc_layered_config* cfg, const nmc_stack* stacks, int n_stacks,
+                 uint64_t first, int n_threads, double* rd, double* absorbed, nmc_totals* totals,
+                 uint64_t* n_rec, uint64_t cap, uint32_t* rec_item, uint32_t* rec_bin,
+                 float* rec_w, float* rec_L, uint32_t* rec_K)
+{
+    if (!cfg || !stacks || n_stacks <= 0 || cfg->nr <= 0 || !(cfg->dr > 0.0f)) return NMC_ERR_ARG;
+    lay_cfg c = {cfg->nr, cfg->dr, cfg->threshold, cfg->chance, cfg->max_events};
+    int nt = n_threads > 0 ? n_threads : nmco_max_threads();
+    double t0 = now_s();
+    uint64_t nrec = 0;
+    for (int j = 0; j < n_stacks; ++j) {
+        lay_stack S;
+        nmco_layered_prepare_stack(&stacks[j], cfg->seed, &S);
+        uint64_t n = stacks[j].n_photons;
+        int ntj = (uint64_t)nt > n ? (n > 0 ? (int)n : 1) : nt;
+        lay_acc* acc = (lay_acc*)calloc((size_t)ntj, sizeof(lay_acc));
+        for (int i = 0; i < ntj; ++i) {
+            acc[i].rd = (double*)calloc((size_t)(cfg->nr + 1), sizeof(double));
+            acc[i].absorbed = (double*)calloc(NMC_MAX_LAYERS, sizeof(double));
+        }
+        if (rec_w) {
+            /* records are written in photon order: run the threads' ranges
+             * into private slots, then copy them in order */
+            for (int i = 0; i < ntj; ++i) {
+                uint64_t lo = n * (uint64_t)i / (uint64_t)ntj, hi = n * (uint64_t)(i + 1) / (uint64_t)ntj;
+                uint64_t m = hi - lo;
+                acc[i].cap = m;
+                acc[i].n_rec = (uint64_t*)calloc(1, sizeof(uint64_t));
+                acc[i].rec_item = (uint32_t*)calloc((size_t)m + 1, sizeof(uint32_t));
+                acc[i].rec_bin = (uint32_t*)calloc((size_t)m + 1, sizeof(uint32_t));
+                acc[i].rec_w = (float*)calloc((size_t)m + 1, sizeof(float));
+                acc[i].rec_L = (float*)calloc((size_t)(m + 1) * NMC_MAX_LAYERS, sizeof(float));
+                acc[i].rec_K = (uint32_t*)calloc((size_t)(m + 1) * NMC_MAX_LAYERS, sizeof(uint32_t));
+                acc[i].item = (uint32_t)j;
+            }
+        }
+#pragma omp parallel for num_threads(ntj) schedule(static)
+        for (int i = 0; i < ntj; ++i) {
+            uint64_t lo = first + n * (uint64_t)i / (uint64_t)ntj;
+            uint64_t hi = first + n * (uint64_t)(i + 1) / (uint64_t)ntj;
+            for (uint64_t p = lo; p < hi; ++p) lay_photon(&S, &c, p, &acc[i]);
+        }
+        for (int i = 0; i < ntj; ++i) {
+            for (int b = 0; b <= cfg->nr; ++b) rd[(int64_t)j * (cfg->nr + 1) + b] += acc[i].rd[b];
+            for (int b = 0; b < NMC_MAX_LAYERS; ++b) absorbed[(int64_t)j * NMC_MAX_LAYERS + b] += acc[i].absorbed[b];
+            totals_add(&totals[j], &acc[i].t);
+            if (rec_w) {
+                uint64_t m = *acc[i].n_rec;
+                for (uint64_t k = 0; k < m; ++k, ++nrec) {
+                    if (nrec >= cap) continue;
+                    rec_item[nrec] = acc[i].rec_item[k];
+                    rec_bin[nrec] = acc[i].rec_bin[k];
+                    rec_w[nrec] = acc[i].rec_w[k];
+                    for (int l = 0; l < NMC_MAX_LAYERS; ++l) {
+                        rec_L[(uint64_t)l * cap + nrec] = acc[i].rec_L[(uint64_t)l * acc[i].cap + k];
+                        rec_K[(uint64_t)l * cap + nrec] = acc[i].rec_K[(uint64_t)l * acc[i].cap + k];
+                    }
+                }
+                free(acc[i].n_rec); free(acc[i].rec_item); free(acc[i].rec_bin);
+                free(acc[i].rec_w); free(acc[i].rec_L); free(acc[i].rec_K);
+            }
+            free(acc[i].rd);
+            free(acc[i].absorbed);
+        }
+        free(acc);
+    }
+    if (n_rec) *n_rec = nrec;
+    g_last_seconds = now_s() - t0;
+    return NMC_OK;
+}
+
+/*
+ * The perturbation-MC estimator over records (the same quantity the GPU
+ * backward kernel computes): grad_mua[j][l] = sum_k g_k (-L_kl),
+ * grad_mus[j][l] = sum_k g_k (K_kl / mus_jl - L_kl), g_k = grad_rd[j][bin_k]
+ * * w_k / n_j for the records k of stack j.
+ */
+void nmco_layered_backward(const nmc_layered_config* cfg, const nmc_stack* stacks, int n_stacks,
+                           uint64_t n_rec, uint64_t cap, const uint32_t* rec_item,
+                           const uint32_t* rec_bin, const float* rec_w, const float* rec_L,
+                           const uint32_t* rec_K, const double* grad_rd, double* grad_mua,
+                           double* grad_mus)
+{
+    uint64_t m = n_rec < cap ? n_rec : cap;
+    for (uint64_t k = 0; k < m; ++k) {
+        uint32_t j = rec_item[k];
+        if ((int)j >= n_stacks) continue;
+        double g = grad_rd[(uint64_t)j * (cfg->nr + 1) + rec_bin[k]] * (double)rec_w[k] /
+                   (double)stacks[j].n_photons;
+        for (int l = 0; l < stacks[j].n_layers; ++l) {
+            double L = (double)rec_L[(uint64_t)l * cap + k];
+            double K = (double)rec_K[(uint64_t)l * cap + k];
+            grad_mua[(uint64_t)j * NMC_MAX_LAYERS + l] += -g * L;
+            grad_mus[(uint64_t)j * NMC_MAX_LAYERS + l] += g * (K / (double)stacks[j].layer[l].mus - L);
+        }
+    }
+}

@@ -45,6 +45,16 @@ int nmco_mcxyz(const RunParams* rp, const TissueParams* tp, const uint8_t* v,
 int nmco_mcsub(const RunParams* rp, const TissueParams* tp, const nmc_ext_config* ext,
                uint64_t first, uint64_t n, int n_threads, nmc_results* out);
 
+int nmco_layered(const nmc_layered_config* cfg, const nmc_stack* stacks, int n_stacks,
+                 uint64_t first, int n_threads, double* rd, double* absorbed, nmc_totals* totals,
+                 uint64_t* n_rec, uint64_t cap, uint32_t* rec_item, uint32_t* rec_bin,
+                 float* rec_w, float* rec_L, uint32_t* rec_K);
+void nmco_layered_backward(const nmc_layered_config* cfg, const nmc_stack* stacks, int n_stacks,
+                           uint64_t n_rec, uint64_t cap, const uint32_t* rec_item,
+                           const uint32_t* rec_bin, const float* rec_w, const float* rec_L,
+                           const uint32_t* rec_K, const double* grad_rd, double* grad_mua,
+                           double* grad_mus);
+
 /* Seconds of the last nmco_mcxyz / nmco_mcsub transport loop (CLOCK_MONOTONIC). */
 double nmco_last_seconds(void);
 int nmco_max_threads(void);

@@ -115,3 +115,64 @@ def run(w: Workload, n_photons: int | None = None, first: int = 0, threads: int 
 
 def max_threads() -> int:
     return lib().nmco_max_threads()
+
+
+class LayeredOut:
+    """Host results of a layered oracle run."""
+
+
+def run_layered(cfg: P.LayeredConfig, stacks, first: int = 0, threads: int = 0,
+                records: bool = True) -> LayeredOut:
+    """Oracle run of an array of Stack structs; records (if asked) in photon order."""
+    L = lib()
+    if not hasattr(L, "_lay_sig"):
+        L.nmco_layered.argtypes = [C.POINTER(P.LayeredConfig), C.POINTER(P.Stack), C.c_int,
+                                   C.c_uint64, C.c_int, C.c_void_p, C.c_void_p,
+                                   C.POINTER(P.Totals), C.POINTER(C.c_uint64), C.c_uint64,
+                                   C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.nmco_layered_backward.argtypes = [C.POINTER(P.LayeredConfig), C.POINTER(P.Stack), C.c_int,
+                                            C.c_uint64, C.c_uint64] + [C.c_void_p] * 8
+        L._lay_sig = True
+    ns = len(stacks)
+    arr = (P.Stack * ns)(*stacks)
+    o = LayeredOut()
+    o.stacks = arr
+    o.cfg = cfg
+    o.rd = np.zeros((ns, cfg.nr + 1))
+    o.absorbed = np.zeros((ns, P.MAX_LAYERS))
+    tots = (P.Totals * ns)()
+    nrec = C.c_uint64(0)
+    cap = int(sum(s.n_photons for s in stacks)) if records else 0
+    o.cap = cap
+    if records:
+        o.rec_item = np.zeros(cap, np.uint32)
+        o.rec_bin = np.zeros(cap, np.uint32)
+        o.rec_w = np.zeros(cap, np.float32)
+        o.rec_L = np.zeros((P.MAX_LAYERS, cap), np.float32)
+        o.rec_K = np.zeros((P.MAX_LAYERS, cap), np.uint32)
+        ptrs = [o.rec_item.ctypes.data, o.rec_bin.ctypes.data, o.rec_w.ctypes.data,
+                o.rec_L.ctypes.data, o.rec_K.ctypes.data]
+    else:
+        ptrs = [None] * 5
+    st = L.nmco_layered(C.byref(cfg), arr, ns, first, threads, o.rd.ctypes.data,
+                        o.absorbed.ctypes.data, tots, C.byref(nrec), cap, *ptrs)
+    if st != 0:
+        raise RuntimeError(f"oracle returned status {st}")
+    o.totals = [t.as_dict() for t in tots]
+    o.n_records = nrec.value
+    o.seconds = L.nmco_last_seconds()
+    return o
+
+
+def layered_backward(o: LayeredOut, grad_rd: np.ndarray):
+    """The perturbation-MC gradient over the oracle's records."""
+    L = lib()
+    ns = len(o.stacks)
+    g = np.ascontiguousarray(grad_rd, np.float64).reshape(ns, o.cfg.nr + 1)
+    gmua = np.zeros((ns, P.MAX_LAYERS))
+    gmus = np.zeros((ns, P.MAX_LAYERS))
+    L.nmco_layered_backward(C.byref(o.cfg), o.stacks, ns, o.n_records, o.cap,
+                            o.rec_item.ctypes.data, o.rec_bin.ctypes.data, o.rec_w.ctypes.data,
+                            o.rec_L.ctypes.data, o.rec_K.ctypes.data, g.ctypes.data,
+                            gmua.ctypes.data, gmus.ctypes.data)
+    return gmua, gmus

@@ -1,0 +1,97 @@
+"""GPU: layered transport, path records and the perturbation-MC backward
+against the oracle on identical photon streams."""
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+
+from nmc_amd import params as P
+from nmc_amd import workloads as W
+from nmc_amd.layered import LayeredMC
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _compare(gpu_res, ref, ns):
+    rd, ab, tots, nrec = gpu_res
+    assert nrec == ref.n_records
+    for j in range(ns):
+        for k in P.TOTAL_INT_FIELDS:
+            assert tots[j][k] == ref.totals[j][k], (j, k, tots[j][k], ref.totals[j][k])
+        for k in P.TOTAL_FLOAT_FIELDS:
+            assert np.isclose(tots[j][k], ref.totals[j][k], rtol=1e-9, atol=1e-9), (j, k)
+    np.testing.assert_allclose(rd, ref.rd, rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(ab, ref.absorbed, rtol=1e-9, atol=1e-12)
+
+
+def test_multilayer_matches_oracle_and_gradient(gpu):
+    cfg, stacks = W.multilayer_5(n_photons=20_000)
+    _say("[layered] 5-layer two-term, 20000 photons")
+    ref = O.run_layered(cfg, stacks, threads=8)
+    with LayeredMC(cfg, stacks) as mc:
+        mc.run()
+        res = mc.results()
+        _compare(res, ref, 1)
+        g = np.random.default_rng(0).normal(size=(1, cfg.nr + 1))
+        gmua, gmus = mc.backward(g)
+    rmua, rmus = O.layered_backward(ref, g)
+    np.testing.assert_allclose(gmua, rmua, rtol=1e-9, atol=1e-15)
+    np.testing.assert_allclose(gmus, rmus, rtol=1e-9, atol=1e-15)
+    assert np.abs(gmua[0, :5]).min() > 0
+
+
+def test_batched_stacks_match_oracle(gpu):
+    cfg, stacks = W.inverse_fit_instances(n_inst=24, budget_scale=0.002)
+    stacks[3].layer[1].phase = 0
+    stacks[3].layer[1].g = 0.0
+    stacks[5].n_above = stacks[5].layer[0].n  # matched top
+    _say(f"[layered] 24 stacks, {sum(s.n_photons for s in stacks)} photons")
+    ref = O.run_layered(cfg, stacks, threads=8)
+    with LayeredMC(cfg, stacks) as mc:
+        mc.run()
+        res = mc.results()
+        g = np.ones((24, cfg.nr + 1))
+        gmua, gmus = mc.backward(g)
+    _compare(res, ref, 24)
+    rmua, rmus = O.layered_backward(ref, g)
+    np.testing.assert_allclose(gmua, rmua, rtol=1e-9, atol=1e-15)
+    np.testing.assert_allclose(gmus, rmus, rtol=1e-9, atol=1e-15)
+
+
+def test_runs_repeat_same_photons(gpu):
+    cfg, stacks = W.multilayer_5(n_photons=5000)
+    with LayeredMC(cfg, stacks) as mc:
+        mc.run()
+        a = mc.results()
+        mc.set_properties(np.array([[s.mua for s in stacks[0].layer]]),
+                          np.array([[s.mus for s in stacks[0].layer]]))
+        mc.run()
+        b = mc.results()
+    assert a[2][0]["interactions"] == b[2][0]["interactions"]
+    np.testing.assert_array_equal(a[0], b[0])
+
+
+def test_autograd_wrapper(gpu):
+    import torch
+
+    from nmc_amd.autograd import layered_rd
+
+    cfg, stacks = W.multilayer_5(n_photons=8000)
+    mua = torch.tensor([[l.mua for l in stacks[0].layer][:5]], dtype=torch.float64, requires_grad=True)
+    mus = torch.tensor([[l.mus for l in stacks[0].layer][:5]], dtype=torch.float64, requires_grad=True)
+    with LayeredMC(cfg, stacks) as mc:
+        rd = layered_rd(mua, mus, mc)
+        w = torch.linspace(0, 1, cfg.nr + 1, dtype=torch.float64)
+        loss = (rd * w).sum()
+        loss.backward()
+        gmua, gmus = mc.backward(np.tile(w.numpy(), (1, 1)))
+    np.testing.assert_allclose(mua.grad.numpy(), gmua[:, :5], rtol=1e-12)
+    np.testing.assert_allclose(mus.grad.numpy(), gmus[:, :5], rtol=1e-12)
