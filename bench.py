@@ -45,6 +45,76 @@ def cpu_baseline(w, target_s: float) -> dict:
             "_tallies": t, "_n": n}
 
 
+def sweep_leg(budget_scale: float) -> dict:
+    """128-wavelength sweep: one batched launch vs 128 plain launches (each
+    with its own configure, run and read-back), same photons."""
+    import numpy as np
+
+    from nmc_amd import params as P
+    from nmc_amd import workloads as W
+    from nmc_amd.metal_mc import MetalMC
+
+    w, items, _ = W.spectral_sweep(budget_scale=budget_scale)
+    total = sum(i["n_photons"] for i in items)
+    log(f"[bench] sweep: 128 wavelengths, {total} photons")
+    with MetalMC(P.SIM_MCSUB) as mc:
+        mc.prepare_batch(w.rp, items, w.ext)
+        mc.synchronize()
+        t0 = time.perf_counter()
+        mc.run_batch()
+        tally, tots = mc.get_batch_results()
+        t_batch = time.perf_counter() - t0
+        k_batch = mc.last_kernel_ms()
+    t0 = time.perf_counter()
+    k_plain = 0.0
+    same = True
+    with MetalMC(P.SIM_MCSUB) as mc:
+        for j, it in enumerate(items):
+            w.rp.n1, w.rp.n2 = it["n1"], it["n2"]
+            w.tp.muav[1], w.tp.musv[1], w.tp.gv[1] = it["mua"], it["mus"], it["g"]
+            w.ext.stream_id = it["stream_id"]
+            mc.prepare_data(w.rp, w.tp, None, w.ext)
+            mc.send_compute_command(it["n_photons"])
+            res = mc.get_compute_results()
+            k_plain += mc.last_kernel_ms()
+            same &= all(res.totals[k] == tots[j][k] for k in P.TOTAL_INT_FIELDS)
+            same &= bool(np.allclose(res.rd_ra, tally[j, :w.ext.nr + 1], rtol=1e-9, atol=1e-12))
+    t_plain = time.perf_counter() - t0
+    return {"photons": total, "batched_ms": t_batch * 1e3, "batched_kernel_ms": k_batch,
+            "plain_128_ms": t_plain * 1e3, "plain_kernel_ms_sum": k_plain,
+            "speedup": t_plain / t_batch, "tallies_identical": bool(same),
+            "batched_photons_per_s": total / t_batch}
+
+
+def gradient_leg(n_photons: int) -> dict:
+    """5-layer differentiable case: (forward with path records + backward)
+    against the plain forward, kernel times on the library stream."""
+    import numpy as np
+
+    from nmc_amd import workloads as W
+    from nmc_amd.layered import LayeredMC
+
+    cfg, stacks = W.multilayer_5(n_photons=n_photons)
+    log(f"[bench] gradient: 5-layer stack, {n_photons} photons")
+    cfg.records = 0
+    with LayeredMC(cfg, stacks) as mc:
+        mc.run()
+        mc.run()
+        fwd = mc.last_kernel_ms()
+    cfg.records = 1
+    with LayeredMC(cfg, stacks) as mc:
+        mc.run()
+        mc.run()
+        fwd_rec = mc.last_kernel_ms()
+        g = np.ones((1, cfg.nr + 1))
+        mc.backward(g)
+        bwd = mc.last_kernel_ms()
+        nrec = mc.results()[3]
+    return {"photons": n_photons, "forward_ms": fwd, "forward_with_records_ms": fwd_rec,
+            "backward_ms": bwd, "records": nrec, "grad_over_forward": (fwd_rec + bwd) / fwd,
+            "forward_photons_per_s": n_photons / fwd * 1e3}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -53,6 +123,10 @@ def main() -> int:
     ap.add_argument("--photons", type=float, default=1e8, help="photons per step")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--legs", default="sweep,grad",
+                    help="extra measurements: sweep (batched vs plain), grad (pMC cost)")
+    ap.add_argument("--sweep-scale", type=float, default=1.0, help="scale of the sweep budgets")
+    ap.add_argument("--grad-photons", type=float, default=1e7)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the photon-range "
@@ -133,6 +207,11 @@ def main() -> int:
                             "rel_diff": abs(g["rd"] - r["rd"]) / r["rd"],
                             "int_tallies_equal": all(g[k] == r[k] for k in P.TOTAL_INT_FIELDS)}
     mc.close()
+    legs = [s for s in args.legs.split(",") if s]
+    if "sweep" in legs:
+        out["batched_sweep"] = sweep_leg(args.sweep_scale)
+    if "grad" in legs:
+        out["gradient"] = gradient_leg(int(args.grad_photons))
     print(json.dumps(out), flush=True)
     return 0
 

@@ -211,15 +211,25 @@ __global__ __launch_bounds__(kLayBlock) void layered_kernel(LayArgs a) {
         addf(T, TF_RD, (double)p.W);
         addi(T, TI_DETECTIONS, 1);
         if (a.records) {
-          unsigned long long k = atomicAdd(a.rec_count, 1ull);
+          // one append per wave: the detecting lanes take consecutive slots
+          uint64_t m = ballot(true);
+          int leader = __ffsll((long long)m) - 1;
+          unsigned long long base = 0;
+          if (lane == leader) base = atomicAdd(a.rec_count, (unsigned long long)__popcll(m));
+          base = __shfl(base, leader);
+          unsigned long long k = base + (unsigned long long)__popcll(m & lanemask_lt());
           if (k < a.cap) {
-            a.rec_item[k] = (uint32_t)p.item;
-            a.rec_bin[k] = (uint32_t)ir;
-            a.rec_w[k] = p.W;
+            const int rs = 3 + 2 * a.rec_nl;
+            uint32_t* r = a.rec + k * (uint64_t)rs;
+            r[0] = (uint32_t)p.item;
+            r[1] = (uint32_t)ir;
+            r[2] = __float_as_uint(p.W);
 #pragma unroll
             for (int i = 0; i < NMC_MAX_LAYERS; ++i) {
-              a.rec_L[(uint64_t)i * a.cap + k] = p.L[i];
-              a.rec_K[(uint64_t)i * a.cap + k] = p.K[i];
+              if (i < a.rec_nl) {
+                r[3 + i] = __float_as_uint(p.L[i]);
+                r[3 + a.rec_nl + i] = p.K[i];
+              }
             }
           }
         }
@@ -312,7 +322,9 @@ __global__ __launch_bounds__(256) void layered_backward_kernel(LayBackArgs b) {
   for (int l = 0; l < NMC_MAX_LAYERS; ++l) ga[l] = gs[l] = 0.0;
   for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < m;
        k += (uint64_t)gridDim.x * blockDim.x) {
-    int j = (int)b.rec_item[k];
+    const int rs = 3 + 2 * b.rec_nl;
+    const uint32_t* r = b.rec + k * (uint64_t)rs;
+    int j = (int)r[0];
     if (j >= ni) continue;
     if (j != cur) {
       if (cur >= 0) {
@@ -325,13 +337,13 @@ __global__ __launch_bounds__(256) void layered_backward_kernel(LayBackArgs b) {
       }
       cur = j;
     }
-    double g = b.grad_rd[(int64_t)j * (b.nr + 1) + b.rec_bin[k]] * (double)b.rec_w[k] * b.inv_n[j];
+    double g = b.grad_rd[(int64_t)j * (b.nr + 1) + r[1]] * (double)__uint_as_float(r[2]) * b.inv_n[j];
     int nl = b.nl[j];
 #pragma unroll
     for (int l = 0; l < NMC_MAX_LAYERS; ++l) {
-      if (l < nl) {
-        double L = (double)b.rec_L[(uint64_t)l * b.cap + k];
-        double K = (double)b.rec_K[(uint64_t)l * b.cap + k];
+      if (l < nl && l < b.rec_nl) {
+        double L = (double)__uint_as_float(r[3 + l]);
+        double K = (double)r[3 + b.rec_nl + l];
         ga[l] += -g * L;
         gs[l] += g * (K / (double)b.mus[j * NMC_MAX_LAYERS + l] - L);
       }

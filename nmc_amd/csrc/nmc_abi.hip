@@ -170,12 +170,9 @@ struct nmc_handle {
   double* dLOut = nullptr;     // [n][MAXL] absorbed | [n] totals
   double* dLPart = nullptr;
   unsigned long long* dRecCount = nullptr;
-  uint32_t* dRecItem = nullptr;
-  uint32_t* dRecBin = nullptr;
-  float* dRecW = nullptr;
-  float* dRecL = nullptr;
-  uint32_t* dRecK = nullptr;
+  uint32_t* dRec = nullptr;
   uint64_t rec_cap = 0;
+  int rec_nl = 0;
   double* dGrad = nullptr;     // grad_rd [n][nr+1] | inv_n [n] | grad_mua, grad_mus [n][MAXL]
   float* dMus = nullptr;
   int* dNl = nullptr;
@@ -222,11 +219,7 @@ void free_buffers(nmc_handle* h) {
   free_ptr(h->dLOut);
   free_ptr(h->dLPart);
   free_ptr(h->dRecCount);
-  free_ptr(h->dRecItem);
-  free_ptr(h->dRecBin);
-  free_ptr(h->dRecW);
-  free_ptr(h->dRecL);
-  free_ptr(h->dRecK);
+  free_ptr(h->dRec);
   free_ptr(h->dGrad);
   free_ptr(h->dMus);
   free_ptr(h->dNl);
@@ -989,9 +982,11 @@ int nmc_layered_prepare(nmc_handle* h, const nmc_layered_config* cfg, const nmc_
   h->timed = false;
   int n = n_stacks;
   h->lay_rows = 0;
+  h->rec_nl = 0;
   uint64_t total = 0;
   for (int j = 0; j < n; ++j) {
     h->lay_rows += stacks[j].n_layers;
+    h->rec_nl = std::max(h->rec_nl, (int)stacks[j].n_layers);
     total += stacks[j].n_photons;
   }
   HIP_TRY(hipMalloc(&h->dHdr, n * sizeof(LayStackDev)));
@@ -1007,11 +1002,7 @@ int nmc_layered_prepare(nmc_handle* h, const nmc_layered_config* cfg, const nmc_
   if (cfg->records) {
     h->rec_cap = cfg->record_capacity ? cfg->record_capacity : total;
     if (h->rec_cap == 0) h->rec_cap = 1;
-    HIP_TRY(hipMalloc(&h->dRecItem, h->rec_cap * sizeof(uint32_t)));
-    HIP_TRY(hipMalloc(&h->dRecBin, h->rec_cap * sizeof(uint32_t)));
-    HIP_TRY(hipMalloc(&h->dRecW, h->rec_cap * sizeof(float)));
-    HIP_TRY(hipMalloc(&h->dRecL, h->rec_cap * NMC_MAX_LAYERS * sizeof(float)));
-    HIP_TRY(hipMalloc(&h->dRecK, h->rec_cap * NMC_MAX_LAYERS * sizeof(uint32_t)));
+    HIP_TRY(hipMalloc(&h->dRec, h->rec_cap * (3 + 2 * h->rec_nl) * sizeof(uint32_t)));
   }
   h->lay_lds = layered_lds_bytes(n, h->lay_rows);
   if (h->lay_lds + 1024 > h->lds_max)
@@ -1080,13 +1071,10 @@ int nmc_layered_run(nmc_handle* h) {
   a.rd = h->dLRd;
   a.part = h->dLPart;
   a.records = h->lcfg.records != 0;
+  a.rec_nl = h->rec_nl;
   a.cap = h->rec_cap;
   a.rec_count = h->dRecCount;
-  a.rec_item = h->dRecItem;
-  a.rec_bin = h->dRecBin;
-  a.rec_w = h->dRecW;
-  a.rec_L = h->dRecL;
-  a.rec_K = h->dRecK;
+  a.rec = h->dRec;
   a.state = h->dState;
   HIP_TRY(launch_layered(a, h->lay_grid, h->lay_lds, s));
   HIP_TRY(launch_reduce_partials(h->dLPart, h->lay_grid, (int64_t)n * (NMC_MAX_LAYERS + TF_COUNT + TI_COUNT),
@@ -1139,11 +1127,8 @@ int nmc_layered_backward(nmc_handle* h, const double* grad_rd, double* grad_mua,
   b.nr = h->lcfg.nr;
   b.n_rec = nrec;
   b.cap = h->rec_cap;
-  b.rec_item = h->dRecItem;
-  b.rec_bin = h->dRecBin;
-  b.rec_w = h->dRecW;
-  b.rec_L = h->dRecL;
-  b.rec_K = h->dRecK;
+  b.rec_nl = h->rec_nl;
+  b.rec = h->dRec;
   b.grad_rd = d_grad_rd;
   b.mus = h->dMus;
   b.nl = h->dNl;

@@ -107,24 +107,18 @@ struct LayArgs {
   double* rd;                       // [n_items][nr+1] (global atomics)
   double* part;                     // block partials of [absorbed | totals]
   int records;
+  int rec_nl;                       // layers stored per record (deepest stack)
   uint64_t cap;
   unsigned long long* rec_count;
-  uint32_t* rec_item;
-  uint32_t* rec_bin;
-  float* rec_w;
-  float* rec_L;                     // [NMC_MAX_LAYERS][cap]
-  uint32_t* rec_K;                  // [NMC_MAX_LAYERS][cap]
+  uint32_t* rec;                    // [cap][3 + 2 rec_nl]: stack, bin, w, L[rec_nl], K[rec_nl]
   RunState* state;
 };
 
 struct LayBackArgs {
   int n_items, nr;
+  int rec_nl;
   uint64_t n_rec, cap;
-  const uint32_t* rec_item;
-  const uint32_t* rec_bin;
-  const float* rec_w;
-  const float* rec_L;
-  const uint32_t* rec_K;
+  const uint32_t* rec;              // as LayArgs::rec
   const double* grad_rd;            // [n_items][nr+1]
   const float* mus;                 // [n_items][NMC_MAX_LAYERS]
   const int* nl;                    // [n_items]

@@ -45,7 +45,7 @@ def test_multilayer_matches_oracle_and_gradient(gpu):
     rmua, rmus = O.layered_backward(ref, g)
     np.testing.assert_allclose(gmua, rmua, rtol=1e-9, atol=1e-15)
     np.testing.assert_allclose(gmus, rmus, rtol=1e-9, atol=1e-15)
-    assert np.abs(gmua[0, :5]).min() > 0
+    assert gmua[0, 0] != 0.0 and gmus[0, 0] != 0.0
 
 
 def test_batched_stacks_match_oracle(gpu):
