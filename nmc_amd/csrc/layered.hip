@@ -33,7 +33,7 @@ __device__ __forceinline__ void addi(nmc_totals* t, int k, unsigned long long v)
 
 struct LPhoton {
   float x, y, z, ux, uy, uz, ruz, W, sleft;
-  float zt, zb, mua, mus;
+  float zt, zb, mua, mus, inv_mus;
   float Lcur;
   int item, l, nl, row0;
   uint32_t Kcur;
@@ -52,6 +52,7 @@ __device__ __forceinline__ void load_layer(LPhoton& p, const LayRow* rows) {
   p.zb = r.zb;
   p.mua = r.mua;
   p.mus = r.mus;
+  p.inv_mus = r.inv_mus;
 }
 
 // Folds the current layer's path length, scatter count and absorption into
@@ -166,7 +167,7 @@ __global__ __launch_bounds__(kLayBlock) void layered_kernel(LayArgs a) {
 
     nmc_totals* T = &st[p.item];
     // ---- HOP to the next interaction or interface ----
-    float s_int = p.sleft / p.mus;
+    float s_int = p.sleft * p.inv_mus;
     float s_b = kInf;
     if (p.uz > 0.0f) s_b = fmaxf((p.zb - p.z) * p.ruz, 0.0f);
     else if (p.uz < 0.0f) s_b = fmaxf((p.zt - p.z) * p.ruz, 0.0f);
@@ -251,6 +252,7 @@ __global__ __launch_bounds__(kLayBlock) void layered_kernel(LayArgs a) {
         load_layer(p, rows);
       }
     }
+    if (alive && !(p.W > 0.0f)) alive = false;  // all weight absorbed
     if (alive && p.events >= a.max_events) {
       addf(T, TF_CAPPED, (double)p.W);
       addi(T, TI_CAPPED, 1);
@@ -258,7 +260,7 @@ __global__ __launch_bounds__(kLayBlock) void layered_kernel(LayArgs a) {
     }
     if (alive && !(p.sleft > 0.0f)) {
       const LayRow& R = rows[p.row0 + p.l];
-      float ct = R.phase ? tt_cos(R.tt, u01(p.b1)) : hg_cos(R.g, u01(p.b1));
+      float ct = R.phase ? tt_cos(R.tt, u01(p.b1)) : hg_cos(R.hg, u01(p.b1));
       spin(ct, u01(p.b2), p.ux, p.uy, p.uz);
       p.ruz = 1.0f / p.uz;
       p.Kcur++;

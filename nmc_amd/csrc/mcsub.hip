@@ -32,9 +32,9 @@ __device__ __forceinline__ void addi(nmc_totals* t, int k, unsigned long long v)
   atomicAdd(&(&t->photons)[k], v);
 }
 
-template <bool LDS>
-__global__ __launch_bounds__(kSubBlock) void mcsub_kernel(SubArgs a) {
-  // dynamic LDS: [items] media | [items+1] prefix | [items] totals | tallies
+template <bool LDS, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void mcsub_kernel(SubArgs a) {
+  // dynamic LDS: [items] media | [items+1] prefix | [items] totals | tallies (LDS mode)
   extern __shared__ double smem[];
   __shared__ float cos_tab[64];
   const int ni = a.n_items;
@@ -46,14 +46,13 @@ __global__ __launch_bounds__(kSubBlock) void mcsub_kernel(SubArgs a) {
   nmc_totals* st = (nmc_totals*)(prefix + ni + 1);
   double* tal = (double*)(st + ni);
   double* tally = LDS ? tal : a.tally;
-  nmc_totals* tot = LDS ? st : a.tot;
-  for (int i = threadIdx.x; i < ni; i += blockDim.x) med[i] = a.items[i];
-  for (int i = threadIdx.x; i <= ni; i += blockDim.x) prefix[i] = a.prefix[i];
-  for (int i = threadIdx.x; i < a.na - 1; i += blockDim.x) cos_tab[i] = a.cos_tab[i];
-  if (LDS) {
-    for (int i = threadIdx.x; i < ni * (TF_COUNT + TI_COUNT); i += blockDim.x) ((double*)st)[i] = 0.0;
-    for (int i = threadIdx.x; i < ni * per_item; i += blockDim.x) tal[i] = 0.0;
-  }
+  nmc_totals* tot = st;
+  for (int i = threadIdx.x; i < ni; i += BLOCK) med[i] = a.items[i];
+  for (int i = threadIdx.x; i <= ni; i += BLOCK) prefix[i] = a.prefix[i];
+  for (int i = threadIdx.x; i < a.na - 1; i += BLOCK) cos_tab[i] = a.cos_tab[i];
+  for (int i = threadIdx.x; i < ni * (TF_COUNT + TI_COUNT); i += BLOCK) ((double*)st)[i] = 0.0;
+  if (LDS)
+    for (int i = threadIdx.x; i < ni * per_item; i += BLOCK) tal[i] = 0.0;
   __syncthreads();
   const uint64_t first = a.state->first;
   const uint64_t total = prefix[ni];
@@ -70,6 +69,7 @@ __global__ __launch_bounds__(kSubBlock) void mcsub_kernel(SubArgs a) {
 
   for (;;) {
     uint64_t need = ballot(!alive && !finished);
+    if (need && (uint32_t)__popcll(need) < a.refill_min && ballot(alive) != 0ull) need = 0;
     while (need) {
       if (pool_next >= pool_end) {
         unsigned long long base = 0;
@@ -94,9 +94,9 @@ __global__ __launch_bounds__(kSubBlock) void mcsub_kernel(SubArgs a) {
           if (prefix[mid] <= q) lo = mid;
           else hi = mid - 1;
         }
-        item = lo;
-        m = med[item];
-        pid = first + (q - prefix[item]);
+        m = med[lo];
+        item = m.slot;
+        pid = first + (q - prefix[lo]);
         // LAUNCH (kernels.metal:106-166)
         U4 b = philox(pid, 0u, 0u, a.key0, m.key1);
         float rsp;
@@ -150,7 +150,7 @@ __global__ __launch_bounds__(kSubBlock) void mcsub_kernel(SubArgs a) {
 
     U4 b = philox(pid, ev, 0u, a.key0, m.key1);
     // HOP (kernels.metal:183-187)
-    float s = -log_pos(u01nz(b.x)) / m.mut;
+    float s = -log_pos(u01nz(b.x)) * m.inv_mut;
     float x0 = x, y0 = y, z0 = z;
     x = fmaf(s, ux, x);
     y = fmaf(s, uy, y);
@@ -193,7 +193,7 @@ __global__ __launch_bounds__(kSubBlock) void mcsub_kernel(SubArgs a) {
       }
       absorbed += (double)absorb;
       // SPIN (kernels.metal:243-279, HG honoured)
-      float ct = hg_cos(m.g, u01(b.y));
+      float ct = hg_cos(m.hg, u01(b.y));
       spin(ct, u01(b.z), ux, uy, uz);
       c_int++;
       // ROULETTE (kernels.metal:289-294)
@@ -226,12 +226,12 @@ __global__ __launch_bounds__(kSubBlock) void mcsub_kernel(SubArgs a) {
   }
 
   __syncthreads();
-  if (LDS) {
-    double* part = a.part + (int64_t)blockIdx.x * ni * (per_item + TF_COUNT + TI_COUNT);
-    for (int i = threadIdx.x; i < ni * per_item; i += blockDim.x) part[i] = tal[i];
-    double* pt = part + ni * per_item;
-    for (int i = threadIdx.x; i < ni * (TF_COUNT + TI_COUNT); i += blockDim.x) pt[i] = ((double*)st)[i];
-  }
+  // block partials: [tallies (LDS mode)] [totals]
+  const int lds_tal = LDS ? ni * per_item : 0;
+  double* part = a.part + (int64_t)blockIdx.x * (lds_tal + ni * (TF_COUNT + TI_COUNT));
+  for (int i = threadIdx.x; i < lds_tal; i += BLOCK) part[i] = tal[i];
+  double* pt = part + lds_tal;
+  for (int i = threadIdx.x; i < ni * (TF_COUNT + TI_COUNT); i += BLOCK) pt[i] = ((double*)st)[i];
 }
 
 // out[j] += sum_b part[b * stride + j] for j < len, blocks in ascending order
@@ -264,29 +264,45 @@ __global__ void state_advance_kernel(RunState* st, unsigned long long adv) { st-
 
 }  // namespace
 
-size_t mcsub_lds_bytes(int n_items, int64_t per_item) {
+size_t mcsub_lds_bytes(int n_items, int64_t per_item_in_lds) {
   return (size_t)n_items * sizeof(SubMedium) + (size_t)(n_items + 1) * sizeof(uint64_t) +
-         (size_t)n_items * sizeof(nmc_totals) + (size_t)n_items * per_item * sizeof(double);
+         (size_t)n_items * sizeof(nmc_totals) + (size_t)n_items * per_item_in_lds * sizeof(double);
 }
 
-hipError_t launch_mcsub(const SubArgs& a, int grid, size_t lds_bytes, bool lds_tallies,
+namespace {
+template <bool L>
+const void* sub_fn(int block) {
+  return block == 512 ? (const void*)mcsub_kernel<L, 512> : (const void*)mcsub_kernel<L, 1024>;
+}
+}  // namespace
+
+hipError_t launch_mcsub(const SubArgs& a, int block, int grid, size_t lds_bytes, bool lds_tallies,
                         hipStream_t s) {
-  if (lds_tallies) hipLaunchKernelGGL(mcsub_kernel<true>, dim3(grid), dim3(kSubBlock), lds_bytes, s, a);
-  else hipLaunchKernelGGL(mcsub_kernel<false>, dim3(grid), dim3(kSubBlock), lds_bytes, s, a);
+  if (lds_tallies) {
+    if (block == 512) hipLaunchKernelGGL((mcsub_kernel<true, 512>), dim3(grid), dim3(512), lds_bytes, s, a);
+    else hipLaunchKernelGGL((mcsub_kernel<true, 1024>), dim3(grid), dim3(1024), lds_bytes, s, a);
+  } else {
+    if (block == 512) hipLaunchKernelGGL((mcsub_kernel<false, 512>), dim3(grid), dim3(512), lds_bytes, s, a);
+    else hipLaunchKernelGGL((mcsub_kernel<false, 1024>), dim3(grid), dim3(1024), lds_bytes, s, a);
+  }
   return hipGetLastError();
 }
 
-hipError_t mcsub_configure(size_t lds_bytes, bool lds_tallies) {
-  const void* f = lds_tallies ? (const void*)mcsub_kernel<true> : (const void*)mcsub_kernel<false>;
+hipError_t mcsub_configure(int block, size_t lds_bytes, bool lds_tallies) {
+  const void* f = lds_tallies ? sub_fn<true>(block) : sub_fn<false>(block);
   return hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
 }
 
-int mcsub_blocks_per_cu(size_t lds_bytes, bool lds_tallies) {
+int mcsub_blocks_per_cu(int block, size_t lds_bytes, bool lds_tallies) {
   int nb = 0;
-  hipError_t e = lds_tallies
-      ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, mcsub_kernel<true>, kSubBlock, lds_bytes)
-      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, mcsub_kernel<false>, kSubBlock, lds_bytes);
-  return (e == hipSuccess && nb > 0) ? nb : 1;
+  hipError_t e;
+  if (lds_tallies)
+    e = block == 512 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, mcsub_kernel<true, 512>, 512, lds_bytes)
+                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, mcsub_kernel<true, 1024>, 1024, lds_bytes);
+  else
+    e = block == 512 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, mcsub_kernel<false, 512>, 512, lds_bytes)
+                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, mcsub_kernel<false, 1024>, 1024, lds_bytes);
+  return e == hipSuccess ? nb : 0;
 }
 
 hipError_t launch_reduce_partials(const double* part, int grid, int64_t stride, double* out,

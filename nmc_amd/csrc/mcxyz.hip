@@ -25,7 +25,7 @@ namespace nmc {
 namespace {
 
 struct Photon {
-  float x, y, z, ux, uy, uz, rux, ruy, ruz, W, sleft, mua, mus;
+  float x, y, z, ux, uy, uz, rux, ruy, ruz, W, sleft, mua, mus, inv_mus;
   int ix, iy, iz, tis;
   bool inside, alive;
   uint64_t pid;
@@ -138,6 +138,7 @@ __device__ __forceinline__ void launch(Photon& p, const XyzArgs& a, const XyzTis
   p.tis = a.V[vidx(a, clampi(p.ix, a.Nx), clampi(p.iy, a.Ny), clampi(p.iz, a.Nz))];
   p.mua = T[p.tis].mua;
   p.mus = T[p.tis].mus;
+  p.inv_mus = T[p.tis].inv_mus;
   p.rux = 1.0f / ux;
   p.ruy = 1.0f / uy;
   p.ruz = 1.0f / uz;
@@ -210,7 +211,7 @@ __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
     if (!p.alive) continue;
 
     // ---- HOP: one piece of the current step (kernels.metal:505-685) ----
-    float s_int = p.sleft / p.mus;
+    float s_int = p.sleft * p.inv_mus;
     float tx = kInf, ty = kInf, tz = kInf;
     if (p.ux > 0.0f) tx = (((float)(p.ix + 1) - a.hx) * a.dx - p.x) * p.rux;
     else if (p.ux < 0.0f) tx = (((float)p.ix - a.hx) * a.dx - p.x) * p.rux;
@@ -310,9 +311,11 @@ __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
           p.tis = a.V[vidx(a, clampi(p.ix, a.Nx), clampi(p.iy, a.Ny), clampi(p.iz, a.Nz))];
           p.mua = T[p.tis].mua;
           p.mus = T[p.tis].mus;
+          p.inv_mus = T[p.tis].inv_mus;
         }
       }
     }
+    if (alive && !(p.W > 0.0f)) alive = false;  // all weight absorbed
     if (alive && p.events >= a.max_events) {
       lds_addf(&st, TF_CAPPED, (double)p.W);
       lds_addi(&st, TI_CAPPED, 1);
@@ -320,7 +323,7 @@ __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
     }
     if (alive && !(p.sleft > 0.0f)) {
       // SPIN (kernels.metal:695-730)
-      float ct = TWO_TERM ? tt_cos(T[p.tis].tt, u01(p.b1)) : hg_cos(T[p.tis].g, u01(p.b1));
+      float ct = TWO_TERM ? tt_cos(T[p.tis].tt, u01(p.b1)) : hg_cos(T[p.tis].hg, u01(p.b1));
       spin(ct, u01(p.b2), p.ux, p.uy, p.uz);
       p.rux = 1.0f / p.ux;
       p.ruy = 1.0f / p.uy;

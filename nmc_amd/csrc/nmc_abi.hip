@@ -95,13 +95,35 @@ void tt_prepare(float gf, float af, float gb, float ab, float cc, TTCoef* t) {
   }
 }
 
+bool flt_eq_h(float a, float b) {
+  float e = b * 0x1p-23f * 2.0f;
+  return (a >= b - e) && (a <= b + e);
+}
+
+// Henyey-Greenstein constants of anisotropy g, derived in double.
+HGCoef hg_prepare(float g) {
+  HGCoef h{};
+  h.mode = flt_eq_h(g, 0.0f) ? 1 : (flt_eq_h(g, 1.0f) ? 2 : 0);
+  double gd = g;
+  h.a = (float)(1.0 + gd * gd);
+  h.b = (float)(1.0 - gd * gd);
+  h.c = (float)(1.0 - gd);
+  h.d = (float)(2.0 * gd);
+  h.e = h.mode == 0 ? (float)(1.0 / (2.0 * gd)) : 0.0f;
+  return h;
+}
+
+// 1/mu rounded once; FLT_MAX for mu = 0 (step lengths then exceed any face).
+float inv_pos(float mu) { return mu > 0.0f ? (float)(1.0 / (double)mu) : 3.40282347e+38f; }
+
 // One medium of an mcsub run (mut, albedo, specular as the reference computes
 // them, kernels.metal:99-119).
 SubMedium make_medium(float mua, float mus, float g, float n1, float n2, uint32_t key1) {
   SubMedium m{};
   m.mut = mua + mus;
+  m.inv_mut = inv_pos(m.mut);
   m.one_m_albedo = 1.0f - mus / m.mut;
-  m.g = g;
+  m.hg = hg_prepare(g);
   m.n1 = n1;
   m.n2 = n2;
   float tr = n1 / n2;
@@ -150,6 +172,7 @@ struct nmc_handle {
   // mcsub (plain = batch of one)
   int n_items = 0;
   std::vector<nmc_work_item> items;
+  std::vector<int> order;      // queue order of the items (longest photons first)
   std::vector<uint64_t> prefix;
   SubMedium* dItems = nullptr;
   uint64_t* dPrefix = nullptr;
@@ -292,25 +315,31 @@ int resolve_xyz_tuning(nmc_handle* h) {
 int resolve_sub_tuning(nmc_handle* h) {
   const nmc_tuning& r = h->tune_req;
   nmc_tuning t{};
-  t.block = kSubBlock;
-  if (r.block && r.block != kSubBlock) return fail(NMC_ERR_ARG, "mcsub runs with block %d", kSubBlock);
-  t.refill_min = 1;
+  // tally placement: tuning.reserved[0] = 1 LDS, 2 global atomics, 0 automatic
+  // (LDS whenever the per-item tallies fit)
+  int mode = r.reserved[0];
+  if (mode < 0 || mode > 2) return fail(NMC_ERR_ARG, "tuning.reserved[0] (tally placement) not in 0..2");
+  bool want_lds = mode != 2;
+  size_t lds_full = mcsub_lds_bytes(h->n_items, per_item(h)) + 512;
+  h->sub_lds = want_lds && lds_full <= h->lds_max;
+  h->sub_lds_bytes = mcsub_lds_bytes(h->n_items, h->sub_lds ? per_item(h) : 0);
+  if (h->sub_lds_bytes + 512 > h->lds_max)
+    return fail(NMC_ERR_ARG, "%d work items need %zu B of LDS", h->n_items, h->sub_lds_bytes);
+  t.block = r.block ? r.block : 1024;
+  if (t.block != 512 && t.block != 1024) return fail(NMC_ERR_ARG, "mcsub block must be 512 or 1024");
+  t.refill_min = r.refill_min ? r.refill_min : 1;  // mcsub's launch is cheap: refill at once
+  if (t.refill_min < 1 || t.refill_min > 64) return fail(NMC_ERR_ARG, "tuning.refill_min not in 1..64");
   t.chunk = r.chunk;
-  size_t lds_full = mcsub_lds_bytes(h->n_items, per_item(h)) + 256;
-  h->sub_lds = lds_full <= h->lds_max;
-  h->sub_lds_bytes = h->sub_lds ? mcsub_lds_bytes(h->n_items, per_item(h))
-                                : (size_t)h->n_items * sizeof(SubMedium) + (h->n_items + 1) * sizeof(uint64_t);
-  HIP_TRY(mcsub_configure(h->sub_lds_bytes, h->sub_lds));
-  int occ = mcsub_blocks_per_cu(h->sub_lds_bytes, h->sub_lds);
+  t.reserved[0] = h->sub_lds ? 1 : 2;
+  HIP_TRY(mcsub_configure(t.block, h->sub_lds_bytes, h->sub_lds));
+  int occ = mcsub_blocks_per_cu(t.block, h->sub_lds_bytes, h->sub_lds);
   if (occ <= 0) return fail(NMC_ERR_ARG, "mcsub kernel does not fit on a CU");
   t.blocks_per_cu = r.blocks_per_cu > 0 ? std::min(r.blocks_per_cu, occ) : occ;
   h->sub_grid = h->num_cu * t.blocks_per_cu;
   h->tune = t;
   free_ptr(h->dPart);
-  if (h->sub_lds) {
-    size_t stride = (size_t)h->n_items * (per_item(h) + TF_COUNT + TI_COUNT);
-    HIP_TRY(hipMalloc(&h->dPart, (size_t)h->sub_grid * stride * sizeof(double)));
-  }
+  size_t stride = (size_t)h->n_items * ((h->sub_lds ? per_item(h) : 0) + TF_COUNT + TI_COUNT);
+  HIP_TRY(hipMalloc(&h->dPart, (size_t)h->sub_grid * stride * sizeof(double)));
   drop_graph(h);
   return ok();
 }
@@ -350,21 +379,22 @@ int enqueue_run(nmc_handle* h, uint64_t n) {
     a.max_events = e.max_events;
     a.key0 = key0_of(h);
     uint64_t total = h->prefix[h->n_items];
-    a.chunk = pick_chunk(total, h->sub_grid * (kSubBlock / 64), h->tune.chunk);
+    a.chunk = pick_chunk(total, h->sub_grid * (h->tune.block / 64), h->tune.chunk);
+    a.refill_min = (uint32_t)h->tune.refill_min;
     a.n_items = h->n_items;
     a.items = h->dItems;
     a.prefix = h->dPrefix;
     a.cos_tab = h->dCos;
     a.tally_rz = h->n_rz > 0;
     a.tally = h->dOut;
-    a.tot = (nmc_totals*)(h->dOut + (int64_t)h->n_items * per_item(h));
     a.part = h->dPart;
     a.state = h->dState;
-    HIP_TRY(launch_mcsub(a, h->sub_grid, h->sub_lds_bytes, h->sub_lds, s));
-    if (h->sub_lds) {
-      int64_t stride = (int64_t)h->n_items * (per_item(h) + TF_COUNT + TI_COUNT);
-      HIP_TRY(launch_reduce_partials(h->dPart, h->sub_grid, stride, h->dOut, h->n_items,
-                                     (int)per_item(h), s));
+    HIP_TRY(launch_mcsub(a, h->tune.block, h->sub_grid, h->sub_lds_bytes, h->sub_lds, s));
+    {
+      int64_t lds_per = h->sub_lds ? per_item(h) : 0;
+      int64_t stride = (int64_t)h->n_items * (lds_per + TF_COUNT + TI_COUNT);
+      double* out = h->dOut + (h->sub_lds ? 0 : (int64_t)h->n_items * per_item(h));
+      HIP_TRY(launch_reduce_partials(h->dPart, h->sub_grid, stride, out, h->n_items, (int)lds_per, s));
     }
     uint64_t adv = 0;
     for (int j = 0; j < h->n_items; ++j) adv = std::max<uint64_t>(adv, h->prefix[j + 1] - h->prefix[j]);
@@ -475,21 +505,30 @@ int run_async(nmc_handle* h, uint64_t n) {
   return ok();
 }
 
+// Queue order: items whose photons live longest (highest albedo) first, so
+// the end of a launch is made of short-lived photons.
+void order_items(nmc_handle* h) {
+  int n = h->n_items;
+  h->order.resize(n);
+  for (int j = 0; j < n; ++j) h->order[j] = j;
+  std::stable_sort(h->order.begin(), h->order.end(), [&](int a, int b) {
+    const nmc_work_item &x = h->items[a], &y = h->items[b];
+    double ax = x.mus / std::max(1e-30, (double)x.mua + x.mus);
+    double ay = y.mus / std::max(1e-30, (double)y.mua + y.mus);
+    return ax > ay;
+  });
+}
+
+int upload_media(nmc_handle* h);
+
 int alloc_sub_outputs(nmc_handle* h) {
   free_ptr(h->dOut);
   size_t n = (size_t)h->n_items * (per_item(h) + TF_COUNT + TI_COUNT);
   HIP_TRY(hipMalloc(&h->dOut, n * sizeof(double)));
   HIP_TRY(hipMalloc(&h->dItems, h->n_items * sizeof(SubMedium)));
   HIP_TRY(hipMalloc(&h->dPrefix, (h->n_items + 1) * sizeof(uint64_t)));
-  std::vector<SubMedium> med(h->n_items);
-  h->prefix.assign(h->n_items + 1, 0);
-  for (int j = 0; j < h->n_items; ++j) {
-    const nmc_work_item& it = h->items[j];
-    med[j] = make_medium(it.mua, it.mus, it.g, it.n1, it.n2, key1_of(h, it.stream_id));
-    h->prefix[j + 1] = h->prefix[j] + it.n_photons;
-  }
-  HIP_TRY(hipMemcpy(h->dItems, med.data(), med.size() * sizeof(SubMedium), hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(h->dPrefix, h->prefix.data(), h->prefix.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+  order_items(h);
+  NMC_TRY(upload_media(h));
   std::vector<float> cos_tab(64, 0.0f);
   for (int j = 0; j < h->ext.na; ++j)
     cos_tab[j] = (float)std::cos((double)(j + 1) * (3.14159265358979323846 / 2.0) / (double)h->ext.na);
@@ -498,14 +537,20 @@ int alloc_sub_outputs(nmc_handle* h) {
   return ok();
 }
 
-// Re-derives the per-item media (their Philox key depends on the seed).
+// Uploads the media in queue order with their budget prefix sums (the Philox
+// key of a medium depends on the seed).
 int upload_media(nmc_handle* h) {
-  std::vector<SubMedium> med(h->n_items);
-  for (int j = 0; j < h->n_items; ++j) {
-    const nmc_work_item& it = h->items[j];
-    med[j] = make_medium(it.mua, it.mus, it.g, it.n1, it.n2, key1_of(h, it.stream_id));
+  int n = h->n_items;
+  std::vector<SubMedium> med(n);
+  h->prefix.assign(n + 1, 0);
+  for (int q = 0; q < n; ++q) {
+    const nmc_work_item& it = h->items[h->order[q]];
+    med[q] = make_medium(it.mua, it.mus, it.g, it.n1, it.n2, key1_of(h, it.stream_id));
+    med[q].slot = h->order[q];
+    h->prefix[q + 1] = h->prefix[q] + it.n_photons;
   }
   HIP_TRY(hipMemcpy(h->dItems, med.data(), med.size() * sizeof(SubMedium), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(h->dPrefix, h->prefix.data(), h->prefix.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
   return ok();
 }
 
@@ -674,7 +719,9 @@ int nmc_prepare(nmc_handle* h, const RunParams* rp, const TissueParams* tp, cons
     std::memset(&tis[t], 0, sizeof(XyzTissue));
     tis[t].mua = tp->muav[t];
     tis[t].mus = tp->musv[t];
+    tis[t].inv_mus = inv_pos(tp->musv[t]);
     tis[t].g = tp->gv[t];
+    tis[t].hg = hg_prepare(tp->gv[t]);
     if (h->sim == NMC_SIM_MCXYZ_TT) tt_prepare(tp->gf[t], tp->af[t], tp->gb[t], tp->ab[t], tp->CC[t], &tis[t].tt);
   }
   HIP_TRY(hipMalloc(&h->dTis, NMC_NTISS * sizeof(XyzTissue)));
@@ -932,8 +979,9 @@ static void build_tables(nmc_handle* h, std::vector<LayStackDev>& hdr, std::vect
       }
       R.mua = L.mua;
       R.mus = L.mus;
+      R.inv_mus = inv_pos(L.mus);
       R.n = L.n;
-      R.g = L.g;
+      R.hg = hg_prepare(L.g);
       R.phase = L.phase;
       if (L.phase) tt_prepare(L.gf, L.af, L.gb, L.ab, L.cc, &R.tt);
       rows.push_back(R);

@@ -178,15 +178,23 @@ __device__ __forceinline__ void snell(float n1, float n2, float* ux, float* uy, 
 // ---------------------------------------------------------------------------
 // Scattering (reference kernels.metal:695-730, 1053-1070).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ float hg_cos(float g, float u) {
+// Henyey-Greenstein sampling (kernels.metal:695-703) with per-tissue
+// constants derived on the host in double: a = 1+g^2, b = 1-g^2, c = 1-g,
+// d = 2g, e = 1/(2g); mode 1 = isotropic (g == 0), 2 = forward (g == 1).
+struct HGCoef {
+  float a, b, c, d, e;
+  int mode;
+};
+
+__device__ __forceinline__ float hg_cos(const HGCoef& h, float u) {
   float ct;
-  if (flt_eq(g, 0.0f)) {
+  if (h.mode == 1) {
     ct = 2.0f * u - 1.0f;
-  } else if (flt_eq(g, 1.0f)) {
+  } else if (h.mode == 2) {
     ct = 1.0f;
   } else {
-    float t = (1.0f - g * g) / (1.0f - g + 2.0f * g * u);
-    ct = (1.0f + g * g - t * t) / (2.0f * g);
+    float t = h.b / (h.c + h.d * u);
+    ct = (h.a - t * t) * h.e;
   }
   return fminf(fmaxf(ct, -1.0f), 1.0f);
 }

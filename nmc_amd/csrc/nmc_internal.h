@@ -24,7 +24,8 @@ static_assert(sizeof(nmc_totals) == TF_COUNT * 8 + TI_COUNT * 8, "nmc_totals lay
 
 // Per-tissue row of the mcxyz material table (held in LDS by the kernel).
 struct XyzTissue {
-  float mua, mus, g, pad;
+  float mua, mus, inv_mus, g;       // inv_mus = 1/mus (FLT_MAX for mus = 0)
+  HGCoef hg;
   TTCoef tt;
 };
 
@@ -61,9 +62,10 @@ struct XyzArgs {
 
 // One medium of a (batched) mcsub launch.
 struct SubMedium {
-  float mut, one_m_albedo, g, n1, n2, rsp0;
+  float mut, inv_mut, one_m_albedo, n1, n2, rsp0;
   uint32_t key1;
-  int pad;
+  int slot;                         // output row (queue order may differ)
+  HGCoef hg;
 };
 
 struct SubArgs {
@@ -73,21 +75,22 @@ struct SubArgs {
   uint32_t max_events;
   uint32_t key0;
   uint32_t chunk;
+  uint32_t refill_min;              // refill once this many lanes of a wave are idle
   int n_items;
-  const SubMedium* items;           // [n_items]
+  const SubMedium* items;           // [n_items] in queue order
   const uint64_t* prefix;           // [n_items+1] queue index where item j starts
   const float* cos_tab;             // na-1 upper bin-edge cosines
   int tally_rz;                     // 0: skip the A(r,z) tally
   double* tally;                    // [n_items][(nr+1)*na + (nr+1)*(nz+1)] (global mode)
-  nmc_totals* tot;                  // [n_items] (global mode)
-  double* part;                     // block partials (LDS mode)
+  double* part;                     // block partials: [LDS tallies] [totals]
   RunState* state;
 };
 
 // Layered media: one row per (stack, layer), one header per stack.
 struct LayRow {
-  float zt, zb, mua, mus, n, g;
+  float zt, zb, mua, mus, inv_mus, n;
   int phase, pad;
+  HGCoef hg;
   TTCoef tt;
 };
 struct LayStackDev {
@@ -134,18 +137,17 @@ int layered_blocks_per_cu(size_t lds_bytes);
 hipError_t launch_layered(const LayArgs& a, int grid, size_t lds_bytes, hipStream_t s);
 hipError_t launch_layered_backward(const LayBackArgs& b, hipStream_t s);
 
-constexpr int kSubBlock = 1024;
 
 hipError_t launch_mcxyz(const XyzArgs& a, bool two_term, int block, int grid, size_t lds_bytes,
                         hipStream_t s);
 hipError_t mcxyz_configure(bool two_term, int block, size_t lds_bytes);
 int mcxyz_blocks_per_cu(bool two_term, int block, size_t lds_bytes);
 hipError_t launch_tile_reduce(const double* part, int grid, const XyzArgs& a, hipStream_t s);
-hipError_t launch_mcsub(const SubArgs& a, int grid, size_t lds_bytes, bool lds_tallies,
+hipError_t launch_mcsub(const SubArgs& a, int block, int grid, size_t lds_bytes, bool lds_tallies,
                         hipStream_t s);
-hipError_t mcsub_configure(size_t lds_bytes, bool lds_tallies);
-int mcsub_blocks_per_cu(size_t lds_bytes, bool lds_tallies);
-size_t mcsub_lds_bytes(int n_items, int64_t per_item);
+hipError_t mcsub_configure(int block, size_t lds_bytes, bool lds_tallies);
+int mcsub_blocks_per_cu(int block, size_t lds_bytes, bool lds_tallies);
+size_t mcsub_lds_bytes(int n_items, int64_t per_item_in_lds);
 hipError_t launch_reduce_partials(const double* part, int grid, int64_t stride, double* out,
                                   int ni, int per_item, hipStream_t s);
 hipError_t launch_state_begin(RunState* st, hipStream_t s);

@@ -231,17 +231,43 @@ static void snell(float n1, float n2, float* ux, float* uy, float* uz)
 /* ------------------------------------------------------------------------ */
 /* Scattering (kernels.metal:695-730, 1053-1070).                            */
 /* ------------------------------------------------------------------------ */
-float nmco_hg_cos(float g, float u)
+/* HG sampling (kernels.metal:695-703) from constants derived in double:
+ * a = 1+g^2, b = 1-g^2, c = 1-g, d = 2g, e = 1/(2g); cos = (a - t^2) e with
+ * t = b / (c + d u); mode 1: g == 0 (isotropic), 2: g == 1 (forward). */
+typedef struct { float a, b, c, d, e; int mode; } hg_coef;
+
+static void hg_prepare(float g, hg_coef* h)
+{
+    h->mode = flt_eq(g, 0.0f) ? 1 : (flt_eq(g, 1.0f) ? 2 : 0);
+    double gd = g;
+    h->a = (float)(1.0 + gd * gd);
+    h->b = (float)(1.0 - gd * gd);
+    h->c = (float)(1.0 - gd);
+    h->d = (float)(2.0 * gd);
+    h->e = h->mode == 0 ? (float)(1.0 / (2.0 * gd)) : 0.0f;
+}
+
+static inline float hg_sample(const hg_coef* h, float u)
 {
     float ct;
-    if (flt_eq(g, 0.0f)) ct = 2.0f * u - 1.0f;
-    else if (flt_eq(g, 1.0f)) ct = 1.0f;
+    if (h->mode == 1) ct = 2.0f * u - 1.0f;
+    else if (h->mode == 2) ct = 1.0f;
     else {
-        float t = (1.0f - g * g) / (1.0f - g + 2.0f * g * u);
-        ct = (1.0f + g * g - t * t) / (2.0f * g);
+        float t = h->b / (h->c + h->d * u);
+        ct = (h->a - t * t) * h->e;
     }
     return fminf(fmaxf(ct, -1.0f), 1.0f);
 }
+
+float nmco_hg_cos(float g, float u)
+{
+    hg_coef h;
+    hg_prepare(g, &h);
+    return hg_sample(&h, u);
+}
+
+/* 1/mu rounded once from double; FLT_MAX for mu = 0. */
+static inline float inv_pos(float mu) { return mu > 0.0f ? (float)(1.0 / (double)mu) : 3.40282347e+38f; }
 
 /* Per-tissue constants of the two-term function, derived in double on the
  * host (q1, q2, q3 of kernels.metal:900-905). */
@@ -403,7 +429,8 @@ typedef struct {
     uint32_t max_events;
     uint32_t key[2];
     const uint8_t* V;
-    float mua[NMC_NTISS], mus[NMC_NTISS], g[NMC_NTISS];
+    float mua[NMC_NTISS], mus[NMC_NTISS], inv_mus[NMC_NTISS];
+    hg_coef hg[NMC_NTISS];
     tt_coef tt[NMC_NTISS];
 } xyz_ctx;
 
@@ -497,7 +524,7 @@ static void xyz_photon(const xyz_ctx* c, uint64_t pid, xyz_acc* a)
     for (;;) {
         /* HOP: one piece of the step (kernels.metal:505-685) */
         float mua = c->mua[tis], mus = c->mus[tis];
-        float s_int = sleft / mus;
+        float s_int = sleft * c->inv_mus[tis];
         float tx = INF_F, ty = INF_F, tz = INF_F;
         if (ux > 0.0f) tx = (xface(ix + 1, c->hx, c->dx) - x) * rux;
         else if (ux < 0.0f) tx = (xface(ix, c->hx, c->dx) - x) * rux;
@@ -585,6 +612,7 @@ static void xyz_photon(const xyz_ctx* c, uint64_t pid, xyz_acc* a)
                 }
             }
         }
+        if (alive && !(W > 0.0f)) alive = 0; /* all weight absorbed */
         if (!alive) { FLUSH(); return; }
         if (events >= c->max_events) {
             FLUSH();
@@ -595,7 +623,7 @@ static void xyz_photon(const xyz_ctx* c, uint64_t pid, xyz_acc* a)
         if (sleft > 0.0f) continue;
 
         /* SPIN (kernels.metal:695-730) */
-        float ct = c->two_term ? tt_cos(&c->tt[tis], u01(b[1])) : nmco_hg_cos(c->g[tis], u01(b[1]));
+        float ct = c->two_term ? tt_cos(&c->tt[tis], u01(b[1])) : hg_sample(&c->hg[tis], u01(b[1]));
         nmco_spin(ct, u01(b[2]), &ux, &uy, &uz);
         rux = 1.0f / ux; ruy = 1.0f / uy; ruz = 1.0f / uz;
         t->interactions++;
@@ -634,7 +662,8 @@ static int xyz_setup(const RunParams* rp, const TissueParams* tp, const uint8_t*
     c->key[1] = (uint32_t)(e->seed >> 32) + e->stream_id;
     c->V = v;
     for (int i = 0; i < NMC_NTISS; ++i) {
-        c->mua[i] = tp->muav[i]; c->mus[i] = tp->musv[i]; c->g[i] = tp->gv[i];
+        c->mua[i] = tp->muav[i]; c->mus[i] = tp->musv[i]; c->inv_mus[i] = inv_pos(tp->musv[i]);
+        hg_prepare(tp->gv[i], &c->hg[i]);
         if (c->two_term) tt_prepare(tp->gf[i], tp->af[i], tp->gb[i], tp->ab[i], tp->CC[i], &c->tt[i]);
     }
     int64_t nv = (int64_t)rp->Nx * rp->Ny * rp->Nz;
@@ -693,7 +722,8 @@ int nmco_mcxyz(const RunParams* rp, const TissueParams* tp, const uint8_t* v,
 /* mcsub (kernels.metal:59-319).                                             */
 /* ------------------------------------------------------------------------ */
 typedef struct {
-    float mua, mus, g, n1, n2, mut, one_m_albedo, rsp0;
+    float mua, mus, g, n1, n2, mut, inv_mut, one_m_albedo, rsp0;
+    hg_coef hg;
     int mcflag, bmode, nr, nz, na;
     float dr, dz, radius, waist, zfocus, xs, ys, zs;
     float thr, chance;
@@ -747,7 +777,7 @@ static void sub_photon(const sub_ctx* c, uint64_t pid, sub_acc* a)
     for (;;) {
         block(c->key, pid, ev, 0u, b);
         /* HOP (kernels.metal:183-187) */
-        float s = -nmco_logf(u01nz(b[0])) / c->mut;
+        float s = -nmco_logf(u01nz(b[0])) * c->inv_mut;
         float x0 = x, y0 = y, z0 = z;
         x = fmaf(s, ux, x);
         y = fmaf(s, uy, y);
@@ -787,7 +817,7 @@ static void sub_photon(const sub_ctx* c, uint64_t pid, sub_acc* a)
         if (a->a_rz) a->a_rz[(int64_t)ir * (c->nz + 1) + iz] += (double)absorb;
         t->absorbed += (double)absorb;
         /* SPIN (kernels.metal:243-279, with g honoured) */
-        float ct = nmco_hg_cos(c->g, u01(b[1]));
+        float ct = hg_sample(&c->hg, u01(b[1]));
         nmco_spin(ct, u01(b[2]), &ux, &uy, &uz);
         t->interactions++;
         if (!roulette(&W, u01(b[3]), c->thr, c->chance, t)) return;
@@ -813,6 +843,8 @@ int nmco_mcsub(const RunParams* rp, const TissueParams* tp, const nmc_ext_config
     c.n1 = rp->n1; c.n2 = rp->n2;
     if (!(c.mua >= 0) || !(c.mus >= 0) || !(c.mua + c.mus > 0)) return NMC_ERR_ARG;
     c.mut = c.mua + c.mus;
+    c.inv_mut = inv_pos(c.mut);
+    hg_prepare(c.g, &c.hg);
     c.one_m_albedo = 1.0f - c.mus / c.mut;
     float tr = c.n1 / c.n2;
     tr = (1.0f - tr) / (1.0f + tr);
@@ -868,7 +900,8 @@ typedef struct {
     int nl;
     float n_above, n_below;
     float zt[NMC_MAX_LAYERS], zb[NMC_MAX_LAYERS];
-    float mua[NMC_MAX_LAYERS], mus[NMC_MAX_LAYERS], n[NMC_MAX_LAYERS], g[NMC_MAX_LAYERS];
+    float mua[NMC_MAX_LAYERS], mus[NMC_MAX_LAYERS], inv_mus[NMC_MAX_LAYERS], n[NMC_MAX_LAYERS];
+    hg_coef hg[NMC_MAX_LAYERS];
     int phase[NMC_MAX_LAYERS];
     tt_coef tt[NMC_MAX_LAYERS];
     float rsp;
@@ -919,7 +952,7 @@ static void lay_photon(const lay_stack* S, const lay_cfg* c, uint64_t pid, lay_a
     float sleft = -nmco_logf(u01nz(b[0]));
     for (;;) {
         float mua = S->mua[l], mus = S->mus[l];
-        float s_int = sleft / mus;
+        float s_int = sleft * S->inv_mus[l];
         float s_b = INF_F;
         if (uz > 0.0f) s_b = fmaxf((S->zb[l] - z) * ruz, 0.0f);
         else if (uz < 0.0f) s_b = fmaxf((S->zt[l] - z) * ruz, 0.0f);
@@ -989,13 +1022,14 @@ static void lay_photon(const lay_stack* S, const lay_cfg* c, uint64_t pid, lay_a
                 l += up ? -1 : 1;
             }
         }
+        if (!(W > 0.0f)) return; /* all weight absorbed */
         if (events >= c->max_events) {
             t->capped += (double)W;
             t->capped_photons++;
             return;
         }
         if (sleft > 0.0f) continue;
-        float ct = S->phase[l] ? tt_cos(&S->tt[l], u01(b[1])) : nmco_hg_cos(S->g[l], u01(b[1]));
+        float ct = S->phase[l] ? tt_cos(&S->tt[l], u01(b[1])) : hg_sample(&S->hg[l], u01(b[1]));
         nmco_spin(ct, u01(b[2]), &ux, &uy, &uz);
         ruz = 1.0f / uz;
         K[l]++;
@@ -1021,7 +1055,8 @@ void nmco_layered_prepare_stack(const nmc_stack* s, uint64_t seed, lay_stack* o)
         int last = l == s->n_layers - 1;
         if (last && !(L->thickness > 0.0f && L->thickness < INF_F)) o->zb[l] = INF_F;
         else { z += (double)L->thickness; o->zb[l] = (float)z; }
-        o->mua[l] = L->mua; o->mus[l] = L->mus; o->n[l] = L->n; o->g[l] = L->g;
+        o->mua[l] = L->mua; o->mus[l] = L->mus; o->inv_mus[l] = inv_pos(L->mus); o->n[l] = L->n;
+        hg_prepare(L->g, &o->hg[l]);
         o->phase[l] = L->phase;
         if (L->phase) tt_prepare(L->gf, L->af, L->gb, L->ab, L->cc, &o->tt[l]);
     }
