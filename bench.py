@@ -127,6 +127,8 @@ def main() -> int:
                     help="extra measurements: sweep (batched vs plain), grad (pMC cost)")
     ap.add_argument("--sweep-scale", type=float, default=1.0, help="scale of the sweep budgets")
     ap.add_argument("--grad-photons", type=float, default=1e7)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="use the library's default launch tuning instead of the autotuner")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the photon-range "
@@ -140,8 +142,16 @@ def main() -> int:
 
     n = int(args.photons)
     w = W.skin_vessel(n_photons=n)
+    tuning = None
+    if not args.no_autotune:
+        from nmc_amd.autotune import autotune_mcxyz
+        t0 = time.perf_counter()
+        tuning = autotune_mcxyz(w)
+        log(f"[bench] autotune {tuning.as_dict()} in {time.perf_counter() - t0:.1f} s")
     mc = MetalMC(w.sim, 0)
     mc.prepare_data(w.rp, w.tp, w.v, w.ext)
+    if tuning is not None:
+        mc.set_tuning(tuning)
     mc.set_num_runs_seed(1, 0)
     log(f"[bench] warmup {args.warmup} x {n} photons")
     for _ in range(args.warmup):

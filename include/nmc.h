@@ -82,6 +82,9 @@ typedef struct TissueParams {
 
 /* flags in nmc_ext_config.flags */
 #define NMC_FLAG_NO_FLUENCE 1u   /* skip the 3-D / (r,z) absorption tally   */
+#define NMC_FLAG_COUNT_VISITS 2u /* mcxyz: the fluence array counts voxel
+                                    visits instead of deposited weight (used
+                                    by the autotuner to place the LDS tile)  */
 
 /*
  * Run-time settings that NMC fixed at compile time (definitions.h:16-82) plus

@@ -48,13 +48,14 @@ __device__ __forceinline__ void lds_addf(nmc_totals* t, int k, double v) {
 
 // Adds the deposits of the current voxel visit to the tile or the global grid.
 __device__ __forceinline__ void flush_visit(Photon& p, const XyzArgs& a, double* tile) {
-  if (p.inside && a.F && p.vdep != 0.0) {
+  if (p.inside && a.F && (a.count_visits || p.vdep != 0.0)) {
+    double v = a.count_visits ? 1.0 : p.vdep;
     unsigned lx = (unsigned)(p.ix - a.tile_x0), ly = (unsigned)(p.iy - a.tile_y0),
              lz = (unsigned)(p.iz - a.tile_z0);
     if (lx < (unsigned)a.tile_nx && ly < (unsigned)a.tile_ny && lz < (unsigned)a.tile_nz)
-      unsafeAtomicAdd(&tile[((int)lz * a.tile_nx + (int)lx) * a.tile_ny + (int)ly], p.vdep);
+      unsafeAtomicAdd(&tile[((int)lz * a.tile_nx + (int)lx) * a.tile_ny + (int)ly], v);
     else
-      unsafeAtomicAdd(&a.F[vidx(a, p.ix, p.iy, p.iz)], p.vdep);
+      unsafeAtomicAdd(&a.F[vidx(a, p.ix, p.iy, p.iz)], v);
   }
   p.vdep = 0.0;
 }

@@ -443,6 +443,7 @@ int enqueue_run(nmc_handle* h, uint64_t n) {
     a.n = n;
     a.chunk = pick_chunk(n, h->xyz_grid * (h->tune.block / 64), h->tune.chunk);
     a.refill_min = (uint32_t)h->tune.refill_min;
+    a.count_visits = (e.flags & NMC_FLAG_COUNT_VISITS) != 0;
     a.tile_x0 = h->tune.tile[0];
     a.tile_y0 = h->tune.tile[1];
     a.tile_z0 = h->tune.tile[2];

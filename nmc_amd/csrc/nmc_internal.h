@@ -48,6 +48,7 @@ struct XyzArgs {
   uint64_t n;                       // photons in this launch
   uint32_t chunk;                   // queue indices taken per wave refill
   uint32_t refill_min;              // refill once this many lanes of a wave are idle
+  int count_visits;                 // 1: F counts voxel visits (tile placement pilot)
   int tile_x0, tile_y0, tile_z0;    // LDS fluence tile origin (voxel indices)
   int tile_nx, tile_ny, tile_nz;    // tile extent; 0 = no tile
   const uint8_t* V;

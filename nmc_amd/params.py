@@ -24,6 +24,7 @@ DET_REFLECTANCE = 0   # definitions.h:167-171
 DET_TRANSMITTANCE = 1
 
 FLAG_NO_FLUENCE = 1
+FLAG_COUNT_VISITS = 2
 
 OK = 0
 ERR_ARG = -1

@@ -422,7 +422,7 @@ static inline int roulette(float* W, float u, float thr, float chance, nmc_total
 typedef struct {
     int Nx, Ny, Nz;
     float dx, dy, dz, hx, hy;
-    int bmode, det_state, mcflag, launchflag, two_term;
+    int bmode, det_state, mcflag, launchflag, two_term, count_visits;
     float zsurf, n1, n2, rmin, rmax, cos_na;
     float xs, ys, zs, ux0, uy0, uz0, radius, waist, zfocus;
     float thr, chance;
@@ -515,10 +515,11 @@ static void xyz_photon(const xyz_ctx* c, uint64_t pid, xyz_acc* a)
     /* Deposits of one voxel visit are summed in double and added to F when
      * the photon leaves the voxel or ends. */
     double vdep = 0.0;
-#define FLUSH()                                                        \
-    do {                                                               \
-        if (inside && a->F && vdep != 0.0) a->F[VIDX(ix, iy, iz)] += vdep; \
-        vdep = 0.0;                                                    \
+#define FLUSH()                                                                        \
+    do {                                                                               \
+        if (inside && a->F && (c->count_visits || vdep != 0.0))                        \
+            a->F[VIDX(ix, iy, iz)] += c->count_visits ? 1.0 : vdep;                     \
+        vdep = 0.0;                                                                    \
     } while (0)
 
     for (;;) {
@@ -658,6 +659,7 @@ static int xyz_setup(const RunParams* rp, const TissueParams* tp, const uint8_t*
     c->ux0 = rp->ux0; c->uy0 = rp->uy0; c->uz0 = rp->uz0;
     c->radius = rp->radius; c->waist = rp->waist; c->zfocus = rp->zfocus;
     c->thr = e->threshold; c->chance = e->chance; c->max_events = e->max_events;
+    c->count_visits = (e->flags & NMC_FLAG_COUNT_VISITS) != 0;
     c->key[0] = (uint32_t)e->seed;
     c->key[1] = (uint32_t)(e->seed >> 32) + e->stream_id;
     c->V = v;

@@ -75,6 +75,30 @@ def test_mcsub_launch_modes(gpu, mcflag):
     assert_same_tallies(got, ref, f"mcsub{mcflag}")
 
 
+def test_visit_counts_bit_exact(gpu):
+    """The autotuner's pilot mode: the fluence array counts voxel visits."""
+    w = W.skin_vessel()
+    w.ext.flags = P.FLAG_COUNT_VISITS
+    _say("[parity] visit counts n=5000")
+    ref = O.run(w, 5000, threads=8)
+    got = simulate(w, 5000)
+    assert np.array_equal(got.fluence, ref.fluence)
+    # one count per face crossing (the voxel left) and one per photon that ends
+    # inside the grid; detector-plane arrivals do not change voxel
+    t = ref.totals
+    ends_inside = t["photons"] - t["escapes_bottom"] - t["escapes_side"] - t["escapes_top"]
+    assert got.fluence.sum() == t["substeps_cross"] - t["surface_hits"] + ends_inside
+
+
+def test_autotuner_runs(gpu):
+    from nmc_amd.autotune import autotune_mcxyz
+
+    w = W.skin_vessel()
+    _say("[autotune] skin-vessel")
+    t = autotune_mcxyz(w, n_trial=2_000_000, use_cache=False)
+    assert t.block in (512, 1024) and t.tile[3] * t.tile[4] * t.tile[5] > 0
+
+
 def test_batches_compose(gpu):
     """Two runs over consecutive id ranges accumulate to the single-run tallies."""
     w = W.skin_vessel()
