@@ -7,10 +7,19 @@ resident in HBM.  The kernel time of every step is taken from HIP events
 recorded by the library on the stream the kernel runs on.  Prints one JSON
 line (see the keys below); the CPU baseline is the oracle run on this host's
 cores over a bounded sample of the same workload.
+
+Every timed step (and the warm-up) launches mcxyz_kernel with the default
+1024-lane blocks; the other launches of this run (visit-count pilot, Rd
+parity run) use 512-lane blocks, so a kernel-trace profile of this command
+reports the timed kernel under its own name.
+
+    python bench.py                      # the measurement
+    python bench.py --calibrate-copy 256 # a 256 MiB copy, to scale PMC byte counters
 """
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import json
 import os
 import sys
@@ -26,18 +35,61 @@ METRIC = "photons/s on 1 MI355X (mcxyz 200^3) at <=0.5% Rd err; grad cost vs for
 OPS_SAME, OPS_CROSS, OPS_INT = 44, 97, 145
 # gfx950: 256 CUs x 64 FP32 lanes x 2.4 GHz, non-packed lane-ops/s (T).
 VALU_PEAK_T = 256 * 64 * 2.4e9 / 1e12
+HBM_PEAK_GBS = 8000.0
+# HBM bytes per photon of the timed kernel on this workload, from the
+# rocprofv3 FETCH_SIZE and WRITE_SIZE passes over one 1e8-photon launch,
+# scaled by the copy calibration (FETCH x2.0, WRITE x1.0):
+# (18 958 541 KiB x 2 + 99 746 748 KiB) x 1024 B / 1e8 photons
+# (profiles/mcxyz_skinvessel_200_pmc_traffic.md).
+TRAFFIC_BYTES_PER_PHOTON = 1409.7
 
 
 def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
+def probe_lib():
+    from nmc_amd.build import PROBE_LIB
+    if not os.path.exists(PROBE_LIB):
+        raise RuntimeError(f"{PROBE_LIB} is not built (run __graft_entry__.build())")
+    L = C.CDLL(PROBE_LIB)
+    L.nmcp_atomic_rate.argtypes = [C.c_uint32, C.c_int, C.POINTER(C.c_float),
+                                   C.POINTER(C.c_double)]
+    L.nmcp_copy.argtypes = [C.c_int64, C.c_int, C.POINTER(C.c_float)]
+    return L
+
+
+def atomic_ceiling(slots: int) -> dict:
+    """FP64 atomic adds/s to scattered addresses over a fluence-sized array."""
+    L = probe_lib()
+    ms, n = C.c_float(), C.c_double()
+    rc = L.nmcp_atomic_rate(slots, 64, C.byref(ms), C.byref(n))
+    if rc != 0:
+        raise RuntimeError(f"nmcp_atomic_rate failed ({rc})")
+    return {"per_s": n.value / (ms.value * 1e-3), "slots": slots, "atomics": n.value,
+            "ms": ms.value}
+
+
+def calibrate_copy(mib: int) -> int:
+    L = probe_lib()
+    ms = C.c_float()
+    nbytes = mib << 20
+    rc = L.nmcp_copy(nbytes, 3, C.byref(ms))
+    if rc != 0:
+        raise RuntimeError(f"nmcp_copy failed ({rc})")
+    print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "reps": 3,
+                      "ms": ms.value, "GB_per_s": 2 * nbytes / (ms.value * 1e-3) / 1e9}),
+          flush=True)
+    return 0
+
+
 def cpu_baseline(w, target_s: float) -> dict:
     from oracle import oracle as O
     cores = len(os.sched_getaffinity(0))
-    probe = O.run(w, 2000, threads=cores, fluence=True)
-    rate = 2000 / max(probe.seconds, 1e-6)
-    n = int(min(5e6, max(1e4, rate * target_s)))
+    n_probe = 50_000
+    probe = O.run(w, n_probe, threads=cores, fluence=True)
+    rate = n_probe / max(probe.seconds, 1e-6)
+    n = int(min(5e7, max(1e5, rate * target_s)))
     log(f"[bench] cpu baseline: oracle on {cores} threads, {n} photons")
     t = O.run(w, n, threads=cores, fluence=True)
     return {"value": n / t.seconds, "unit": "photons/s", "cores": cores, "kind": "port",
@@ -127,31 +179,43 @@ def main() -> int:
                     help="extra measurements: sweep (batched vs plain), grad (pMC cost)")
     ap.add_argument("--sweep-scale", type=float, default=1.0, help="scale of the sweep budgets")
     ap.add_argument("--grad-photons", type=float, default=1e7)
-    ap.add_argument("--no-autotune", action="store_true",
-                    help="use the library's default launch tuning instead of the autotuner")
+    ap.add_argument("--autotune", action="store_true",
+                    help="run the launch autotuner first (its trial launches share the "
+                         "timed kernel's name in a profile)")
+    ap.add_argument("--calibrate-copy", type=int, metavar="MIB",
+                    help="only run a plain copy of MIB MiB (scales PMC byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the photon-range "
               "sharding over several GPUs is described in DESIGN.md but not run here.",
               file=sys.stderr)
         return 2
+    if args.calibrate_copy:
+        return calibrate_copy(args.calibrate_copy)
 
     from nmc_amd import params as P
     from nmc_amd import workloads as W
+    from nmc_amd.autotune import box_coverage, visit_histogram
     from nmc_amd.metal_mc import MetalMC
 
     n = int(args.photons)
     w = W.skin_vessel(n_photons=n)
     tuning = None
-    if not args.no_autotune:
+    if args.autotune:
         from nmc_amd.autotune import autotune_mcxyz
         t0 = time.perf_counter()
         tuning = autotune_mcxyz(w)
         log(f"[bench] autotune {tuning.as_dict()} in {time.perf_counter() - t0:.1f} s")
+    log("[bench] atomic ceiling probe")
+    ceiling = atomic_ceiling(w.rp.Nx * w.rp.Ny * w.rp.Nz)
+    log("[bench] visit-count pilot (512-lane blocks)")
+    hist = visit_histogram(w)
     mc = MetalMC(w.sim, 0)
     mc.prepare_data(w.rp, w.tp, w.v, w.ext)
     if tuning is not None:
         mc.set_tuning(tuning)
+    eff = mc.get_tuning()
+    coverage = box_coverage(hist, eff.tile)
     mc.set_num_runs_seed(1, 0)
     log(f"[bench] warmup {args.warmup} x {n} photons")
     for _ in range(args.warmup):
@@ -170,16 +234,39 @@ def main() -> int:
     wall = t1 - t0
     res = mc.get_compute_results(fluence=False)
     tot = res.totals
-    n_all = n * args.steps
-    value = n_all / wall
+    value = n * args.steps / wall
+    ksec = sum(kern_ms) * 1e-3
     avg_ms = sum(kern_ms) / len(kern_ms)
 
     ops = OPS_SAME * tot["substeps_same"] + OPS_CROSS * tot["substeps_cross"] \
         + OPS_INT * tot["interactions"]
-    achieved = ops / (sum(kern_ms) * 1e-3) / 1e12
-    visits = tot["substeps_cross"] + tot["photons"]   # voxel-visit flushes (LDS tile or global)
+    achieved = ops / ksec / 1e12
+    # voxel-visit flushes: one per voxel left plus one per photon ending in the grid
+    visits = tot["substeps_cross"] - tot["surface_hits"] + tot["photons"] \
+        - tot["escapes_bottom"] - tot["escapes_side"] - tot["escapes_top"]
+    global_atomics = visits * (1.0 - coverage) + tot["detections"]
     rd = tot["rd"] / tot["photons"]
+    per_step = tot["photons"] / args.steps
 
+    roof = {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_T,
+            "unit": "Tlane-op/s", "frac": achieved / VALU_PEAK_T, "traffic": None,
+            "kernel": "mcxyz_kernel<false, 1024>", "kernel_ms": avg_ms,
+            "ops_per_photon": ops / tot["photons"],
+            "events_per_photon": {"same": tot["substeps_same"] / tot["photons"],
+                                  "cross": tot["substeps_cross"] / tot["photons"],
+                                  "interactions": tot["interactions"] / tot["photons"]},
+            "atomic": {"bound": "fp64 atomic adds, scattered", "ceiling_per_s": ceiling["per_s"],
+                       "visits_per_photon": visits / tot["photons"],
+                       "lds_tile_fraction": coverage,
+                       "global_atomics_per_s": global_atomics / ksec,
+                       "frac": global_atomics / ksec / ceiling["per_s"],
+                       "photons_per_s_at_ceiling": ceiling["per_s"] * tot["photons"]
+                       / global_atomics},
+            "tuning": eff.as_dict()}
+    if TRAFFIC_BYTES_PER_PHOTON is not None:
+        roof["traffic"] = TRAFFIC_BYTES_PER_PHOTON * per_step
+        roof["hbm_GB_per_s"] = roof["traffic"] / (avg_ms * 1e-3) / 1e9
+        roof["hbm_frac"] = roof["hbm_GB_per_s"] / HBM_PEAK_GBS
     out = {
         "metric": METRIC, "value": value, "unit": "photons/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps,
@@ -187,16 +274,8 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": "mcxyz skin-vessel 200^3 uint8 (0.005 cm voxels; air/epidermis/"
                    "dermis/blood cylinder), flat collimated beam r=0.06 cm, n=1, 1e8 photons/step",
-                   "photons_per_step": n, "kernel": "mcxyz_kernel<false>"},
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_T,
-                     "unit": "Tlane-op/s", "frac": achieved / VALU_PEAK_T, "traffic": None,
-                     "kernel_ms": avg_ms,
-                     "ops_per_photon": ops / tot["photons"],
-                     "events_per_photon": {"same": tot["substeps_same"] / tot["photons"],
-                                           "cross": tot["substeps_cross"] / tot["photons"],
-                                           "interactions": tot["interactions"] / tot["photons"]},
-                     "voxel_flushes_per_s": visits / (sum(kern_ms) * 1e-3),
-                     "tuning": mc.get_tuning().as_dict()},
+                   "photons_per_step": n, "autotuned": tuning is not None},
+        "roofline": roof,
         "rd": rd,
         "energy_balance": (tot["rd"] + tot["rd_outside"] + tot["absorbed"] + tot["tt"]
                            + tot["lateral"] + tot["top"] + tot["roulette_loss"] + tot["capped"]
@@ -206,7 +285,9 @@ def main() -> int:
         cb = cpu_baseline(w, args.cpu_seconds)
         ref = cb.pop("_tallies")
         m = cb.pop("_n")
-        # the same photon ids on the GPU: Rd must match the oracle's
+        # the same photon ids on the GPU (512-lane blocks: a separate kernel
+        # name in a profile; the launch shape never changes results)
+        mc.set_tuning(P.Tuning.make(block=512, tile=tuple(eff.tile)))
         mc.reset_tallies()
         mc.set_num_runs_seed(1, 0)
         mc.send_compute_command(m)

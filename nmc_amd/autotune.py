@@ -60,13 +60,14 @@ def _store_cache(key: str, value: dict) -> None:
     os.replace(tmp, CACHE)
 
 
-def visit_histogram(w, n_pilot: int = 200_000, device: int = 0) -> np.ndarray:
-    """Voxel-visit counts [Nz, Nx, Ny] from a GPU pilot run."""
+def visit_histogram(w, n_pilot: int = 200_000, device: int = 0, block: int = 512) -> np.ndarray:
+    """Voxel-visit counts [Nz, Nx, Ny] from a GPU pilot run (launched with its
+    own block size, so a profile tells its dispatches from the transport's)."""
     ext = P.ExtConfig.from_buffer_copy(w.ext)
     ext.flags = (ext.flags | P.FLAG_COUNT_VISITS) & ~P.FLAG_NO_FLUENCE
     with MetalMC(w.sim, device) as mc:
         mc.prepare_data(w.rp, w.tp, w.v, ext)
-        mc.set_tuning(P.Tuning.make(tile=(0, 0, 0, -1, 0, 0)))
+        mc.set_tuning(P.Tuning.make(block=block, tile=(0, 0, 0, -1, 0, 0)))
         mc.set_num_runs_seed(ext.seed, 1 << 41)
         mc.send_compute_command(n_pilot)
         t = mc.get_compute_results()
@@ -103,6 +104,17 @@ def best_boxes(hist: np.ndarray, budget: int, n_best: int = 3):
         if len(out) == n_best:
             break
     return out
+
+
+def box_coverage(hist: np.ndarray, tile) -> float:
+    """Fraction of the visits in `hist` [Nz, Nx, Ny] inside the tile box
+    (x0, y0, z0, nx, ny, nz)."""
+    x0, y0, z0, nx, ny, nz = (int(t) for t in tile)
+    if nx <= 0 or ny <= 0 or nz <= 0:
+        return 0.0
+    total = float(hist.sum())
+    inside = float(hist[z0:z0 + nz, x0:x0 + nx, y0:y0 + ny].sum())
+    return inside / total if total > 0 else 0.0
 
 
 def _time(mc: MetalMC, n: int, seed: int) -> float:

@@ -28,16 +28,26 @@ FLAGS = [
 ]
 
 
-def build(verbose: bool = False) -> str:
-    """Builds the library in-tree (always recompiles) and returns its path."""
-    cmd = [HIPCC, *FLAGS, "-o", LIB + ".tmp", *[os.path.join(CSRC, s) for s in SOURCES]]
+# measurement kernels used by bench.py only (atomic-rate ceiling, copy calibration)
+PROBE_LIB = os.path.join(HERE, "libnmc_probe.so")
+PROBE_SOURCES = ["probe.hip"]
+
+
+def _hipcc(out: str, sources: list[str], verbose: bool) -> None:
+    cmd = [HIPCC, *FLAGS, "-o", out + ".tmp", *[os.path.join(CSRC, s) for s in sources]]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed building libnmc_hip.so")
-    os.replace(LIB + ".tmp", LIB)
+        raise RuntimeError(f"hipcc failed building {os.path.basename(out)}")
+    os.replace(out + ".tmp", out)
+
+
+def build(verbose: bool = False) -> str:
+    """Builds both libraries in-tree (always recompiles); returns the main one."""
+    _hipcc(LIB, SOURCES, verbose)
+    _hipcc(PROBE_LIB, PROBE_SOURCES, verbose)
     return LIB
 
 
