@@ -25,6 +25,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -95,6 +97,30 @@ def cpu_baseline(w, target_s: float) -> dict:
     return {"value": n / t.seconds, "unit": "photons/s", "cores": cores, "kind": "port",
             "sample": f"photons [0,{n}) of the same workload (oracle, OpenMP, FP64 tallies)",
             "_tallies": t, "_n": n}
+
+
+def tile32_leg(mc, hist, n: int, steps: int = 2) -> dict:
+    """The same workload with the FP32 LDS tile (twice the voxels; sums within a
+    workgroup in FP32, across workgroups in FP64), box from the visit pilot."""
+    from nmc_amd import params as P
+    from nmc_amd.autotune import best_boxes, box_coverage
+
+    mc.set_tuning(P.Tuning.make(mode=1))
+    budget = int(np.prod(mc.get_tuning().tile[3:6]))
+    box = best_boxes(hist, budget, 1)[0][1]
+    mc.set_tuning(P.Tuning.make(mode=1, tile=box))
+    log(f"[bench] fp32 tile {box}: {steps} x {n} photons")
+    mc.reset_tallies()
+    mc.set_num_runs_seed(1, 0)
+    ms = []
+    for _ in range(steps):
+        mc.send_compute_command(n)
+        ms.append(mc.last_kernel_ms())
+    avg = sum(ms) / len(ms)
+    return {"kernel": "mcxyz_kernel<false, 1024, float>", "tile": list(box),
+            "lds_tile_fraction": box_coverage(hist, box), "kernel_ms": avg,
+            "photons_per_s": n / (avg * 1e-3), "fluence_parity": "integer tallies bit-exact; "
+            "fluence within 2e-5 relative of the oracle (tests/test_gpu_parity.py)"}
 
 
 def sweep_leg(budget_scale: float) -> dict:
@@ -175,8 +201,9 @@ def main() -> int:
     ap.add_argument("--photons", type=float, default=1e8, help="photons per step")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--legs", default="sweep,grad",
-                    help="extra measurements: sweep (batched vs plain), grad (pMC cost)")
+    ap.add_argument("--legs", default="tile32,sweep,grad",
+                    help="extra measurements: tile32 (FP32 LDS tile), sweep (batched vs "
+                         "plain), grad (pMC cost)")
     ap.add_argument("--sweep-scale", type=float, default=1.0, help="scale of the sweep budgets")
     ap.add_argument("--grad-photons", type=float, default=1e7)
     ap.add_argument("--autotune", action="store_true",
@@ -250,7 +277,7 @@ def main() -> int:
 
     roof = {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_T,
             "unit": "Tlane-op/s", "frac": achieved / VALU_PEAK_T, "traffic": None,
-            "kernel": "mcxyz_kernel<false, 1024>", "kernel_ms": avg_ms,
+            "kernel": "mcxyz_kernel<false, 1024, double>", "kernel_ms": avg_ms,
             "ops_per_photon": ops / tot["photons"],
             "events_per_photon": {"same": tot["substeps_same"] / tot["photons"],
                                   "cross": tot["substeps_cross"] / tot["photons"],
@@ -297,8 +324,10 @@ def main() -> int:
         out["rd_parity"] = {"photons": m, "gpu_rd": g["rd"] / m, "oracle_rd": r["rd"] / m,
                             "rel_diff": abs(g["rd"] - r["rd"]) / r["rd"],
                             "int_tallies_equal": all(g[k] == r[k] for k in P.TOTAL_INT_FIELDS)}
-    mc.close()
     legs = [s for s in args.legs.split(",") if s]
+    if "tile32" in legs:
+        out["fp32_tile"] = tile32_leg(mc, hist, n)
+    mc.close()
     if "sweep" in legs:
         out["batched_sweep"] = sweep_leg(args.sweep_scale)
     if "grad" in legs:

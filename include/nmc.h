@@ -207,7 +207,12 @@ typedef struct nmc_tuning {
     int32_t chunk;          /* photon ids a wave takes from the queue at once   */
     int32_t tile[6];        /* mcxyz LDS fluence tile x0,y0,z0,nx,ny,nz;
                                nx == 0: automatic box, nx < 0: no tile          */
-    int32_t reserved[6];
+    int32_t mode;           /* mcsub: tally placement, 0 automatic (LDS when the
+                               per-item tallies fit), 1 LDS, 2 global atomics;
+                               mcxyz: LDS tile element, 0 FP64, 1 FP32 (twice
+                               the voxels; sums within a workgroup in FP32,
+                               across workgroups in FP64)                      */
+    int32_t reserved[5];
 } nmc_tuning;
 
 int nmc_set_tuning(nmc_handle* h, const nmc_tuning* t);

@@ -5,7 +5,8 @@ For an mcxyz workload it
      of voxel visits (= fluence flushes) per voxel;
   2. searches, with a 3-D summed-area table, the boxes of several aspect ratios
      that fit the LDS tile budget and cover the most visits;
-  3. times short trial launches over (tile box x block size x refill batch)
+  3. times short trial launches over (tile element FP64 or FP32 x tile box x
+     block size x refill batch)
      and keeps the fastest.
 Results are cached in a JSON file keyed by a hash of the workload, so a
 repeated configuration is tuned once.  For batched mcsub and layered handles
@@ -125,7 +126,7 @@ def _time(mc: MetalMC, n: int, seed: int) -> float:
 
 def autotune_mcxyz(w, n_trial: int = 20_000_000, device: int = 0, use_cache: bool = True,
                    verbose: bool = False) -> P.Tuning:
-    key = workload_key(w, "mcxyz-v1")
+    key = workload_key(w, "mcxyz-v2")
     if use_cache:
         hit = _load_cache().get(key)
         if hit:
@@ -133,12 +134,16 @@ def autotune_mcxyz(w, n_trial: int = 20_000_000, device: int = 0, use_cache: boo
     hist = visit_histogram(w, device=device)
     with MetalMC(w.sim, device) as mc:
         mc.prepare_data(w.rp, w.tp, w.v, w.ext)
-        default = mc.get_tuning()
-        budget = default.tile[3] * default.tile[4] * default.tile[5]
-        boxes = [b for _, b in best_boxes(hist, budget)] if budget > 0 else []
-        trials = [P.Tuning.make(block=1024, refill_min=r, tile=b)
-                  for b in boxes + [tuple(default.tile)] for r in (8, 16, 32)]
-        trials += [P.Tuning.make(block=512, refill_min=16, tile=b) for b in boxes[:1]]
+        trials = []
+        for mode in (0, 1):     # FP64 tile, FP32 tile (twice the voxels)
+            mc.set_tuning(P.Tuning.make(mode=mode))
+            default = mc.get_tuning()
+            budget = default.tile[3] * default.tile[4] * default.tile[5]
+            boxes = [b for _, b in best_boxes(hist, budget)] if budget > 0 else []
+            trials += [P.Tuning.make(block=1024, refill_min=r, tile=b, mode=mode)
+                       for b in boxes + [tuple(default.tile)] for r in (8, 16, 32)]
+            trials += [P.Tuning.make(block=512, refill_min=16, tile=b, mode=mode)
+                       for b in boxes[:1]]
         best, best_ms, log = None, float("inf"), []
         _time(mc, n_trial // 4, w.ext.seed)   # warm-up
         for t in trials:

@@ -42,6 +42,7 @@ struct LPhoton {
   uint32_t n_same, n_cross, n_int;
   double acur;
   bool alive;
+  KeySched ks;  // this stack's key schedule
   float L[NMC_MAX_LAYERS];
   uint32_t K[NMC_MAX_LAYERS];
 };
@@ -91,6 +92,7 @@ __global__ __launch_bounds__(kLayBlock) void layered_kernel(LayArgs a) {
   const int lane = lane_id();
   LPhoton p;
   p.alive = false;
+  p.ks = key_sched(0u);
   p.item = 0;
   p.l = 0;
   p.acur = 0.0;
@@ -151,7 +153,8 @@ __global__ __launch_bounds__(kLayBlock) void layered_kernel(LayArgs a) {
           p.Lcur = 0.0f;
           p.Kcur = 0u;
           p.acur = 0.0;
-          U4 b = philox(p.pid, 1u, 0u, a.key0, H.key1);
+          p.ks = key_sched(H.key1);
+          U4 b = philox(p.pid, 1u, 0u, a.key0, p.ks);
           p.b1 = b.y;
           p.b2 = b.z;
           p.b3 = b.w;
@@ -198,7 +201,7 @@ __global__ __launch_bounds__(kLayBlock) void layered_kernel(LayArgs a) {
                     : (p.l == p.nl - 1 ? H.n_below : rows[p.row0 + p.l + 1].n);
       float ca2;
       float r = fresnel(ni_, nt, fabsf(p.uz), &ca2);
-      U4 aux = philox(p.pid, p.hits, 2u, a.key0, H.key1);
+      U4 aux = philox(p.pid, p.hits, 2u, a.key0, p.ks);
       p.hits++;
       if (u01(aux.x) < r) {
         p.uz = -p.uz;
@@ -279,7 +282,7 @@ __global__ __launch_bounds__(kLayBlock) void layered_kernel(LayArgs a) {
       }
       if (alive) {
         p.ev++;
-        U4 b = philox(p.pid, p.ev, 0u, a.key0, hdr[p.item].key1);
+        U4 b = philox(p.pid, p.ev, 0u, a.key0, p.ks);
         p.b1 = b.y;
         p.b2 = b.z;
         p.b3 = b.w;

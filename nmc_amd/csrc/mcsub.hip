@@ -25,6 +25,17 @@ namespace {
 
 __device__ __forceinline__ int rbin(float q, int n) { return q >= (float)n ? n : (int)q; }
 
+// Largest j in [0, n) with prefix[j] <= q (the work item of queue index q).
+__device__ __forceinline__ int find_item(const uint64_t* prefix, int n, uint64_t q) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (prefix[mid] <= q) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
 __device__ __forceinline__ void addf(nmc_totals* t, int k, double v) {
   unsafeAtomicAdd(&((double*)t)[k], v);
 }
@@ -60,11 +71,16 @@ __global__ __launch_bounds__(BLOCK) void mcsub_kernel(SubArgs a) {
   const int lane = lane_id();
   float x = 0.0f, y = 0.0f, z = 0.0f, ux = 0.0f, uy = 0.0f, uz = 1.0f, W = 0.0f;
   SubMedium m{};
+  KeySched ks = key_sched(0u);
   int item = 0;
   uint64_t pid = 0;
   uint32_t ev = 0, events = 0, c_int = 0;
   bool alive = false, finished = false;
   uint64_t pool_next = 0, pool_end = 0;  // wave-uniform
+  // wave-uniform: the item holding queue range [pool_lo, pool_hi), found once
+  // per pool position so most refills need no per-lane search
+  int pool_item = 0;
+  uint64_t pool_lo = 0, pool_hi = 0;
   double absorbed = 0.0;
 
   for (;;) {
@@ -82,26 +98,30 @@ __global__ __launch_bounds__(BLOCK) void mcsub_kernel(SubArgs a) {
         pool_next = base;
         pool_end = base + a.chunk < total ? base + a.chunk : total;
       }
+      if (pool_next < pool_lo || pool_next >= pool_hi) {
+        pool_item = find_item(prefix, ni, pool_next);
+        pool_lo = prefix[pool_item];
+        pool_hi = prefix[pool_item + 1];
+      }
       uint64_t avail = pool_end - pool_next;
       uint64_t cnt = (uint64_t)__popcll(need);
       uint32_t take = (uint32_t)(cnt < avail ? cnt : avail);
       uint32_t rank = (uint32_t)__popcll(need & lanemask_lt());
       if (((need >> lane) & 1ull) && rank < take) {
         uint64_t q = pool_next + rank;
-        int lo = 0, hi = ni - 1;  // largest j with prefix[j] <= q
-        while (lo < hi) {
-          int mid = (lo + hi + 1) >> 1;
-          if (prefix[mid] <= q) lo = mid;
-          else hi = mid - 1;
-        }
+        int lo = q < pool_hi ? pool_item : find_item(prefix, ni, q);
         m = med[lo];
+        ks = key_sched(m.key1);
         item = m.slot;
         pid = first + (q - prefix[lo]);
-        // LAUNCH (kernels.metal:106-166)
-        U4 b = philox(pid, 0u, 0u, a.key0, m.key1);
+        // LAUNCH (kernels.metal:106-166); a pencil beam (radius 0) uses no draw
         float rsp;
         if (a.mcflag == 0) {
-          x = a.radius * sqrtf(u01(b.x));
+          x = 0.0f;
+          if (a.radius != 0.0f) {
+            U4 b = philox(pid, 0u, 0u, a.key0, m.key1);
+            x = a.radius * sqrtf(u01(b.x));
+          }
           y = 0.0f;
           z = 0.0f;
           ux = 0.0f;
@@ -109,6 +129,7 @@ __global__ __launch_bounds__(BLOCK) void mcsub_kernel(SubArgs a) {
           uz = 1.0f;
           rsp = m.rsp0;
         } else if (a.mcflag == 1) {
+          U4 b = philox(pid, 0u, 0u, a.key0, m.key1);
           x = a.radius * sqrtf(-log_pos(u01nz(b.x)));
           y = 0.0f;
           z = 0.0f;
@@ -121,6 +142,7 @@ __global__ __launch_bounds__(BLOCK) void mcsub_kernel(SubArgs a) {
           ux = sqrtf(1.0f - uz * uz);
           uy = 0.0f;
         } else {
+          U4 b = philox(pid, 0u, 0u, a.key0, m.key1);
           float ct = 1.0f - 2.0f * u01(b.x);
           float stt = sqrtf(1.0f - ct * ct);
           float sp, cp;
@@ -148,7 +170,7 @@ __global__ __launch_bounds__(BLOCK) void mcsub_kernel(SubArgs a) {
     if (ballot(alive) == 0ull) break;
     if (!alive) continue;
 
-    U4 b = philox(pid, ev, 0u, a.key0, m.key1);
+    U4 b = philox(pid, ev, 0u, a.key0, ks);
     // HOP (kernels.metal:183-187)
     float s = -log_pos(u01nz(b.x)) * m.inv_mut;
     float x0 = x, y0 = y, z0 = z;
@@ -159,11 +181,10 @@ __global__ __launch_bounds__(BLOCK) void mcsub_kernel(SubArgs a) {
     bool ended = false;
     // ESCAPE at z <= 0 (kernels.metal:190-214)
     if (a.bmode == 1 && z <= 0.0f) {
-      U4 aux = philox(pid, ev, 1u, a.key0, m.key1);
       addi(&tot[item], TI_SURFACE_HITS, 1);
       float uz1;
       float rf = fresnel(m.n1, m.n2, -uz, &uz1);
-      if (u01(aux.x) > rf) {
+      if (u01_low(b) > rf) {
         float s2 = z0 / -uz;
         float xe = fmaf(s2, ux, x0);
         float ye = fmaf(s2, uy, y0);

@@ -47,13 +47,14 @@ __device__ __forceinline__ void lds_addf(nmc_totals* t, int k, double v) {
 }
 
 // Adds the deposits of the current voxel visit to the tile or the global grid.
-__device__ __forceinline__ void flush_visit(Photon& p, const XyzArgs& a, double* tile) {
+template <typename TileT>
+__device__ __forceinline__ void flush_visit(Photon& p, const XyzArgs& a, TileT* tile) {
   if (p.inside && a.F && (a.count_visits || p.vdep != 0.0)) {
     double v = a.count_visits ? 1.0 : p.vdep;
     unsigned lx = (unsigned)(p.ix - a.tile_x0), ly = (unsigned)(p.iy - a.tile_y0),
              lz = (unsigned)(p.iz - a.tile_z0);
     if (lx < (unsigned)a.tile_nx && ly < (unsigned)a.tile_ny && lz < (unsigned)a.tile_nz)
-      unsafeAtomicAdd(&tile[((int)lz * a.tile_nx + (int)lx) * a.tile_ny + (int)ly], v);
+      unsafeAtomicAdd(&tile[((int)lz * a.tile_nx + (int)lx) * a.tile_ny + (int)ly], (TileT)v);
     else
       unsafeAtomicAdd(&a.F[vidx(a, p.ix, p.iy, p.iz)], v);
   }
@@ -151,14 +152,15 @@ __device__ __forceinline__ void launch(Photon& p, const XyzArgs& a, const XyzTis
   p.alive = true;
 }
 
-template <bool TWO_TERM, int BLOCK>
+template <bool TWO_TERM, int BLOCK, typename TileT>
 __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
-  extern __shared__ double tile[];  // [tile_nz][tile_nx][tile_ny]
+  extern __shared__ double tile_mem[];
+  TileT* tile = (TileT*)tile_mem;  // [tile_nz][tile_nx][tile_ny]
   __shared__ XyzTissue T[NMC_NTISS];
   __shared__ nmc_totals st;
   const int tile_n = a.tile_nx * a.tile_ny * a.tile_nz;
   for (int i = threadIdx.x; i < NMC_NTISS; i += BLOCK) T[i] = a.tissue[i];
-  for (int i = threadIdx.x; i < tile_n; i += BLOCK) tile[i] = 0.0;
+  for (int i = threadIdx.x; i < tile_n; i += BLOCK) tile[i] = (TileT)0;
   if (threadIdx.x < TF_COUNT) ((double*)&st)[threadIdx.x] = 0.0;
   if (threadIdx.x < TI_COUNT) (&st.photons)[threadIdx.x] = 0ull;
   __syncthreads();
@@ -368,7 +370,7 @@ __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
   __syncthreads();
   if (tile_n) {
     double* part = a.tile_part + (int64_t)blockIdx.x * tile_n;
-    for (int i = threadIdx.x; i < tile_n; i += BLOCK) part[i] = tile[i];
+    for (int i = threadIdx.x; i < tile_n; i += BLOCK) part[i] = (double)tile[i];
   }
   if (threadIdx.x < TF_COUNT) unsafeAtomicAdd(&((double*)a.tot)[threadIdx.x], ((double*)&st)[threadIdx.x]);
   if (threadIdx.x < TI_COUNT)
@@ -389,56 +391,55 @@ __global__ void tile_reduce_kernel(const double* __restrict__ part, int grid, Xy
   }
 }
 
-template <bool TT>
+template <bool TT, typename TileT>
 hipError_t launch_block(const XyzArgs& a, int block, int grid, size_t lds, hipStream_t s) {
   switch (block) {
     case 256:
-      hipLaunchKernelGGL((mcxyz_kernel<TT, 256>), dim3(grid), dim3(256), lds, s, a);
+      hipLaunchKernelGGL((mcxyz_kernel<TT, 256, TileT>), dim3(grid), dim3(256), lds, s, a);
       break;
     case 512:
-      hipLaunchKernelGGL((mcxyz_kernel<TT, 512>), dim3(grid), dim3(512), lds, s, a);
+      hipLaunchKernelGGL((mcxyz_kernel<TT, 512, TileT>), dim3(grid), dim3(512), lds, s, a);
       break;
     default:
-      hipLaunchKernelGGL((mcxyz_kernel<TT, 1024>), dim3(grid), dim3(1024), lds, s, a);
+      hipLaunchKernelGGL((mcxyz_kernel<TT, 1024, TileT>), dim3(grid), dim3(1024), lds, s, a);
       break;
   }
   return hipGetLastError();
 }
 
-template <bool TT>
-hipError_t configure(int block, size_t lds) {
-  const void* f = block == 256   ? (const void*)mcxyz_kernel<TT, 256>
-                  : block == 512 ? (const void*)mcxyz_kernel<TT, 512>
-                                 : (const void*)mcxyz_kernel<TT, 1024>;
-  return hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+template <bool TT, typename TileT>
+const void* kernel_of(int block) {
+  return block == 256   ? (const void*)mcxyz_kernel<TT, 256, TileT>
+         : block == 512 ? (const void*)mcxyz_kernel<TT, 512, TileT>
+                        : (const void*)mcxyz_kernel<TT, 1024, TileT>;
 }
 
-template <bool TT>
-int occupancy(int block, size_t lds) {
-  int nb = 0;
-  hipError_t e;
-  switch (block) {
-    case 256: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, mcxyz_kernel<TT, 256>, 256, lds); break;
-    case 512: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, mcxyz_kernel<TT, 512>, 512, lds); break;
-    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, mcxyz_kernel<TT, 1024>, 1024, lds); break;
-  }
-  return (e == hipSuccess) ? nb : 0;
+const void* kernel_for(bool two_term, bool tile32, int block) {
+  if (two_term) return tile32 ? kernel_of<true, float>(block) : kernel_of<true, double>(block);
+  return tile32 ? kernel_of<false, float>(block) : kernel_of<false, double>(block);
 }
 
 }  // namespace
 
-hipError_t launch_mcxyz(const XyzArgs& a, bool two_term, int block, int grid, size_t lds_bytes,
-                        hipStream_t s) {
-  return two_term ? launch_block<true>(a, block, grid, lds_bytes, s)
-                  : launch_block<false>(a, block, grid, lds_bytes, s);
+hipError_t launch_mcxyz(const XyzArgs& a, bool two_term, bool tile32, int block, int grid,
+                        size_t lds_bytes, hipStream_t s) {
+  if (two_term)
+    return tile32 ? launch_block<true, float>(a, block, grid, lds_bytes, s)
+                  : launch_block<true, double>(a, block, grid, lds_bytes, s);
+  return tile32 ? launch_block<false, float>(a, block, grid, lds_bytes, s)
+                : launch_block<false, double>(a, block, grid, lds_bytes, s);
 }
 
-hipError_t mcxyz_configure(bool two_term, int block, size_t lds_bytes) {
-  return two_term ? configure<true>(block, lds_bytes) : configure<false>(block, lds_bytes);
+hipError_t mcxyz_configure(bool two_term, bool tile32, int block, size_t lds_bytes) {
+  return hipFuncSetAttribute(kernel_for(two_term, tile32, block),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
 }
 
-int mcxyz_blocks_per_cu(bool two_term, int block, size_t lds_bytes) {
-  return two_term ? occupancy<true>(block, lds_bytes) : occupancy<false>(block, lds_bytes);
+int mcxyz_blocks_per_cu(bool two_term, bool tile32, int block, size_t lds_bytes) {
+  int nb = 0;
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &nb, kernel_for(two_term, tile32, block), block, lds_bytes);
+  return e == hipSuccess ? nb : 0;
 }
 
 hipError_t launch_tile_reduce(const double* part, int grid, const XyzArgs& a, hipStream_t s) {

@@ -265,8 +265,12 @@ int resolve_xyz_tuning(nmc_handle* h) {
   t.chunk = r.chunk;
   bool tt = h->sim == NMC_SIM_MCXYZ_TT;
   const RunParams& rp = h->rp;
+  if (r.mode < 0 || r.mode > 1) return fail(NMC_ERR_ARG, "tuning.mode (tile element) not 0 or 1");
+  t.mode = r.mode;
+  const bool tile32 = t.mode == 1;
+  const size_t elem = tile32 ? sizeof(float) : sizeof(double);
   const size_t static_lds = sizeof(XyzTissue) * NMC_NTISS + sizeof(nmc_totals) + 256;
-  int64_t budget = h->lds_max > static_lds ? (int64_t)((h->lds_max - static_lds) / sizeof(double)) : 0;
+  int64_t budget = h->lds_max > static_lds ? (int64_t)((h->lds_max - static_lds) / elem) : 0;
   if (!h->dF || r.tile[3] < 0) {
     for (int i = 0; i < 6; ++i) t.tile[i] = 0;
   } else if (r.tile[3] > 0) {
@@ -298,9 +302,9 @@ int resolve_xyz_tuning(nmc_handle* h) {
     if (tile_n > budget) return fail(NMC_ERR_ARG, "tuning.tile needs %lld voxels, LDS holds %lld",
                                      (long long)tile_n, (long long)budget);
   }
-  h->xyz_lds = (size_t)tile_n * sizeof(double);
-  HIP_TRY(mcxyz_configure(tt, t.block, h->xyz_lds));
-  int occ = mcxyz_blocks_per_cu(tt, t.block, h->xyz_lds);
+  h->xyz_lds = (size_t)tile_n * elem;
+  HIP_TRY(mcxyz_configure(tt, tile32, t.block, h->xyz_lds));
+  int occ = mcxyz_blocks_per_cu(tt, tile32, t.block, h->xyz_lds);
   if (occ <= 0) return fail(NMC_ERR_ARG, "kernel does not fit on a CU with block %d, %zu B LDS", t.block, h->xyz_lds);
   t.blocks_per_cu = r.blocks_per_cu > 0 ? std::min(r.blocks_per_cu, occ) : occ;
   h->xyz_grid = h->num_cu * t.blocks_per_cu;
@@ -315,10 +319,10 @@ int resolve_xyz_tuning(nmc_handle* h) {
 int resolve_sub_tuning(nmc_handle* h) {
   const nmc_tuning& r = h->tune_req;
   nmc_tuning t{};
-  // tally placement: tuning.reserved[0] = 1 LDS, 2 global atomics, 0 automatic
+  // tally placement: tuning.mode = 1 LDS, 2 global atomics, 0 automatic
   // (LDS whenever the per-item tallies fit)
-  int mode = r.reserved[0];
-  if (mode < 0 || mode > 2) return fail(NMC_ERR_ARG, "tuning.reserved[0] (tally placement) not in 0..2");
+  int mode = r.mode;
+  if (mode < 0 || mode > 2) return fail(NMC_ERR_ARG, "tuning.mode (tally placement) not in 0..2");
   bool want_lds = mode != 2;
   size_t lds_full = mcsub_lds_bytes(h->n_items, per_item(h)) + 512;
   h->sub_lds = want_lds && lds_full <= h->lds_max;
@@ -330,7 +334,7 @@ int resolve_sub_tuning(nmc_handle* h) {
   t.refill_min = r.refill_min ? r.refill_min : 1;  // mcsub's launch is cheap: refill at once
   if (t.refill_min < 1 || t.refill_min > 64) return fail(NMC_ERR_ARG, "tuning.refill_min not in 1..64");
   t.chunk = r.chunk;
-  t.reserved[0] = h->sub_lds ? 1 : 2;
+  t.mode = h->sub_lds ? 1 : 2;
   HIP_TRY(mcsub_configure(t.block, h->sub_lds_bytes, h->sub_lds));
   int occ = mcsub_blocks_per_cu(t.block, h->sub_lds_bytes, h->sub_lds);
   if (occ <= 0) return fail(NMC_ERR_ARG, "mcsub kernel does not fit on a CU");
@@ -459,7 +463,7 @@ int enqueue_run(nmc_handle* h, uint64_t n) {
     a.tot = h->dTot;
     a.state = h->dState;
     bool tt = h->sim == NMC_SIM_MCXYZ_TT;
-    HIP_TRY(launch_mcxyz(a, tt, h->tune.block, h->xyz_grid, h->xyz_lds, s));
+    HIP_TRY(launch_mcxyz(a, tt, h->tune.mode == 1, h->tune.block, h->xyz_grid, h->xyz_lds, s));
     HIP_TRY(launch_tile_reduce(h->dTilePart, h->xyz_grid, a, s));
     HIP_TRY(launch_state_advance(h->dState, n, s));
   }

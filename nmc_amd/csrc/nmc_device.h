@@ -39,11 +39,48 @@ __device__ __forceinline__ U4 philox(uint64_t pid, uint32_t ev, uint32_t tag, ui
   return U4{c0, c1, c2, c3};
 }
 
+// The same generator with the per-lane half of the key schedule precomputed
+// (k1s[r] = k1 + r * 0xBB67AE85), for loops whose key changes rarely.
+struct KeySched {
+  uint32_t k1[10];
+};
+
+__device__ __forceinline__ KeySched key_sched(uint32_t k1) {
+  KeySched s;
+#pragma unroll
+  for (int r = 0; r < 10; ++r) s.k1[r] = k1 + (uint32_t)r * 0xBB67AE85u;
+  return s;
+}
+
+__device__ __forceinline__ U4 philox(uint64_t pid, uint32_t ev, uint32_t tag, uint32_t k0,
+                                     const KeySched& ks) {
+  uint32_t c0 = (uint32_t)pid, c1 = (uint32_t)(pid >> 32), c2 = ev, c3 = tag;
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+    uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+    uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+    uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ ks.k1[r];
+    c1 = (uint32_t)p1;
+    c3 = (uint32_t)p0;
+    c0 = n0;
+    c2 = n2;
+    k0 += 0x9E3779B9u;
+  }
+  return U4{c0, c1, c2, c3};
+}
+
 // [0,1): top 24 bits scaled by 2^-24 (the reference's rand() mapping,
 // helpers.metal:319-371); (0,1]: the same shifted by one ulp of 2^-24, used
 // where the reference rejects zero.
 __device__ __forceinline__ float u01(uint32_t x) { return (float)(x >> 8) * 0x1p-24f; }
 __device__ __forceinline__ float u01nz(uint32_t x) { return (float)((x >> 8) + 1u) * 0x1p-24f; }
+// A fifth uniform in [0,1) from the low bytes of words x, y, z, which the
+// four top-24-bit uniforms of a block leave unused.
+__device__ __forceinline__ float u01_low(const U4& b) {
+  uint32_t v = ((b.x & 0xffu) << 16) | ((b.y & 0xffu) << 8) | (b.z & 0xffu);
+  return (float)v * 0x1p-24f;
+}
 
 // ---------------------------------------------------------------------------
 // Float transcendentals (argument domains as used by the transport loop).

@@ -139,10 +139,10 @@ hipError_t launch_layered(const LayArgs& a, int grid, size_t lds_bytes, hipStrea
 hipError_t launch_layered_backward(const LayBackArgs& b, hipStream_t s);
 
 
-hipError_t launch_mcxyz(const XyzArgs& a, bool two_term, int block, int grid, size_t lds_bytes,
-                        hipStream_t s);
-hipError_t mcxyz_configure(bool two_term, int block, size_t lds_bytes);
-int mcxyz_blocks_per_cu(bool two_term, int block, size_t lds_bytes);
+hipError_t launch_mcxyz(const XyzArgs& a, bool two_term, bool tile32, int block, int grid,
+                        size_t lds_bytes, hipStream_t s);
+hipError_t mcxyz_configure(bool two_term, bool tile32, int block, size_t lds_bytes);
+int mcxyz_blocks_per_cu(bool two_term, bool tile32, int block, size_t lds_bytes);
 hipError_t launch_tile_reduce(const double* part, int grid, const XyzArgs& a, hipStream_t s);
 hipError_t launch_mcsub(const SubArgs& a, int block, int grid, size_t lds_bytes, bool lds_tallies,
                         hipStream_t s);

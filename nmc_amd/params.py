@@ -122,20 +122,25 @@ class Results(C.Structure):
 
 
 class Tuning(C.Structure):
-    """nmc_tuning: launch shape (block, blocks/CU, refill batch, queue chunk) and
-    the mcxyz LDS fluence tile box (x0, y0, z0, nx, ny, nz)."""
+    """nmc_tuning: launch shape (block, blocks/CU, refill batch, queue chunk), the
+    mcxyz LDS fluence tile box (x0, y0, z0, nx, ny, nz) and `mode` (mcsub tally
+    placement 0 auto / 1 LDS / 2 global; mcxyz tile element 0 FP64 / 1 FP32)."""
 
     _fields_ = [("block", C.c_int32), ("blocks_per_cu", C.c_int32), ("refill_min", C.c_int32),
-                ("chunk", C.c_int32), ("tile", C.c_int32 * 6), ("reserved", C.c_int32 * 6)]
+                ("chunk", C.c_int32), ("tile", C.c_int32 * 6), ("mode", C.c_int32),
+                ("reserved", C.c_int32 * 5)]
 
     def as_dict(self) -> dict:
         return {"block": self.block, "blocks_per_cu": self.blocks_per_cu,
-                "refill_min": self.refill_min, "chunk": self.chunk, "tile": list(self.tile)}
+                "refill_min": self.refill_min, "chunk": self.chunk, "tile": list(self.tile),
+                "mode": self.mode}
 
     @classmethod
-    def make(cls, block=0, blocks_per_cu=0, refill_min=0, chunk=0, tile=None) -> "Tuning":
+    def make(cls, block=0, blocks_per_cu=0, refill_min=0, chunk=0, tile=None,
+             mode=0) -> "Tuning":
         t = cls()
         t.block, t.blocks_per_cu, t.refill_min, t.chunk = block, blocks_per_cu, refill_min, chunk
+        t.mode = mode
         for i, v in enumerate(tile or (0,) * 6):
             t.tile[i] = v
         return t
@@ -197,7 +202,7 @@ class LayeredConfig(C.Structure):
 
 assert C.sizeof(RunParams) == 172
 assert C.sizeof(TissueParams) == 608
-assert C.sizeof(WorkItem) == 32
+assert C.sizeof(WorkItem) == 32 and C.sizeof(Tuning) == 64
 assert C.sizeof(Layer) == 48 and C.sizeof(Stack) == 408 and C.sizeof(LayeredConfig) == 80
 
 

@@ -240,3 +240,19 @@ def tiny_voxel_case(kind: str = "mismatch", n_photons: int = 2000, seed: int = 7
         tp.gf[3], tp.af[3], tp.gb[3], tp.ab[3], tp.CC[3] = 0.7, 0.7, 0.2, 1.0, 0.6
     return Workload("tiny_" + kind, rp.mckernelflag, rp, tp, v,
                     P.ExtConfig.default(seed=seed), n_photons)
+
+
+def example0_slab(n_photons: int = 100_000, seed: int = 1) -> Workload:
+    """The homogeneous run of NMC-samples/example0.ipynb: 200^3 voxels of
+    0.02, mua 1, mus 100, g 0.9, n 1.4 against 1.001, a flat beam of radius
+    1.2 entering at z = zsurf = 0, escape at the surface only."""
+    n = 200
+    rp = _rp("test", Nx=n, Ny=n, Nz=n, dx=0.02, dy=0.02, dz=0.02, Nt=1, mcflag=0, launchflag=0,
+             boundaryflag=2, xs=0.0, ys=0.0, zs=0.0, zfocus=1e12, radius=1.2, waist=1.2,
+             zsurf=0.0, det_radius=5.0, xd=0.0, n1=1.4, n2=1.001, na=90.0, lambda_=640.0e-6,
+             lc=0.001, semiflag=1, time_min=1.0, mckernelflag=P.SIM_MCXYZ,
+             det_state=P.DET_REFLECTANCE, ux0=0.7, uy0=0.4, uz0=(1 - 0.7 ** 2 - 0.4 ** 2) ** 0.5)
+    tp = _tp({1: (1.0, 100.0, 0.9)})
+    v = np.ones(n * n * n, np.uint8)
+    return Workload("example0_slab", P.SIM_MCXYZ, rp, tp, v, P.ExtConfig.default(seed=seed),
+                    n_photons)

@@ -78,6 +78,13 @@ static inline void block(const uint32_t key[2], uint64_t pid, uint32_t ev, uint3
 static inline float u01(uint32_t x) { return (float)(x >> 8) * 0x1p-24f; }
 /* uniform in (0,1]: used where the reference loops `while ((rnd=rand())<=0)` */
 static inline float u01nz(uint32_t x) { return (float)((x >> 8) + 1u) * 0x1p-24f; }
+/* a fifth uniform in [0,1) from the low bytes of words 0-2, which the four
+ * top-24-bit uniforms of the block leave unused (mcsub's escape test) */
+static inline float u01_low(const uint32_t b[4])
+{
+    uint32_t v = ((b[0] & 0xffu) << 16) | ((b[1] & 0xffu) << 8) | (b[2] & 0xffu);
+    return (float)v * 0x1p-24f;
+}
 
 /* ------------------------------------------------------------------------ */
 /* Transcendentals.                                                          */
@@ -787,12 +794,10 @@ static void sub_photon(const sub_ctx* c, uint64_t pid, sub_acc* a)
         events++;
         /* ESCAPE at z <= 0 (kernels.metal:190-214) */
         if (c->bmode == 1 && z <= 0.0f) {
-            uint32_t aux[4];
-            block(c->key, pid, ev, 1u, aux);
             t->surface_hits++;
             float uz1;
             float rf = nmco_fresnel(c->n1, c->n2, -uz, &uz1);
-            if (u01(aux[0]) > rf) {
+            if (u01_low(b) > rf) {
                 float s2 = z0 / -uz;
                 x = fmaf(s2, ux, x0);
                 y = fmaf(s2, uy, y0);

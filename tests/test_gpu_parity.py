@@ -90,6 +90,29 @@ def test_visit_counts_bit_exact(gpu):
     assert got.fluence.sum() == t["substeps_cross"] - t["surface_hits"] + ends_inside
 
 
+def test_fp32_tile(gpu):
+    """FP32 LDS tile (tuning.mode 1): integer tallies bit-exact; fluence within
+    FP32 summation error of the oracle (sums of one workgroup's visits in FP32,
+    across workgroups in FP64)."""
+    w = W.skin_vessel()
+    n = 200_000
+    _say("[parity] skin-vessel fp32 tile n=200000")
+    ref = O.run(w, n, threads=8)
+    with MetalMC(w.sim) as mc:
+        mc.prepare_data(w.rp, w.tp, w.v, w.ext)
+        mc.set_tuning(P.Tuning.make(mode=1))
+        t = mc.get_tuning()
+        assert t.mode == 1 and t.tile[3] * t.tile[4] * t.tile[5] > 20000   # twice the FP64 box
+        mc.set_num_runs_seed(w.ext.seed, 0)
+        mc.send_compute_command(n)
+        got = mc.get_compute_results()
+    for k in P.TOTAL_INT_FIELDS:
+        assert got.totals[k] == ref.totals[k], k
+    assert np.array_equal(got.det2d, ref.det2d)
+    np.testing.assert_allclose(got.fluence, ref.fluence, rtol=2e-5, atol=1e-9)
+    assert abs(got.fluence.sum() - ref.fluence.sum()) <= 1e-6 * ref.fluence.sum()
+
+
 def test_autotuner_runs(gpu):
     from nmc_amd.autotune import autotune_mcxyz
 

@@ -27,9 +27,10 @@ needs_ref = pytest.mark.skipif(not os.path.exists(REF),
                                reason="oracle/_ref/ref_mcio not built (needs the reference sources)")
 
 
-def notebook_mci(path_name: str, rp: P.RunParams, tp: P.TissueParams) -> None:
+def notebook_mci(path_name: str, rp: P.RunParams, tp: P.TissueParams, props=None) -> None:
     """The notebooks' writer (NMC-samples/example2.ipynb:533-562): fixed
-    decimals, `inf` for an infinite focus, tissues 1..Nt."""
+    decimals, `inf` for an infinite focus, tissues 1..Nt.  `props` gives the
+    tissues' (mua, mus, g) in double precision, as the notebooks hold them."""
     f4 = lambda x: f"{x:.4f}"  # noqa: E731
     lines = [f"{rp.time_min:.2f}", str(rp.Nx), str(rp.Ny), str(rp.Nz), f4(rp.dx), f4(rp.dy),
              f4(rp.dz), str(rp.mcflag), str(rp.launchflag), str(rp.boundaryflag), f4(rp.xs),
@@ -39,9 +40,10 @@ def notebook_mci(path_name: str, rp: P.RunParams, tp: P.TissueParams) -> None:
              f4(rp.lc), str(rp.speckleflag), str(rp.semiflag), str(rp.det_state),
              str(rp.mckernelflag), str(rp.Nt)]
     for i in range(1, rp.Nt + 1):
-        lines += [f4(tp.muav[i]), f4(tp.musv[i])]
+        mua, mus, g = props[i - 1] if props else (tp.muav[i], tp.musv[i], tp.gv[i])
+        lines += [f4(mua), f4(mus)]
         if rp.mckernelflag < 2:
-            lines.append(f4(tp.gv[i]))
+            lines.append(f4(g))
         else:
             lines += [f4(getattr(tp, nm)[i]) for nm in mcio.TT_FIELDS]
     with open(path_name + "_H.mci", "w") as f:

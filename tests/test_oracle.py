@@ -48,6 +48,24 @@ def test_philox_matches_aten_engine():
         assert O.philox(v["ctr"], v["key"]) == v["out"], v
 
 
+def test_spare_low_bytes_give_an_independent_uniform():
+    """mcsub's escape test uses a fifth uniform from the low bytes of words
+    0-2 of the event's block: uniform on 64 bins and independent of the
+    step-length draw (top 24 bits of word 0), by chi-square."""
+    from scipy import stats
+    n = 40000
+    low = np.empty(n)
+    top = np.empty(n)
+    for i in range(n):
+        b = O.philox([i * 7919 & 0xFFFFFFFF, 0, 1 + (i % 13), 0], [1, 0])
+        low[i] = (((b[0] & 0xFF) << 16) | ((b[1] & 0xFF) << 8) | (b[2] & 0xFF)) * 2.0 ** -24
+        top[i] = (b[0] >> 8) * 2.0 ** -24
+    counts = np.histogram(low, bins=64, range=(0, 1))[0]
+    assert stats.chisquare(counts).pvalue > 1e-4
+    joint = np.histogram2d(low, top, bins=8, range=[[0, 1], [0, 1]])[0]
+    assert stats.chi2_contingency(joint)[1] > 1e-4
+
+
 # --------------------------------------------------------- transcendentals ---
 def test_logf_all_uniform_draws():
     """-log(u) is taken of every u = k 2^-24, k = 1..2^24: check them all."""
