@@ -307,6 +307,12 @@ int nmc_layered_run(nmc_handle* h);
  * NMC_MAX_LAYERS; totals: n_stacks; n_records: records stored (any may be NULL). */
 int nmc_layered_results(nmc_handle* h, double* rd, double* absorbed, nmc_totals* totals,
                         uint64_t* n_records);
+/* Copies the path records of the last run to the caller, max_records at most,
+ * each 3 + 2 * rec_nl u32 words: stack, Rd bin, weight (f32 bits), L[rec_nl]
+ * (f32 bits), K[rec_nl], where rec_nl is the deepest stack's layer count; the
+ * order is unspecified.  *n_out = records copied; *rec_nl_out = rec_nl. */
+int nmc_layered_records(nmc_handle* h, uint32_t* out, uint64_t max_records, uint64_t* n_out,
+                        int* rec_nl_out);
 /* Perturbation-MC backward pass over the stored records: given
  * grad_rd[j][ir] = dLoss/d(rd[j][ir] / n_photons_j), returns
  * grad_mua[j][l] = sum over records of grad_rd * w * (-L_l) / n_j and

@@ -61,6 +61,8 @@ def lib():
         "nmc_layered_results": (C.c_int, [H, C.c_void_p, C.c_void_p, C.POINTER(P.Totals),
                                           C.POINTER(C.c_uint64)]),
         "nmc_layered_backward": (C.c_int, [H, C.c_void_p, C.c_void_p, C.c_void_p]),
+        "nmc_layered_records": (C.c_int, [H, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64),
+                                          C.POINTER(C.c_int)]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(L, name)
@@ -76,7 +78,7 @@ EXPORTED = ("nmc_ext_config_init", "nmc_validate", "nmc_create", "nmc_prepare", 
             "nmc_version", "nmc_set_tuning", "nmc_get_tuning", "nmc_prepare_batch",
             "nmc_run_batch", "nmc_run_batch_async", "nmc_get_batch_results",
             "nmc_set_graph_mode", "nmc_layered_prepare", "nmc_layered_update", "nmc_layered_run",
-            "nmc_layered_results", "nmc_layered_backward")
+            "nmc_layered_results", "nmc_layered_backward", "nmc_layered_records")
 
 
 def check(status: int) -> None:

@@ -42,6 +42,8 @@ struct LPhoton {
   uint32_t n_same, n_cross, n_int;
   double acur;
   bool alive;
+  bool rec_pending;  // left through the top; record written at the next wave-wide point
+  int rec_ir;
   KeySched ks;  // this stack's key schedule
   float L[NMC_MAX_LAYERS];
   uint32_t K[NMC_MAX_LAYERS];
@@ -72,6 +74,50 @@ __device__ __forceinline__ void fold_layer(LPhoton& p, double* absorbed) {
   p.acur = 0.0;
 }
 
+// Writes the records of the wave's lanes that left through the top since the
+// last call (all lanes of the wave call it together).  Slots come from a
+// wave-private run [next, end) of kRecChunk slots reserved with one global
+// atomic; a flush that overruns the run continues in a fresh one.
+__device__ __forceinline__ void flush_records(LPhoton& p, const LayArgs& a, int lane,
+                                              uint64_t& next, uint64_t& end,
+                                              uint32_t& written) {
+  uint64_t m = ballot(p.rec_pending);
+  if (!m) return;
+  uint64_t c = (uint64_t)__popcll(m);
+  uint64_t rem = end - next;
+  unsigned long long fresh = 0;
+  if (c > rem) {
+    if (lane == 0) fresh = atomicAdd(&a.rec_count[0], (unsigned long long)kRecChunk);
+    fresh = __shfl(fresh, 0);
+  }
+  if (p.rec_pending) {
+    uint64_t rank = (uint64_t)__popcll(m & lanemask_lt());
+    uint64_t k = rank < rem ? next + rank : fresh + (rank - rem);
+    if (k < a.cap) {
+      const int rs = 3 + 2 * a.rec_nl;
+      uint32_t* r = a.rec + k * (uint64_t)rs;
+      r[0] = (uint32_t)p.item;
+      r[1] = (uint32_t)p.rec_ir;
+      r[2] = __float_as_uint(p.W);
+#pragma unroll
+      for (int i = 0; i < NMC_MAX_LAYERS; ++i) {
+        if (i < a.rec_nl) {
+          r[3 + i] = __float_as_uint(p.L[i]);
+          r[3 + a.rec_nl + i] = p.K[i];
+        }
+      }
+      written++;
+    }
+    p.rec_pending = false;
+  }
+  if (c > rem) {
+    next = fresh + (c - rem);
+    end = fresh + kRecChunk;
+  } else {
+    next += c;
+  }
+}
+
 __global__ __launch_bounds__(kLayBlock) void layered_kernel(LayArgs a) {
   extern __shared__ double smem[];
   // [items] absorbed[MAXL] | [items] totals | [items] headers | [items+1] prefix | [rows]
@@ -96,10 +142,15 @@ __global__ __launch_bounds__(kLayBlock) void layered_kernel(LayArgs a) {
   p.item = 0;
   p.l = 0;
   p.acur = 0.0;
+  p.rec_pending = false;
+  p.rec_ir = 0;
   bool finished = false;
   uint64_t pool_next = 0, pool_end = 0;
+  uint64_t rec_next = 0, rec_end = 0;  // wave-uniform record slot run
+  uint32_t rec_written = 0;
 
   for (;;) {
+    if (a.records) flush_records(p, a, lane, rec_next, rec_end, rec_written);
     uint64_t idle = ballot(!p.alive && !finished);
     if (idle && ((uint32_t)__popcll(idle) >= a.refill_min || ballot(p.alive) == 0ull)) {
       while (idle) {
@@ -215,27 +266,8 @@ __global__ __launch_bounds__(kLayBlock) void layered_kernel(LayArgs a) {
         addf(T, TF_RD, (double)p.W);
         addi(T, TI_DETECTIONS, 1);
         if (a.records) {
-          // one append per wave: the detecting lanes take consecutive slots
-          uint64_t m = ballot(true);
-          int leader = __ffsll((long long)m) - 1;
-          unsigned long long base = 0;
-          if (lane == leader) base = atomicAdd(a.rec_count, (unsigned long long)__popcll(m));
-          base = __shfl(base, leader);
-          unsigned long long k = base + (unsigned long long)__popcll(m & lanemask_lt());
-          if (k < a.cap) {
-            const int rs = 3 + 2 * a.rec_nl;
-            uint32_t* r = a.rec + k * (uint64_t)rs;
-            r[0] = (uint32_t)p.item;
-            r[1] = (uint32_t)ir;
-            r[2] = __float_as_uint(p.W);
-#pragma unroll
-            for (int i = 0; i < NMC_MAX_LAYERS; ++i) {
-              if (i < a.rec_nl) {
-                r[3 + i] = __float_as_uint(p.L[i]);
-                r[3 + a.rec_nl + i] = p.K[i];
-              }
-            }
-          }
+          p.rec_pending = true;
+          p.rec_ir = ir;
         }
         alive = false;
       } else if (!up && p.l == p.nl - 1) {
@@ -299,6 +331,17 @@ __global__ __launch_bounds__(kLayBlock) void layered_kernel(LayArgs a) {
     }
   }
 
+  if (a.records) {
+    // the unused tail of the wave's slot run is marked as holes; the count of
+    // written records goes out with one atomic per wave
+    const int rs = 3 + 2 * a.rec_nl;
+    for (uint64_t k = rec_next + (uint64_t)lane; k < rec_end; k += 64)
+      if (k < a.cap) a.rec[k * (uint64_t)rs] = kRecHole;
+    uint32_t w = rec_written;
+    for (int o = 32; o > 0; o >>= 1) w += __shfl_xor(w, o);
+    if (lane == 0 && w) atomicAdd(&a.rec_count[1], (unsigned long long)w);
+  }
+
   __syncthreads();
   // absorbed per layer: FP64 LDS -> partials; totals: absorbed sum + the rest
   const int per = ni * (NMC_MAX_LAYERS + TF_COUNT + TI_COUNT);
@@ -329,8 +372,9 @@ __global__ __launch_bounds__(256) void layered_backward_kernel(LayBackArgs b) {
        k += (uint64_t)gridDim.x * blockDim.x) {
     const int rs = 3 + 2 * b.rec_nl;
     const uint32_t* r = b.rec + k * (uint64_t)rs;
-    int j = (int)r[0];
-    if (j >= ni) continue;
+    const uint32_t jr = r[0];
+    if (jr >= (uint32_t)ni) continue;   // a hole (kRecHole) or a foreign stack index
+    const int j = (int)jr;
     if (j != cur) {
       if (cur >= 0) {
 #pragma unroll

@@ -1050,13 +1050,8 @@ int nmc_layered_prepare(nmc_handle* h, const nmc_layered_config* cfg, const nmc_
   HIP_TRY(hipMalloc(&h->dMus, (size_t)n * NMC_MAX_LAYERS * sizeof(float)));
   HIP_TRY(hipMalloc(&h->dNl, n * sizeof(int)));
   HIP_TRY(hipMalloc(&h->dGrad, ((size_t)n * (cfg->nr + 1) + n + 2 * (size_t)n * NMC_MAX_LAYERS) * sizeof(double)));
-  HIP_TRY(hipMalloc(&h->dRecCount, sizeof(unsigned long long)));
+  HIP_TRY(hipMalloc(&h->dRecCount, 2 * sizeof(unsigned long long)));
   h->rec_cap = 0;
-  if (cfg->records) {
-    h->rec_cap = cfg->record_capacity ? cfg->record_capacity : total;
-    if (h->rec_cap == 0) h->rec_cap = 1;
-    HIP_TRY(hipMalloc(&h->dRec, h->rec_cap * (3 + 2 * h->rec_nl) * sizeof(uint32_t)));
-  }
   h->lay_lds = layered_lds_bytes(n, h->lay_rows);
   if (h->lay_lds + 1024 > h->lds_max)
     return fail(NMC_ERR_ARG, "%d stacks / %d layers need %zu B of LDS (%zu available)", n, h->lay_rows,
@@ -1072,6 +1067,14 @@ int nmc_layered_prepare(nmc_handle* h, const nmc_layered_config* cfg, const nmc_
   h->tune.refill_min = h->tune_req.refill_min ? h->tune_req.refill_min : 16;
   h->tune.chunk = h->tune_req.chunk;
   HIP_TRY(hipMalloc(&h->dLPart, (size_t)h->lay_grid * n * (NMC_MAX_LAYERS + TF_COUNT + TI_COUNT) * sizeof(double)));
+  if (cfg->records) {
+    // every photon leaves at most one record; each wave may leave the unused
+    // tail of its last slot run (< kRecChunk slots) as holes
+    uint64_t waves = (uint64_t)h->lay_grid * (kLayBlock / 64);
+    h->rec_cap = cfg->record_capacity ? cfg->record_capacity : total + waves * kRecChunk;
+    if (h->rec_cap == 0) h->rec_cap = 1;
+    HIP_TRY(hipMalloc(&h->dRec, h->rec_cap * (3 + 2 * h->rec_nl) * sizeof(uint32_t)));
+  }
   NMC_TRY(upload_stacks(h));
   h->prepared = true;
   return write_first(h);
@@ -1103,7 +1106,7 @@ int nmc_layered_run(nmc_handle* h) {
   hipStream_t s = h->stream;
   HIP_TRY(hipMemsetAsync(h->dLRd, 0, (size_t)n * (h->lcfg.nr + 1) * sizeof(double), s));
   HIP_TRY(hipMemsetAsync(h->dLOut, 0, (size_t)n * (NMC_MAX_LAYERS + TF_COUNT + TI_COUNT) * sizeof(double), s));
-  HIP_TRY(hipMemsetAsync(h->dRecCount, 0, sizeof(unsigned long long), s));
+  HIP_TRY(hipMemsetAsync(h->dRecCount, 0, 2 * sizeof(unsigned long long), s));
   HIP_TRY(hipEventRecord(h->e0, s));
   HIP_TRY(launch_state_begin(h->dState, s));
   LayArgs a{};
@@ -1150,10 +1153,37 @@ int nmc_layered_results(nmc_handle* h, double* rd, double* absorbed, nmc_totals*
   if (totals)
     HIP_TRY(hipMemcpy(totals, h->dLOut + n * NMC_MAX_LAYERS, n * sizeof(nmc_totals), hipMemcpyDeviceToHost));
   if (n_records) {
-    unsigned long long c = 0;
-    HIP_TRY(hipMemcpy(&c, h->dRecCount, sizeof c, hipMemcpyDeviceToHost));
-    *n_records = c;
+    unsigned long long c[2] = {0, 0};
+    HIP_TRY(hipMemcpy(c, h->dRecCount, sizeof c, hipMemcpyDeviceToHost));
+    *n_records = c[1];
   }
+  return ok();
+}
+
+int nmc_layered_records(nmc_handle* h, uint32_t* out, uint64_t max_records, uint64_t* n_out,
+                        int* rec_nl_out) {
+  if (!h || !n_out) return fail(NMC_ERR_ARG, "null argument");
+  if (h->sim != NMC_SIM_LAYERED || !h->prepared) return fail(NMC_ERR_STATE, "nmc_layered_prepare has not succeeded");
+  if (!h->lcfg.records) return fail(NMC_ERR_STATE, "records were not enabled at prepare");
+  HIP_TRY(hipSetDevice(h->device));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  unsigned long long cnt[2] = {0, 0};
+  HIP_TRY(hipMemcpy(cnt, h->dRecCount, sizeof cnt, hipMemcpyDeviceToHost));
+  const uint64_t slots = std::min<uint64_t>(cnt[0], h->rec_cap);
+  const size_t rs = 3 + 2 * (size_t)h->rec_nl;
+  if (rec_nl_out) *rec_nl_out = h->rec_nl;
+  *n_out = 0;
+  if (!out || max_records == 0 || slots == 0) return ok();
+  std::vector<uint32_t> host(slots * rs);
+  HIP_TRY(hipMemcpy(host.data(), h->dRec, host.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  uint64_t n = 0;
+  for (uint64_t k = 0; k < slots && n < max_records; ++k) {
+    const uint32_t* r = &host[k * rs];
+    if (r[0] == kRecHole) continue;
+    std::copy(r, r + rs, out + n * rs);
+    ++n;
+  }
+  *n_out = n;
   return ok();
 }
 
@@ -1164,10 +1194,11 @@ int nmc_layered_backward(nmc_handle* h, const double* grad_rd, double* grad_mua,
   HIP_TRY(hipSetDevice(h->device));
   hipStream_t s = h->stream;
   size_t n = h->stacks.size(), nb = n * (h->lcfg.nr + 1);
-  unsigned long long nrec = 0;
-  HIP_TRY(hipMemcpy(&nrec, h->dRecCount, sizeof nrec, hipMemcpyDeviceToHost));
+  unsigned long long cnt[2] = {0, 0};   // slots handed out, records written
+  HIP_TRY(hipMemcpy(cnt, h->dRecCount, sizeof cnt, hipMemcpyDeviceToHost));
+  const unsigned long long nrec = cnt[0];
   if (nrec > h->rec_cap)
-    return fail(NMC_ERR_NOMEM, "%llu records exceed the capacity %llu; raise record_capacity",
+    return fail(NMC_ERR_NOMEM, "%llu record slots exceed the capacity %llu; raise record_capacity",
                 (unsigned long long)nrec, (unsigned long long)h->rec_cap);
   double* d_grad_rd = h->dGrad;
   double* d_inv = h->dGrad + nb;

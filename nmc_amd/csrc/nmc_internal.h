@@ -113,10 +113,15 @@ struct LayArgs {
   int records;
   int rec_nl;                       // layers stored per record (deepest stack)
   uint64_t cap;
-  unsigned long long* rec_count;
-  uint32_t* rec;                    // [cap][3 + 2 rec_nl]: stack, bin, w, L[rec_nl], K[rec_nl]
+  unsigned long long* rec_count;    // [0] slots handed out, [1] records written
+  uint32_t* rec;                    // [cap][3 + 2 rec_nl]: stack, bin, w, L[rec_nl], K[rec_nl];
+                                    // stack 0xFFFFFFFF marks an unused slot
   RunState* state;
 };
+
+// Record slots a wave reserves at once (one global atomic per kRecChunk records).
+constexpr int kRecChunk = 256;
+constexpr uint32_t kRecHole = 0xFFFFFFFFu;
 
 struct LayBackArgs {
   int n_items, nr;

@@ -66,6 +66,21 @@ class LayeredMC:
         check(lib().nmc_layered_results(self._h, rd.ctypes.data, ab.ctypes.data, tots, C.byref(nrec)))
         return rd, ab, [t.as_dict() for t in tots], nrec.value
 
+    def records(self) -> dict:
+        """The last run's path records (unspecified order): stack, bin, w,
+        L [n, rec_nl], K [n, rec_nl]."""
+        nrec = self.results()[3]
+        n_out, nl = C.c_uint64(), C.c_int()
+        check(lib().nmc_layered_records(self._h, None, 0, C.byref(n_out), C.byref(nl)))
+        rs = 3 + 2 * nl.value
+        buf = np.zeros((max(nrec, 1), rs), np.uint32)
+        check(lib().nmc_layered_records(self._h, buf.ctypes.data, nrec, C.byref(n_out),
+                                        C.byref(nl)))
+        buf = buf[:n_out.value]
+        k = nl.value
+        return {"stack": buf[:, 0], "bin": buf[:, 1], "w": buf[:, 2].view(np.float32),
+                "L": buf[:, 3:3 + k].view(np.float32), "K": buf[:, 3 + k:3 + 2 * k]}
+
     def rd_per_photon(self) -> np.ndarray:
         """Rd(r) per launched photon, [n_stacks, nr+1] (the backward's variable)."""
         rd = self.results()[0]
