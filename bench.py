@@ -88,14 +88,24 @@ def calibrate_copy(mib: int) -> int:
 def cpu_baseline(w, target_s: float) -> dict:
     from oracle import oracle as O
     cores = len(os.sched_getaffinity(0))
-    n_probe = 50_000
-    probe = O.run(w, n_probe, threads=cores, fluence=True)
-    rate = n_probe / max(probe.seconds, 1e-6)
-    n = int(min(5e7, max(1e5, rate * target_s)))
-    log(f"[bench] cpu baseline: oracle on {cores} threads, {n} photons")
+    # the oracle's per-thread tallies cost a fixed set-up and merge time, so a
+    # small probe underestimates the rate: size the sample from the probe and
+    # enlarge it once if it ran short
+    n = 200_000
     t = O.run(w, n, threads=cores, fluence=True)
+    for _ in range(2):
+        if t.seconds >= 0.5 * target_s:
+            break
+        n = int(min(2e8, n * target_s / max(t.seconds, 1e-3)))
+        log(f"[bench] cpu baseline: oracle on {cores} threads, {n} photons")
+        t = O.run(w, n, threads=cores, fluence=True)
+    n1 = max(1000, int(n / cores / 4))
+    log(f"[bench] cpu baseline: oracle on 1 thread, {n1} photons")
+    t1 = O.run(w, n1, threads=1, fluence=True)
     return {"value": n / t.seconds, "unit": "photons/s", "cores": cores, "kind": "port",
-            "sample": f"photons [0,{n}) of the same workload (oracle, OpenMP, FP64 tallies)",
+            "sample": f"photons [0,{n}) of the same workload in {t.seconds:.1f} s "
+                      "(oracle, OpenMP, FP64 tallies)",
+            "single_thread": {"value": n1 / t1.seconds, "photons": n1},
             "_tallies": t, "_n": n}
 
 

@@ -265,20 +265,22 @@ __device__ __forceinline__ float tt_cos(const TTCoef& t, float u) {
 
 // Rotate direction (ux,uy,uz) by polar cosine `cost` and azimuth 2*pi*u_psi.
 __device__ __forceinline__ void spin(float cost, float u_psi, float& ux, float& uy, float& uz) {
-  float sint = sqrtf(1.0f - cost * cost);
   float sp, cp;
   sincos2pi(u_psi, &sp, &cp);
   float x = ux, y = uy, z = uz;
   if (fabsf(z) > 1.0f - 1.0e-6f) {
+    float sint = sqrtf(1.0f - cost * cost);
     ux = sint * cp;
     uy = sint * sp;
     uz = z >= 0.0f ? cost : -cost;
   } else {
-    float tmp = sqrtf(x * x + y * y);
-    float st = sint / tmp;
+    // st = sin(theta) / sqrt(ux^2 + uy^2) as one square root; the uz term
+    // sin(theta) * sqrt(ux^2 + uy^2) is st * (ux^2 + uy^2)
+    float r2 = x * x + y * y;
+    float st = sqrtf((1.0f - cost * cost) / r2);
     ux = st * (x * z * cp - y * sp) + x * cost;
     uy = st * (y * z * cp + x * sp) + y * cost;
-    uz = -(sint * cp * tmp) + z * cost;
+    uz = -(st * cp * r2) + z * cost;
   }
 }
 

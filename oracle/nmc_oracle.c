@@ -337,20 +337,22 @@ float nmco_tt_cos(float gf, float af, float gb, float ab, float cc, float u)
  * (kernels.metal:705-730; sin(psi) is computed directly). */
 void nmco_spin(float cost, float u_psi, float* ux, float* uy, float* uz)
 {
-    float sint = sqrtf(1.0f - cost * cost);
     float sp, cp;
     nmco_sincos2pi(u_psi, &sp, &cp);
     float x = *ux, y = *uy, z = *uz;
     if (fabsf(z) > 1.0f - 1.0e-6f) {
+        float sint = sqrtf(1.0f - cost * cost);
         *ux = sint * cp;
         *uy = sint * sp;
         *uz = z >= 0.0f ? cost : -cost;
     } else {
-        float tmp = sqrtf(x * x + y * y);
-        float st = sint / tmp;
+        /* st = sin(theta) / sqrt(ux^2 + uy^2) as one square root; the uz term
+         * sin(theta) * sqrt(ux^2 + uy^2) is st * (ux^2 + uy^2) */
+        float r2 = x * x + y * y;
+        float st = sqrtf((1.0f - cost * cost) / r2);
         *ux = st * (x * z * cp - y * sp) + x * cost;
         *uy = st * (y * z * cp + x * sp) + y * cost;
-        *uz = -(sint * cp * tmp) + z * cost;
+        *uz = -(st * cp * r2) + z * cost;
     }
 }
 

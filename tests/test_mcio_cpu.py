@@ -278,3 +278,24 @@ def test_mcout_matches_reference(tmp_path, nr, nz):
                                 d[5], d[6], d[7], 0, d[8], d[9], d[10], d[11], d[12], d[13],
                                 d[14], d[15])
     assert open(mine).read() == ref
+
+
+def test_cli_rejects_bad_inputs_before_any_device_work(tmp_path, capsys):
+    """The nmc driver fails with a message and status 1 (the reference exits
+    or reads garbage) for a missing file, an unsupported kernel and a bad
+    photon count; none of these reaches the GPU."""
+    from nmc_amd import cli
+
+    assert cli.main([str(tmp_path / "missing")]) == 1
+    assert "cannot open" in capsys.readouterr().err
+    rp, tp, v = small_case()
+    rp.mckernelflag = 3                      # polarisation: readable, not built here
+    name = str(tmp_path / "pol")
+    mcio.write_mci(name, rp, tp)
+    mcio.write_tissue(name, v)
+    assert cli.main([name]) == 1
+    assert "not built" in capsys.readouterr().err
+    rp.mckernelflag = 1
+    mcio.write_mci(name, rp, tp)
+    assert cli.main([name, "--photons", "0"]) == 1
+    assert "positive" in capsys.readouterr().err
