@@ -17,7 +17,6 @@ the search is over block size / refill batch / tally placement only.
 """
 from __future__ import annotations
 
-import ctypes as C
 import hashlib
 import json
 import os

@@ -7,7 +7,6 @@ size, refill batch, queue chunk, LDS tile box) may change any result.
 """
 from __future__ import annotations
 
-import ctypes as C
 import sys
 
 import numpy as np
@@ -15,7 +14,7 @@ import pytest
 
 from nmc_amd import params as P
 from nmc_amd import workloads as W
-from nmc_amd.metal_mc import MetalMC, simulate
+from nmc_amd.metal_mc import MetalMC
 from oracle import oracle as O
 
 from parity_util import RTOL, assert_same_tallies

@@ -9,7 +9,6 @@ import sys
 import numpy as np
 import pytest
 
-from nmc_amd import params as P
 from nmc_amd import workloads as W
 from nmc_amd.layered import LayeredMC
 from nmc_amd.metal_mc import simulate

@@ -17,7 +17,6 @@ import numpy as np
 import pytest
 
 from nmc_amd import diffusion as D
-from nmc_amd import params as P
 from nmc_amd import spectral
 from nmc_amd import workloads as W
 from nmc_amd.metal_mc import simulate
