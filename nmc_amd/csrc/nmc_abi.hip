@@ -795,10 +795,18 @@ int nmc_set_seed(nmc_handle* h, uint64_t seed, uint64_t first_photon) {
 
 int nmc_set_tuning(nmc_handle* h, const nmc_tuning* t) {
   if (!h || !t) return fail(NMC_ERR_ARG, "null argument");
+  if (t->refill_min < 0 || t->refill_min > 64) return fail(NMC_ERR_ARG, "tuning.refill_min not in 1..64");
   h->tune_req = *t;
   if (!h->prepared) return ok();
   HIP_TRY(hipSetDevice(h->device));
   HIP_TRY(hipStreamSynchronize(h->stream));
+  if (h->sim == NMC_SIM_LAYERED) {
+    // the grid, and the buffers sized by it, are fixed at nmc_layered_prepare;
+    // the refill batch and the queue chunk can change between runs
+    h->tune.refill_min = t->refill_min ? t->refill_min : 16;
+    h->tune.chunk = t->chunk;
+    return ok();
+  }
   return h->sim == NMC_SIM_MCSUB ? resolve_sub_tuning(h) : resolve_xyz_tuning(h);
 }
 
@@ -1090,7 +1098,8 @@ int nmc_layered_update(nmc_handle* h, const nmc_stack* stacks, int n_stacks) {
     if (stacks[j].n_layers != h->stacks[j].n_layers) return fail(NMC_ERR_ARG, "update must keep each stack's layer count");
     total += stacks[j].n_photons;
   }
-  if (h->lcfg.records && !h->lcfg.record_capacity && total > h->rec_cap)
+  const uint64_t holes = (uint64_t)h->lay_grid * (kLayBlock / 64) * kRecChunk;
+  if (h->lcfg.records && !h->lcfg.record_capacity && total + holes > h->rec_cap)
     return fail(NMC_ERR_ARG, "budgets exceed the record capacity set at prepare");
   HIP_TRY(hipSetDevice(h->device));
   HIP_TRY(hipStreamSynchronize(h->stream));

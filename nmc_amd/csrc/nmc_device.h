@@ -249,16 +249,20 @@ __device__ __forceinline__ float pow_neg(float b, float e) {
   return exp_f(e * log_pos(b));
 }
 
+// The branch's constants are selected first and the inverse CDF evaluated
+// once, so lanes of a wave that took different branches share one
+// evaluation (the forward branch's ct, or the backward branch's negated).
 __device__ __forceinline__ float tt_cos(const TTCoef& t, float u) {
+  const bool fwd = u < t.cc;
+  float v = (fwd ? u : u - t.cc) / (fwd ? t.cc : 1.0f - t.cc);
   float ct;
-  if (u < t.cc) {
-    float v = u / t.cc;
-    if (t.isof) ct = 2.0f * v - 1.0f;
-    else ct = t.c1f - t.c2f * pow_neg(v * t.iq2f + (1.0f - v) * t.iq3f, t.ef);
+  if (fwd ? t.isof : t.isob) {
+    ct = 2.0f * v - 1.0f;
   } else {
-    float v = (u - t.cc) / (1.0f - t.cc);
-    if (t.isob) ct = 2.0f * v - 1.0f;
-    else ct = -(t.c1b - t.c2b * pow_neg(v * t.iq2b + (1.0f - v) * t.iq3b, t.eb));
+    float iq2 = fwd ? t.iq2f : t.iq2b, iq3 = fwd ? t.iq3f : t.iq3b;
+    float c1 = fwd ? t.c1f : t.c1b, c2 = fwd ? t.c2f : t.c2b;
+    ct = c1 - c2 * pow_neg(v * iq2 + (1.0f - v) * iq3, fwd ? t.ef : t.eb);
+    if (!fwd) ct = -ct;
   }
   return fminf(fmaxf(ct, -1.0f), 1.0f);
 }

@@ -81,6 +81,11 @@ class LayeredMC:
         return {"stack": buf[:, 0], "bin": buf[:, 1], "w": buf[:, 2].view(np.float32),
                 "L": buf[:, 3:3 + k].view(np.float32), "K": buf[:, 3 + k:3 + 2 * k]}
 
+    def set_tuning(self, tuning: P.Tuning) -> None:
+        """Refill batch and queue chunk for later runs (the grid is fixed at
+        prepare; pass the whole tuning to the constructor to change it)."""
+        check(lib().nmc_set_tuning(self._h, C.byref(tuning)))
+
     def rd_per_photon(self) -> np.ndarray:
         """Rd(r) per launched photon, [n_stacks, nr+1] (the backward's variable)."""
         rd = self.results()[0]
