@@ -120,14 +120,19 @@ __device__ __forceinline__ void flush_records(LPhoton& p, const LayArgs& a, int 
 
 __global__ __launch_bounds__(kLayBlock) void layered_kernel(LayArgs a) {
   extern __shared__ double smem[];
-  // [items] absorbed[MAXL] | [items] totals | [items] headers | [items+1] prefix | [rows]
+  // [items] absorbed[MAXL] | [items] totals | [items][nr+1] Rd (rd_lds only) |
+  // [items] headers | [items+1] prefix | [rows]
   const int ni = a.n_items;
+  const int rd_len = a.rd_lds ? ni * (a.nr + 1) : 0;
   double* absorbed = smem;
   nmc_totals* st = (nmc_totals*)(absorbed + ni * NMC_MAX_LAYERS);
-  LayStackDev* hdr = (LayStackDev*)(st + ni);
+  double* rd_tile = (double*)(st + ni);
+  LayStackDev* hdr = (LayStackDev*)(rd_tile + rd_len);
   uint64_t* prefix = (uint64_t*)(hdr + ni);
   LayRow* rows = (LayRow*)(prefix + ni + 1);
-  for (int i = threadIdx.x; i < ni * (NMC_MAX_LAYERS + TF_COUNT + TI_COUNT); i += kLayBlock) smem[i] = 0.0;
+  double* rd_rep = a.rd + (int64_t)(blockIdx.x % (unsigned)a.rd_reps) * ni * (a.nr + 1);
+  for (int i = threadIdx.x; i < ni * (NMC_MAX_LAYERS + TF_COUNT + TI_COUNT) + rd_len; i += kLayBlock)
+    smem[i] = 0.0;
   for (int i = threadIdx.x; i < ni; i += kLayBlock) hdr[i] = a.stacks[i];
   for (int i = threadIdx.x; i <= ni; i += kLayBlock) prefix[i] = a.prefix[i];
   for (int i = threadIdx.x; i < a.n_rows; i += kLayBlock) rows[i] = a.rows[i];
@@ -262,7 +267,13 @@ __global__ __launch_bounds__(kLayBlock) void layered_kernel(LayArgs a) {
         fold_layer(p, absorbed);
         float re = sqrtf(p.x * p.x + p.y * p.y);
         int ir = rbin(re / a.dr, a.nr);
-        unsafeAtomicAdd(&a.rd[(int64_t)p.item * (a.nr + 1) + ir], (double)p.W);
+        // the bins near r = 0 take a large share of all exits: an LDS
+        // histogram (or a per-XCD replica) keeps them from serialising the
+        // whole chip on a few global addresses
+        if (a.rd_lds)
+          unsafeAtomicAdd(&rd_tile[p.item * (a.nr + 1) + ir], (double)p.W);
+        else
+          unsafeAtomicAdd(&rd_rep[(int64_t)p.item * (a.nr + 1) + ir], (double)p.W);
         addf(T, TF_RD, (double)p.W);
         addi(T, TI_DETECTIONS, 1);
         if (a.records) {
@@ -353,6 +364,10 @@ __global__ __launch_bounds__(kLayBlock) void layered_kernel(LayArgs a) {
   __syncthreads();
   double* part = a.part + (int64_t)blockIdx.x * per;
   for (int i = threadIdx.x; i < per; i += kLayBlock) part[i] = smem[i];
+  if (a.rd_lds) {
+    double* rp = a.rd + (int64_t)blockIdx.x * rd_len;
+    for (int i = threadIdx.x; i < rd_len; i += kLayBlock) rp[i] = rd_tile[i];
+  }
 }
 
 // Gradient accumulation over the records.  Each thread keeps the running sums
@@ -416,8 +431,8 @@ __global__ __launch_bounds__(256) void layered_backward_kernel(LayBackArgs b) {
 
 }  // namespace
 
-size_t layered_lds_bytes(int n_items, int n_rows) {
-  return (size_t)n_items * (NMC_MAX_LAYERS + TF_COUNT + TI_COUNT) * sizeof(double) +
+size_t layered_lds_bytes(int n_items, int n_rows, int rd_len) {
+  return ((size_t)n_items * (NMC_MAX_LAYERS + TF_COUNT + TI_COUNT) + (size_t)rd_len) * sizeof(double) +
          (size_t)n_items * sizeof(LayStackDev) + (size_t)(n_items + 1) * sizeof(uint64_t) +
          (size_t)n_rows * sizeof(LayRow);
 }

@@ -56,7 +56,7 @@ __global__ __launch_bounds__(BLOCK) void mcsub_kernel(SubArgs a) {
   uint64_t* prefix = (uint64_t*)(med + ni);
   nmc_totals* st = (nmc_totals*)(prefix + ni + 1);
   double* tal = (double*)(st + ni);
-  double* tally = LDS ? tal : a.tally;
+  double* tally = LDS ? tal : a.tally + (int64_t)(blockIdx.x % (unsigned)a.reps) * ni * per_item;
   nmc_totals* tot = st;
   for (int i = threadIdx.x; i < ni; i += BLOCK) med[i] = a.items[i];
   for (int i = threadIdx.x; i <= ni; i += BLOCK) prefix[i] = a.prefix[i];
@@ -258,8 +258,9 @@ __global__ __launch_bounds__(BLOCK) void mcsub_kernel(SubArgs a) {
 // out[j] += sum_b part[b * stride + j] for j < len, blocks in ascending order
 // (deterministic).  Integer totals are stored as u64 bit patterns in the
 // partials and are summed as integers.
-__global__ void reduce_partials_kernel(const double* __restrict__ part, int grid, int64_t stride,
-                                       int64_t len, double* __restrict__ out, int ni, int per_item) {
+__global__ void reduce_partials_kernel(double* __restrict__ part, int grid, int64_t stride,
+                                       int64_t len, double* __restrict__ out, int ni, int per_item,
+                                       int clear) {
   int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= len) return;
   int64_t tal_len = (int64_t)ni * per_item;
@@ -278,6 +279,8 @@ __global__ void reduce_partials_kernel(const double* __restrict__ part, int grid
     for (int b = 0; b < grid; ++b) s += part[(int64_t)b * stride + j];
     out[j] += s;
   }
+  if (clear)
+    for (int b = 0; b < grid; ++b) part[(int64_t)b * stride + j] = 0.0;
 }
 
 __global__ void state_begin_kernel(RunState* st) { st->counter = 0ull; }
@@ -326,11 +329,11 @@ int mcsub_blocks_per_cu(int block, size_t lds_bytes, bool lds_tallies) {
   return e == hipSuccess ? nb : 0;
 }
 
-hipError_t launch_reduce_partials(const double* part, int grid, int64_t stride, double* out,
-                                  int ni, int per_item, hipStream_t s) {
+hipError_t launch_reduce_partials(double* part, int grid, int64_t stride, double* out,
+                                  int ni, int per_item, hipStream_t s, bool clear) {
   if (stride <= 0) return hipSuccess;
   hipLaunchKernelGGL(reduce_partials_kernel, dim3((unsigned)((stride + 255) / 256)), dim3(256), 0,
-                     s, part, grid, stride, stride, out, ni, per_item);
+                     s, part, grid, stride, stride, out, ni, per_item, clear ? 1 : 0);
   return hipGetLastError();
 }
 

@@ -178,6 +178,8 @@ struct nmc_handle {
   uint64_t* dPrefix = nullptr;
   float* dCos = nullptr;
   double* dOut = nullptr;   // [n_items * per_item tallies | n_items nmc_totals]
+  double* dRep = nullptr;   // global-mode tally replicas [sub_reps][n_items * per_item]
+  int sub_reps = 1;
   double* dPart = nullptr;
   int64_t n_ra = 0, n_rz = 0;
   bool sub_lds = false;
@@ -192,6 +194,8 @@ struct nmc_handle {
   double* dLRd = nullptr;      // [n][nr+1]
   double* dLOut = nullptr;     // [n][MAXL] absorbed | [n] totals
   double* dLPart = nullptr;
+  double* dLRdPart = nullptr;  // Rd partials [lay_rd_count][n][nr+1]
+  int lay_rd_lds = 0, lay_rd_count = 0;
   unsigned long long* dRecCount = nullptr;
   uint32_t* dRec = nullptr;
   uint64_t rec_cap = 0;
@@ -234,6 +238,7 @@ void free_buffers(nmc_handle* h) {
   free_ptr(h->dPrefix);
   free_ptr(h->dCos);
   free_ptr(h->dOut);
+  free_ptr(h->dRep);
   free_ptr(h->dPart);
   free_ptr(h->dHdr);
   free_ptr(h->dRows);
@@ -241,6 +246,7 @@ void free_buffers(nmc_handle* h) {
   free_ptr(h->dLRd);
   free_ptr(h->dLOut);
   free_ptr(h->dLPart);
+  free_ptr(h->dLRdPart);
   free_ptr(h->dRecCount);
   free_ptr(h->dRec);
   free_ptr(h->dGrad);
@@ -344,6 +350,15 @@ int resolve_sub_tuning(nmc_handle* h) {
   free_ptr(h->dPart);
   size_t stride = (size_t)h->n_items * ((h->sub_lds ? per_item(h) : 0) + TF_COUNT + TI_COUNT);
   HIP_TRY(hipMalloc(&h->dPart, (size_t)h->sub_grid * stride * sizeof(double)));
+  // global-mode tallies go to per-XCD replicas (the bins near the beam axis
+  // take a large share of all adds); zeroed here, then by each reduction
+  free_ptr(h->dRep);
+  if (!h->sub_lds) {
+    h->sub_reps = replica_count((int64_t)h->n_items * per_item(h), h->sub_grid);
+    size_t rep_n = (size_t)h->sub_reps * h->n_items * per_item(h);
+    HIP_TRY(hipMalloc(&h->dRep, rep_n * sizeof(double)));
+    HIP_TRY(hipMemsetAsync(h->dRep, 0, rep_n * sizeof(double), h->stream));
+  }
   drop_graph(h);
   return ok();
 }
@@ -390,7 +405,8 @@ int enqueue_run(nmc_handle* h, uint64_t n) {
     a.prefix = h->dPrefix;
     a.cos_tab = h->dCos;
     a.tally_rz = h->n_rz > 0;
-    a.tally = h->dOut;
+    a.tally = h->sub_lds ? h->dOut : h->dRep;
+    a.reps = h->sub_lds ? 1 : h->sub_reps;
     a.part = h->dPart;
     a.state = h->dState;
     HIP_TRY(launch_mcsub(a, h->tune.block, h->sub_grid, h->sub_lds_bytes, h->sub_lds, s));
@@ -399,6 +415,9 @@ int enqueue_run(nmc_handle* h, uint64_t n) {
       int64_t stride = (int64_t)h->n_items * (lds_per + TF_COUNT + TI_COUNT);
       double* out = h->dOut + (h->sub_lds ? 0 : (int64_t)h->n_items * per_item(h));
       HIP_TRY(launch_reduce_partials(h->dPart, h->sub_grid, stride, out, h->n_items, (int)lds_per, s));
+      if (!h->sub_lds)
+        HIP_TRY(launch_reduce_partials(h->dRep, h->sub_reps, (int64_t)h->n_items * per_item(h), h->dOut,
+                                       h->n_items, (int)per_item(h), s, true));
     }
     uint64_t adv = 0;
     for (int j = 0; j < h->n_items; ++j) adv = std::max<uint64_t>(adv, h->prefix[j + 1] - h->prefix[j]);
@@ -1060,13 +1079,20 @@ int nmc_layered_prepare(nmc_handle* h, const nmc_layered_config* cfg, const nmc_
   HIP_TRY(hipMalloc(&h->dGrad, ((size_t)n * (cfg->nr + 1) + n + 2 * (size_t)n * NMC_MAX_LAYERS) * sizeof(double)));
   HIP_TRY(hipMalloc(&h->dRecCount, 2 * sizeof(unsigned long long)));
   h->rec_cap = 0;
-  h->lay_lds = layered_lds_bytes(n, h->lay_rows);
+  h->lay_lds = layered_lds_bytes(n, h->lay_rows, 0);
   if (h->lay_lds + 1024 > h->lds_max)
     return fail(NMC_ERR_ARG, "%d stacks / %d layers need %zu B of LDS (%zu available)", n, h->lay_rows,
                 h->lay_lds, h->lds_max);
-  HIP_TRY(layered_configure(h->lay_lds));
+  // Rd(r) goes to an LDS histogram when it fits without costing occupancy,
+  // else to per-XCD global replicas
+  const int rd_len = n * (cfg->nr + 1);
+  const size_t lds_rd = layered_lds_bytes(n, h->lay_rows, rd_len);
+  const bool rd_fits = lds_rd + 1024 <= h->lds_max;
+  HIP_TRY(layered_configure(rd_fits ? lds_rd : h->lay_lds));
   int occ = layered_blocks_per_cu(h->lay_lds);
   if (occ <= 0) return fail(NMC_ERR_ARG, "layered kernel does not fit on a CU");
+  h->lay_rd_lds = rd_fits && layered_blocks_per_cu(lds_rd) >= occ;
+  if (h->lay_rd_lds) h->lay_lds = lds_rd;
   if (h->tune_req.blocks_per_cu > 0) occ = std::min(occ, h->tune_req.blocks_per_cu);
   h->lay_grid = h->num_cu * occ;
   h->tune = nmc_tuning{};
@@ -1075,6 +1101,9 @@ int nmc_layered_prepare(nmc_handle* h, const nmc_layered_config* cfg, const nmc_
   h->tune.refill_min = h->tune_req.refill_min ? h->tune_req.refill_min : 16;
   h->tune.chunk = h->tune_req.chunk;
   HIP_TRY(hipMalloc(&h->dLPart, (size_t)h->lay_grid * n * (NMC_MAX_LAYERS + TF_COUNT + TI_COUNT) * sizeof(double)));
+  h->lay_rd_count = h->lay_rd_lds ? h->lay_grid : replica_count(rd_len, h->lay_grid);
+  HIP_TRY(hipMalloc(&h->dLRdPart, (size_t)h->lay_rd_count * rd_len * sizeof(double)));
+  HIP_TRY(hipMemsetAsync(h->dLRdPart, 0, (size_t)h->lay_rd_count * rd_len * sizeof(double), h->stream));
   if (cfg->records) {
     // every photon leaves at most one record; each wave may leave the unused
     // tail of its last slot run (< kRecChunk slots) as holes
@@ -1113,7 +1142,8 @@ int nmc_layered_run(nmc_handle* h) {
   HIP_TRY(hipSetDevice(h->device));
   int n = (int)h->stacks.size();
   hipStream_t s = h->stream;
-  HIP_TRY(hipMemsetAsync(h->dLRd, 0, (size_t)n * (h->lcfg.nr + 1) * sizeof(double), s));
+  const int rd_len = n * (h->lcfg.nr + 1);
+  HIP_TRY(hipMemsetAsync(h->dLRd, 0, (size_t)rd_len * sizeof(double), s));
   HIP_TRY(hipMemsetAsync(h->dLOut, 0, (size_t)n * (NMC_MAX_LAYERS + TF_COUNT + TI_COUNT) * sizeof(double), s));
   HIP_TRY(hipMemsetAsync(h->dRecCount, 0, 2 * sizeof(unsigned long long), s));
   HIP_TRY(hipEventRecord(h->e0, s));
@@ -1133,7 +1163,9 @@ int nmc_layered_run(nmc_handle* h) {
   a.stacks = h->dHdr;
   a.rows = h->dRows;
   a.prefix = h->dLPrefix;
-  a.rd = h->dLRd;
+  a.rd = h->dLRdPart;
+  a.rd_lds = h->lay_rd_lds;
+  a.rd_reps = h->lay_rd_lds ? 1 : h->lay_rd_count;
   a.part = h->dLPart;
   a.records = h->lcfg.records != 0;
   a.rec_nl = h->rec_nl;
@@ -1144,6 +1176,8 @@ int nmc_layered_run(nmc_handle* h) {
   HIP_TRY(launch_layered(a, h->lay_grid, h->lay_lds, s));
   HIP_TRY(launch_reduce_partials(h->dLPart, h->lay_grid, (int64_t)n * (NMC_MAX_LAYERS + TF_COUNT + TI_COUNT),
                                  h->dLOut, n, NMC_MAX_LAYERS, s));
+  HIP_TRY(launch_reduce_partials(h->dLRdPart, h->lay_rd_count, rd_len, h->dLRd, n, h->lcfg.nr + 1, s,
+                                 !h->lay_rd_lds));
   HIP_TRY(hipEventRecord(h->e1, s));
   h->timed = true;
   HIP_TRY(hipStreamSynchronize(s));

@@ -82,7 +82,10 @@ struct SubArgs {
   const uint64_t* prefix;           // [n_items+1] queue index where item j starts
   const float* cos_tab;             // na-1 upper bin-edge cosines
   int tally_rz;                     // 0: skip the A(r,z) tally
-  double* tally;                    // [n_items][(nr+1)*na + (nr+1)*(nz+1)] (global mode)
+  // global mode: [reps][n_items][(nr+1)*na + (nr+1)*(nz+1)], FP64 atomics into
+  // replica blockIdx % reps, summed into the output after the launch
+  double* tally;
+  int reps;
   double* part;                     // block partials: [LDS tallies] [totals]
   RunState* state;
 };
@@ -108,7 +111,13 @@ struct LayArgs {
   const LayStackDev* stacks;        // [n_items]
   const LayRow* rows;               // [n_rows]
   const uint64_t* prefix;           // [n_items+1]
-  double* rd;                       // [n_items][nr+1] (global atomics)
+  // Rd(r) partials, reduced over their first index after the launch:
+  // rd_lds = 1: [grid][n_items][nr+1], each block's LDS histogram written out;
+  // rd_lds = 0: [rd_reps][n_items][nr+1], FP64 global atomics into replica
+  // blockIdx % rd_reps (blocks are dealt to the XCDs round-robin, so any
+  // multiple of 8 replicas keeps each replica on one XCD).
+  double* rd;
+  int rd_lds, rd_reps;
   double* part;                     // block partials of [absorbed | totals]
   int records;
   int rec_nl;                       // layers stored per record (deepest stack)
@@ -137,7 +146,18 @@ struct LayBackArgs {
 };
 
 constexpr int kLayBlock = 1024;
-size_t layered_lds_bytes(int n_items, int n_rows);
+size_t layered_lds_bytes(int n_items, int n_rows, int rd_len);
+// Global tally replicas (Rd(r) of the layered kernel, Rd/A of batched mcsub)
+// when the tallies do not fit in LDS: as many as kRepBytes allows, at least
+// kMinReps (one per XCD), at most one per block.
+constexpr int kMinReps = 8;
+constexpr size_t kRepBytes = (size_t)256 << 20;
+inline int replica_count(int64_t len, int grid) {
+  int64_t r = (int64_t)(kRepBytes / (sizeof(double) * (size_t)(len > 0 ? len : 1)));
+  if (r > grid) r = grid;
+  if (r < kMinReps) r = kMinReps;
+  return (int)r;
+}
 hipError_t layered_configure(size_t lds_bytes);
 int layered_blocks_per_cu(size_t lds_bytes);
 hipError_t launch_layered(const LayArgs& a, int grid, size_t lds_bytes, hipStream_t s);
@@ -154,8 +174,11 @@ hipError_t launch_mcsub(const SubArgs& a, int block, int grid, size_t lds_bytes,
 hipError_t mcsub_configure(int block, size_t lds_bytes, bool lds_tallies);
 int mcsub_blocks_per_cu(int block, size_t lds_bytes, bool lds_tallies);
 size_t mcsub_lds_bytes(int n_items, int64_t per_item_in_lds);
-hipError_t launch_reduce_partials(const double* part, int grid, int64_t stride, double* out,
-                                  int ni, int per_item, hipStream_t s);
+// out[j] += sum over b < grid of part[b * stride + j] (integer sums for the
+// nmc_totals counters); with clear, each part[] element is zeroed once read,
+// so replicas start the next run at zero without a memset.
+hipError_t launch_reduce_partials(double* part, int grid, int64_t stride, double* out,
+                                  int ni, int per_item, hipStream_t s, bool clear = false);
 hipError_t launch_state_begin(RunState* st, hipStream_t s);
 hipError_t launch_state_advance(RunState* st, uint64_t adv, hipStream_t s);
 
