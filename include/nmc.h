@@ -211,7 +211,11 @@ typedef struct nmc_tuning {
                                per-item tallies fit), 1 LDS, 2 global atomics;
                                mcxyz: LDS tile element, 0 FP64, 1 FP32 (twice
                                the voxels; sums within a workgroup in FP32,
-                               across workgroups in FP64)                      */
+                               across workgroups in FP64);
+                               layered: Rd(r) placement, read at
+                               nmc_layered_prepare, 0 or 1 LDS histogram when
+                               it fits without lowering occupancy, 2 global
+                               replicas (effective value reported as 1 / 2)   */
     int32_t reserved[5];
 } nmc_tuning;
 

@@ -1083,15 +1083,17 @@ int nmc_layered_prepare(nmc_handle* h, const nmc_layered_config* cfg, const nmc_
   if (h->lay_lds + 1024 > h->lds_max)
     return fail(NMC_ERR_ARG, "%d stacks / %d layers need %zu B of LDS (%zu available)", n, h->lay_rows,
                 h->lay_lds, h->lds_max);
-  // Rd(r) goes to an LDS histogram when it fits without costing occupancy,
-  // else to per-XCD global replicas
+  // Rd(r) goes to an LDS histogram when it fits without costing occupancy
+  // (tuning.mode 0 or 1), else, or with tuning.mode 2, to global replicas
+  if (h->tune_req.mode < 0 || h->tune_req.mode > 2)
+    return fail(NMC_ERR_ARG, "tuning.mode (Rd placement) not in 0..2");
   const int rd_len = n * (cfg->nr + 1);
   const size_t lds_rd = layered_lds_bytes(n, h->lay_rows, rd_len);
   const bool rd_fits = lds_rd + 1024 <= h->lds_max;
   HIP_TRY(layered_configure(rd_fits ? lds_rd : h->lay_lds));
   int occ = layered_blocks_per_cu(h->lay_lds);
   if (occ <= 0) return fail(NMC_ERR_ARG, "layered kernel does not fit on a CU");
-  h->lay_rd_lds = rd_fits && layered_blocks_per_cu(lds_rd) >= occ;
+  h->lay_rd_lds = h->tune_req.mode != 2 && rd_fits && layered_blocks_per_cu(lds_rd) >= occ;
   if (h->lay_rd_lds) h->lay_lds = lds_rd;
   if (h->tune_req.blocks_per_cu > 0) occ = std::min(occ, h->tune_req.blocks_per_cu);
   h->lay_grid = h->num_cu * occ;
@@ -1100,6 +1102,7 @@ int nmc_layered_prepare(nmc_handle* h, const nmc_layered_config* cfg, const nmc_
   h->tune.blocks_per_cu = occ;
   h->tune.refill_min = h->tune_req.refill_min ? h->tune_req.refill_min : 16;
   h->tune.chunk = h->tune_req.chunk;
+  h->tune.mode = h->lay_rd_lds ? 1 : 2;
   HIP_TRY(hipMalloc(&h->dLPart, (size_t)h->lay_grid * n * (NMC_MAX_LAYERS + TF_COUNT + TI_COUNT) * sizeof(double)));
   h->lay_rd_count = h->lay_rd_lds ? h->lay_grid : replica_count(rd_len, h->lay_grid);
   HIP_TRY(hipMalloc(&h->dLRdPart, (size_t)h->lay_rd_count * rd_len * sizeof(double)));

@@ -81,6 +81,12 @@ class LayeredMC:
         return {"stack": buf[:, 0], "bin": buf[:, 1], "w": buf[:, 2].view(np.float32),
                 "L": buf[:, 3:3 + k].view(np.float32), "K": buf[:, 3 + k:3 + 2 * k]}
 
+    def get_tuning(self) -> P.Tuning:
+        """Effective launch values; `mode` 1 = Rd(r) in LDS, 2 = global replicas."""
+        t = P.Tuning()
+        check(lib().nmc_get_tuning(self._h, C.byref(t)))
+        return t
+
     def set_tuning(self, tuning: P.Tuning) -> None:
         """Refill batch and queue chunk for later runs (the grid is fixed at
         prepare; pass the whole tuning to the constructor to change it)."""

@@ -74,11 +74,17 @@ def test_batch_uneven_budgets_all_consumed(gpu):
         assert abs(bal - 1.0) < 1e-5, j
 
 
-def test_plain_mcsub_graph_replay(gpu):
+@pytest.mark.parametrize("placement", [0, 2])
+def test_plain_mcsub_graph_replay(gpu, placement):
+    """Replays equal consecutive photons, with the tallies in LDS (0 picks it
+    here) and in the global replicas (2), whose reduction must leave them
+    cleared for the next replay."""
     w = W.mcsub_semi_infinite()
-    _say("[graph] mcsub 3 replays x 20000")
+    _say(f"[graph] mcsub 3 replays x 20000, placement {placement}")
     with MetalMC(w.sim) as mc:
         mc.prepare_data(w.rp, w.tp, None, w.ext)
+        mc.set_tuning(P.Tuning.make(mode=placement))
+        assert mc.get_tuning().mode == (1 if placement == 0 else 2)
         mc.set_graph_mode(True)
         for _ in range(3):
             mc.send_compute_command(20000)

@@ -48,18 +48,24 @@ def test_multilayer_matches_oracle_and_gradient(gpu):
     assert gmua[0, 0] != 0.0 and gmus[0, 0] != 0.0
 
 
-def test_batched_stacks_match_oracle(gpu):
+@pytest.mark.parametrize("rd_mode", [0, 2])
+def test_batched_stacks_match_oracle(gpu, rd_mode):
+    """Rd(r) in the LDS histogram (mode 0 picks it here) and in the global
+    replicas (mode 2) give the oracle's tallies and gradients."""
     cfg, stacks = W.inverse_fit_instances(n_inst=24, budget_scale=0.002)
     stacks[3].layer[1].phase = 0
     stacks[3].layer[1].g = 0.0
     stacks[5].n_above = stacks[5].layer[0].n  # matched top
-    _say(f"[layered] 24 stacks, {sum(s.n_photons for s in stacks)} photons")
+    _say(f"[layered] 24 stacks, {sum(s.n_photons for s in stacks)} photons, Rd mode {rd_mode}")
     ref = O.run_layered(cfg, stacks, threads=8)
-    with LayeredMC(cfg, stacks) as mc:
+    with LayeredMC(cfg, stacks, tuning=P.Tuning.make(mode=rd_mode)) as mc:
+        assert mc.get_tuning().mode == (1 if rd_mode == 0 else 2)
         mc.run()
         res = mc.results()
         g = np.ones((24, cfg.nr + 1))
         gmua, gmus = mc.backward(g)
+        mc.run()                         # a second run starts from cleared tallies
+        np.testing.assert_allclose(mc.results()[0], res[0], rtol=1e-12, atol=1e-15)
     _compare(res, ref, 24)
     rmua, rmus = O.layered_backward(ref, g)
     np.testing.assert_allclose(gmua, rmua, rtol=1e-9, atol=1e-15)
