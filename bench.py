@@ -8,7 +8,11 @@ recorded by the library on the stream the kernel runs on.  Prints one JSON
 line (see the keys below); the CPU baseline is the oracle run on this host's
 cores over a bounded sample of the same workload.
 
-Every timed step (and the warm-up) launches mcxyz_kernel with the default
+The timed steps use the FP32 LDS fluence tile (sums within a workgroup in
+FP32, across workgroups and outside the tile in FP64; the reference adds every
+deposit into an FP32 global array) on the box the visit-count pilot finds;
+`--tile fp64` times the all-FP64 tile instead, and the `tile` leg reports the
+other element.  Every timed step (and the warm-up) launches mcxyz_kernel with
 1024-lane blocks; the other launches of this run (visit-count pilot, Rd
 parity run) use 512-lane blocks, so a kernel-trace profile of this command
 reports the timed kernel under its own name.
@@ -38,12 +42,13 @@ OPS_SAME, OPS_CROSS, OPS_INT = 44, 97, 145
 # gfx950: 256 CUs x 64 FP32 lanes x 2.4 GHz, non-packed lane-ops/s (T).
 VALU_PEAK_T = 256 * 64 * 2.4e9 / 1e12
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per photon of the timed kernel on this workload, from the
-# rocprofv3 FETCH_SIZE and WRITE_SIZE passes over one 1e8-photon launch,
-# scaled by the copy calibration (FETCH x2.0, WRITE x1.0):
-# (18 958 541 KiB x 2 + 99 746 748 KiB) x 1024 B / 1e8 photons
-# (profiles/mcxyz_skinvessel_200_pmc_traffic.md).
-TRAFFIC_BYTES_PER_PHOTON = 1409.7
+# HBM bytes per photon of the timed kernel on this workload, by tile element
+# (tuning.mode 0 FP64, 1 FP32), from the rocprofv3 FETCH_SIZE and WRITE_SIZE
+# passes over one 1e8-photon launch, scaled by the copy calibration (FETCH
+# x2.0, WRITE x1.0): FP64 (18 955 068 KiB x 2 + 98 872 849 KiB) x 1024 B / 1e8,
+# FP32 (18 945 313 KiB x 2 + 92 328 816 KiB) x 1024 B / 1e8, each on the
+# visit-pilot box (profiles/mcxyz_skinvessel_200_pmc_traffic.md).
+TRAFFIC_BYTES_PER_PHOTON = {0: 1400.7, 1: 1333.4}
 
 
 def log(msg: str) -> None:
@@ -109,17 +114,27 @@ def cpu_baseline(w, target_s: float) -> dict:
             "_tallies": t, "_n": n}
 
 
-def tile32_leg(mc, hist, n: int, steps: int = 2) -> dict:
-    """The same workload with the FP32 LDS tile (twice the voxels; sums within a
-    workgroup in FP32, across workgroups in FP64), box from the visit pilot."""
+def pilot_tuning(mc, hist, mode: int):
+    """Tuning with tile element `mode` (0 FP64, 1 FP32) and the box of that
+    element's LDS budget holding the most pilot visits."""
     from nmc_amd import params as P
-    from nmc_amd.autotune import best_boxes, box_coverage
+    from nmc_amd.autotune import best_boxes
 
-    mc.set_tuning(P.Tuning.make(mode=1))
+    mc.set_tuning(P.Tuning.make(mode=mode))
     budget = int(np.prod(mc.get_tuning().tile[3:6]))
     box = best_boxes(hist, budget, 1)[0][1]
-    mc.set_tuning(P.Tuning.make(mode=1, tile=box))
-    log(f"[bench] fp32 tile {box}: {steps} x {n} photons")
+    return P.Tuning.make(mode=mode, tile=box)
+
+
+def tile_leg(mc, hist, n: int, mode: int, steps: int = 2) -> dict:
+    """The same workload with the other LDS tile element (`mode` 0 FP64, 1
+    FP32), box from the visit pilot."""
+    from nmc_amd.autotune import box_coverage
+
+    t = pilot_tuning(mc, hist, mode)
+    mc.set_tuning(t)
+    box = tuple(t.tile)
+    log(f"[bench] {'fp32' if mode else 'fp64'} tile {box}: {steps} x {n} photons")
     mc.reset_tallies()
     mc.set_num_runs_seed(1, 0)
     ms = []
@@ -127,11 +142,13 @@ def tile32_leg(mc, hist, n: int, steps: int = 2) -> dict:
         mc.send_compute_command(n)
         ms.append(mc.last_kernel_ms())
     avg = sum(ms) / len(ms)
-    return {"kernel": "mcxyz_kernel<false, 1024, float>", "tile": list(box),
+    elem = "float" if mode else "double"
+    return {"kernel": f"mcxyz_kernel<false, 1024, {elem}>", "tile": list(box),
             "lds_tile_fraction": box_coverage(hist, box), "kernel_ms": avg,
-            "photons_per_s": n / (avg * 1e-3), "fluence_parity": "integer tallies bit-exact; "
-            "fluence within 2e-5 relative of the oracle (tests/test_gpu_parity.py)"}
-
+            "photons_per_s": n / (avg * 1e-3),
+            "fluence_parity": ("integer tallies bit-exact; fluence within 2e-5 relative of the "
+                               "oracle (tests/test_gpu_parity.py)") if mode else
+                              "every tally within 1e-9 relative of the oracle"}
 
 def sweep_leg(budget_scale: float) -> dict:
     """128-wavelength sweep: one batched launch vs 128 plain launches (each
@@ -211,9 +228,11 @@ def main() -> int:
     ap.add_argument("--photons", type=float, default=1e8, help="photons per step")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--legs", default="tile32,sweep,grad",
-                    help="extra measurements: tile32 (FP32 LDS tile), sweep (batched vs "
-                         "plain), grad (pMC cost)")
+    ap.add_argument("--tile", choices=("fp32", "fp64"), default="fp32",
+                    help="LDS fluence tile element of the timed steps")
+    ap.add_argument("--legs", default="tile,sweep,grad",
+                    help="extra measurements: tile (the other LDS tile element), sweep "
+                         "(batched vs plain), grad (pMC cost)")
     ap.add_argument("--sweep-scale", type=float, default=1.0, help="scale of the sweep budgets")
     ap.add_argument("--grad-photons", type=float, default=1e7)
     ap.add_argument("--autotune", action="store_true",
@@ -247,10 +266,12 @@ def main() -> int:
     ceiling = atomic_ceiling(w.rp.Nx * w.rp.Ny * w.rp.Nz)
     log("[bench] visit-count pilot (512-lane blocks)")
     hist = visit_histogram(w)
+    mode = 1 if args.tile == "fp32" else 0
     mc = MetalMC(w.sim, 0)
     mc.prepare_data(w.rp, w.tp, w.v, w.ext)
-    if tuning is not None:
-        mc.set_tuning(tuning)
+    if tuning is None:
+        tuning = pilot_tuning(mc, hist, mode)
+    mc.set_tuning(tuning)
     eff = mc.get_tuning()
     coverage = box_coverage(hist, eff.tile)
     mc.set_num_runs_seed(1, 0)
@@ -287,7 +308,8 @@ def main() -> int:
 
     roof = {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_T,
             "unit": "Tlane-op/s", "frac": achieved / VALU_PEAK_T, "traffic": None,
-            "kernel": "mcxyz_kernel<false, 1024, double>", "kernel_ms": avg_ms,
+            "kernel": f"mcxyz_kernel<false, {eff.block}, {'float' if eff.mode else 'double'}>",
+            "kernel_ms": avg_ms,
             "ops_per_photon": ops / tot["photons"],
             "events_per_photon": {"same": tot["substeps_same"] / tot["photons"],
                                   "cross": tot["substeps_cross"] / tot["photons"],
@@ -300,8 +322,8 @@ def main() -> int:
                        "photons_per_s_at_ceiling": ceiling["per_s"] * tot["photons"]
                        / global_atomics},
             "tuning": eff.as_dict()}
-    if TRAFFIC_BYTES_PER_PHOTON is not None:
-        roof["traffic"] = TRAFFIC_BYTES_PER_PHOTON * per_step
+    if TRAFFIC_BYTES_PER_PHOTON.get(eff.mode) is not None:
+        roof["traffic"] = TRAFFIC_BYTES_PER_PHOTON[eff.mode] * per_step
         roof["hbm_GB_per_s"] = roof["traffic"] / (avg_ms * 1e-3) / 1e9
         roof["hbm_frac"] = roof["hbm_GB_per_s"] / HBM_PEAK_GBS
     out = {
@@ -311,7 +333,10 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": "mcxyz skin-vessel 200^3 uint8 (0.005 cm voxels; air/epidermis/"
                    "dermis/blood cylinder), flat collimated beam r=0.06 cm, n=1, 1e8 photons/step",
-                   "photons_per_step": n, "autotuned": tuning is not None},
+                   "photons_per_step": n, "autotuned": args.autotune,
+                   "lds_tile": ("FP32 sums within a workgroup, FP64 across workgroups and "
+                                "outside the tile" if eff.mode else "FP64"),
+                   "tile_box": list(eff.tile)},
         "roofline": roof,
         "rd": rd,
         "energy_balance": (tot["rd"] + tot["rd_outside"] + tot["absorbed"] + tot["tt"]
@@ -324,7 +349,7 @@ def main() -> int:
         m = cb.pop("_n")
         # the same photon ids on the GPU (512-lane blocks: a separate kernel
         # name in a profile; the launch shape never changes results)
-        mc.set_tuning(P.Tuning.make(block=512, tile=tuple(eff.tile)))
+        mc.set_tuning(P.Tuning.make(block=512, tile=tuple(eff.tile), mode=eff.mode))
         mc.reset_tallies()
         mc.set_num_runs_seed(1, 0)
         mc.send_compute_command(m)
@@ -335,8 +360,8 @@ def main() -> int:
                             "rel_diff": abs(g["rd"] - r["rd"]) / r["rd"],
                             "int_tallies_equal": all(g[k] == r[k] for k in P.TOTAL_INT_FIELDS)}
     legs = [s for s in args.legs.split(",") if s]
-    if "tile32" in legs:
-        out["fp32_tile"] = tile32_leg(mc, hist, n)
+    if "tile" in legs:
+        out["fp32_tile" if eff.mode == 0 else "fp64_tile"] = tile_leg(mc, hist, n, 1 - eff.mode)
     mc.close()
     if "sweep" in legs:
         out["batched_sweep"] = sweep_leg(args.sweep_scale)

@@ -296,20 +296,20 @@ size_t mcsub_lds_bytes(int n_items, int64_t per_item_in_lds) {
 namespace {
 template <bool L>
 const void* sub_fn(int block) {
-  return block == 512 ? (const void*)mcsub_kernel<L, 512> : (const void*)mcsub_kernel<L, 1024>;
+  switch (block) {
+    case 256: return (const void*)mcsub_kernel<L, 256>;
+    case 512: return (const void*)mcsub_kernel<L, 512>;
+    default: return (const void*)mcsub_kernel<L, 1024>;
+  }
 }
 }  // namespace
 
 hipError_t launch_mcsub(const SubArgs& a, int block, int grid, size_t lds_bytes, bool lds_tallies,
                         hipStream_t s) {
-  if (lds_tallies) {
-    if (block == 512) hipLaunchKernelGGL((mcsub_kernel<true, 512>), dim3(grid), dim3(512), lds_bytes, s, a);
-    else hipLaunchKernelGGL((mcsub_kernel<true, 1024>), dim3(grid), dim3(1024), lds_bytes, s, a);
-  } else {
-    if (block == 512) hipLaunchKernelGGL((mcsub_kernel<false, 512>), dim3(grid), dim3(512), lds_bytes, s, a);
-    else hipLaunchKernelGGL((mcsub_kernel<false, 1024>), dim3(grid), dim3(1024), lds_bytes, s, a);
-  }
-  return hipGetLastError();
+  const void* f = lds_tallies ? sub_fn<true>(block) : sub_fn<false>(block);
+  SubArgs arg = a;
+  void* args[] = {&arg};
+  return hipLaunchKernel(f, dim3(grid), dim3(block), args, lds_bytes, s);
 }
 
 hipError_t mcsub_configure(int block, size_t lds_bytes, bool lds_tallies) {
@@ -319,13 +319,8 @@ hipError_t mcsub_configure(int block, size_t lds_bytes, bool lds_tallies) {
 
 int mcsub_blocks_per_cu(int block, size_t lds_bytes, bool lds_tallies) {
   int nb = 0;
-  hipError_t e;
-  if (lds_tallies)
-    e = block == 512 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, mcsub_kernel<true, 512>, 512, lds_bytes)
-                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, mcsub_kernel<true, 1024>, 1024, lds_bytes);
-  else
-    e = block == 512 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, mcsub_kernel<false, 512>, 512, lds_bytes)
-                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, mcsub_kernel<false, 1024>, 1024, lds_bytes);
+  const void* f = lds_tallies ? sub_fn<true>(block) : sub_fn<false>(block);
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, block, lds_bytes);
   return e == hipSuccess ? nb : 0;
 }
 

@@ -336,7 +336,8 @@ int resolve_sub_tuning(nmc_handle* h) {
   if (h->sub_lds_bytes + 512 > h->lds_max)
     return fail(NMC_ERR_ARG, "%d work items need %zu B of LDS", h->n_items, h->sub_lds_bytes);
   t.block = r.block ? r.block : 1024;
-  if (t.block != 512 && t.block != 1024) return fail(NMC_ERR_ARG, "mcsub block must be 512 or 1024");
+  if (t.block != 256 && t.block != 512 && t.block != 1024)
+    return fail(NMC_ERR_ARG, "mcsub block must be 256, 512 or 1024");
   t.refill_min = r.refill_min ? r.refill_min : 1;  // mcsub's launch is cheap: refill at once
   if (t.refill_min < 1 || t.refill_min > 64) return fail(NMC_ERR_ARG, "tuning.refill_min not in 1..64");
   t.chunk = r.chunk;
