@@ -75,57 +75,104 @@ __device__ __forceinline__ void tally_escape(nmc_totals* st, const XyzArgs& a, i
   }
 }
 
-// LAUNCH (kernels.metal:366-486).  Leaves p.alive false if the photon starts
-// outside a grid whose faces absorb it.
-__device__ __forceinline__ void launch(Photon& p, const XyzArgs& a, const XyzTissue* T,
-                                      nmc_totals* st) {
-  float x, y, z, ux, uy, uz, W = 1.0f;
-  if (a.launchflag == 1) {
-    x = a.xs; y = a.ys; z = a.zs;
-    ux = a.ux0; uy = a.uy0; uz = a.uz0;
+// Start position, direction and weight of photon `pid` for the run's source
+// (kernels.metal:366-463), and the specular loss of a focused-Gaussian beam
+// entering the tissue (0 otherwise).  Not inlined: refills are rare, and the
+// source parameters then stay out of the transport loop's registers.
+struct SourceState {
+  float x, y, z, ux, uy, uz, W, spec;
+};
+
+__device__ __noinline__ SourceState source_photon(uint64_t pid, uint32_t key0, uint32_t key1,
+                                                  const XyzAux* __restrict__ aux) {
+  const XyzAux& s = *aux;
+  float x, y, z, ux, uy, uz, W = 1.0f, spec = 0.0f;
+  if (s.launchflag == 1) {
+    x = s.xs; y = s.ys; z = s.zs;
+    ux = s.ux0; uy = s.uy0; uz = s.uz0;
   } else {
-    U4 b = philox(p.pid, 0u, 0u, a.key0, a.key1);
-    if (a.mcflag == 0 || a.mcflag == 1) {
+    U4 b = philox(pid, 0u, 0u, key0, key1);
+    if (s.mcflag == 0 || s.mcflag == 1) {
       float r;
-      if (a.mcflag == 0) r = a.radius * sqrtf(u01nz(b.x));
-      else r = a.radius * sqrtf(-log_pos(u01nz(b.x)));
+      if (s.mcflag == 0) r = s.radius * sqrtf(u01nz(b.x));
+      else r = s.radius * sqrtf(-log_pos(u01nz(b.x)));
       float sp, cp;
       sincos2pi(u01nz(b.y), &sp, &cp);
-      x = a.xs + r * cp;
-      y = a.ys + r * sp;
-      z = a.zs;
-      float rf = a.waist * sqrtf(u01nz(b.z));
+      x = s.xs + r * cp;
+      y = s.ys + r * sp;
+      z = s.zs;
+      float rf = s.waist * sqrtf(u01nz(b.z));
       sincos2pi(u01nz(b.w), &sp, &cp);
       float xf = rf * cp, yf = rf * sp;
       float ddx = x - xf, ddy = y - yf;
-      float tmp = sqrtf(ddx * ddx + ddy * ddy + a.zfocus * a.zfocus);
+      float tmp = sqrtf(ddx * ddx + ddy * ddy + s.zfocus * s.zfocus);
       ux = -ddx / tmp;
       uy = -ddy / tmp;
       uz = sqrtf(1.0f - ux * ux - uy * uy);
-      if (a.mcflag == 1) {
+      if (s.mcflag == 1) {
         float ca2;
-        float rsp = fresnel(a.n2, a.n1, fabsf(uz), &ca2);
-        lds_addf(st, TF_SPECULAR, (double)(W * rsp));
+        float rsp = fresnel(s.n2, s.n1, fabsf(uz), &ca2);
+        spec = W * rsp;
         W = W * (1.0f - rsp);
-        snell(a.n2, a.n1, &ux, &uy, &uz);
+        snell(s.n2, s.n1, &ux, &uy, &uz);
       }
-    } else if (a.mcflag == 2) {
+    } else if (s.mcflag == 2) {
       float ct = 1.0f - 2.0f * u01(b.x);
       float stt = sqrtf(1.0f - ct * ct);
       float sp, cp;
       sincos2pi(u01(b.y), &sp, &cp);
-      x = a.xs; y = a.ys; z = a.zs;
+      x = s.xs; y = s.ys; z = s.zs;
       ux = stt * cp; uy = stt * sp; uz = ct;
-    } else if (a.mcflag == 3) {
-      x = a.radius * (u01nz(b.x) * 2.0f - 1.0f);
-      y = a.radius * (u01nz(b.y) * 2.0f - 1.0f);
-      z = a.zs;
+    } else if (s.mcflag == 3) {
+      x = s.radius * (u01nz(b.x) * 2.0f - 1.0f);
+      y = s.radius * (u01nz(b.y) * 2.0f - 1.0f);
+      z = s.zs;
       ux = 0.0f; uy = 0.0f; uz = 1.0f;
     } else {
-      x = 0.0f; y = 0.0f; z = a.zs;
+      x = 0.0f; y = 0.0f; z = s.zs;
       ux = 0.0f; uy = 0.0f; uz = 1.0f;
     }
   }
+  return SourceState{x, y, z, ux, uy, uz, W, spec};
+}
+
+// A photon reaching the detector plane z = zsurf (kernels.metal:556-630):
+// the Fresnel reflectance `rsp` of the plane and, for the transmitted part
+// `wt`, whether it is detected (where = 1: inside the detector annulus and
+// numerical aperture; its weight and count go to Ryx/DetPhot2D here) or
+// leaves outside it (where = 2); where = 0 when nothing is transmitted.
+struct DetectorHit {
+  float rsp, wt;
+  int where;
+};
+
+__device__ __noinline__ DetectorHit detector_plane(float x, float y, float uz, float W, int ix,
+                                                   int iy, const XyzAux* __restrict__ aux) {
+  const XyzAux& d = *aux;
+  float uzt;
+  DetectorHit h{fresnel(d.n1, d.n2, fabsf(uz), &uzt), 0.0f, 0};
+  if (h.rsp < 1.0f) {
+    h.wt = W * (1.0f - h.rsp);
+    float ro = sqrtf(x * x + y * y);
+    if (ro >= d.rmin && ro <= d.rmax && uzt >= d.cos_na) {
+      int64_t k = (int64_t)clampi(ix, d.Nx) * d.Ny + clampi(iy, d.Ny);
+      unsafeAtomicAdd(&d.ryx[k], (double)h.wt);
+      atomicAdd(&d.det2d[k], 1u);
+      h.where = 1;
+    } else {
+      h.where = 2;
+    }
+  }
+  return h;
+}
+
+// LAUNCH (kernels.metal:366-486).  Leaves p.alive false if the photon starts
+// outside a grid whose faces absorb it.
+__device__ __forceinline__ void launch(Photon& p, const XyzArgs& a, const XyzTissue* T,
+                                      nmc_totals* st) {
+  const SourceState o = source_photon(p.pid, a.key0, a.key1, a.aux);
+  if (o.spec != 0.0f) lds_addf(st, TF_SPECULAR, (double)o.spec);
+  const float x = o.x, y = o.y, z = o.z, ux = o.ux, uy = o.uy, uz = o.uz, W = o.W;
   p.x = x; p.y = y; p.z = z; p.ux = ux; p.uy = uy; p.uz = uz; p.W = W;
   p.ix = (int)floorf(a.hx + x / a.dx);
   p.iy = (int)floorf(a.hy + y / a.dy);
@@ -281,24 +328,16 @@ __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
         // detector plane (kernels.metal:556-630)
         p.z = a.zsurf;
         lds_addi(&st, TI_SURFACE_HITS, 1);
-        float uzt;
-        float rsp = fresnel(a.n1, a.n2, fabsf(p.uz), &uzt);
-        if (rsp < 1.0f) {
-          float wt = p.W * (1.0f - rsp);
-          float ro = sqrtf(p.x * p.x + p.y * p.y);
-          if (ro >= a.rmin && ro <= a.rmax && uzt >= a.cos_na) {
-            int64_t k = (int64_t)clampi(p.ix, a.Nx) * a.Ny + clampi(p.iy, a.Ny);
-            unsafeAtomicAdd(&a.ryx[k], (double)wt);
-            atomicAdd(&a.det2d[k], 1u);
-            lds_addf(&st, TF_RD, (double)wt);
-            lds_addi(&st, TI_DETECTIONS, 1);
-          } else {
-            lds_addf(&st, TF_RD_OUTSIDE, (double)wt);
-          }
+        const DetectorHit d = detector_plane(p.x, p.y, p.uz, p.W, p.ix, p.iy, a.aux);
+        if (d.where == 1) {
+          lds_addf(&st, TF_RD, (double)d.wt);
+          lds_addi(&st, TI_DETECTIONS, 1);
+        } else if (d.where == 2) {
+          lds_addf(&st, TF_RD_OUTSIDE, (double)d.wt);
         }
         p.uz = -p.uz;
         p.ruz = -p.ruz;
-        p.W = p.W * rsp;
+        p.W = p.W * d.rsp;
         if (!(p.W > 0.0f)) alive = false;
       } else {
         // voxel face (kernels.metal:641-679)

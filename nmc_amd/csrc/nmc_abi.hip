@@ -163,6 +163,7 @@ struct nmc_handle {
   // mcxyz
   uint8_t* dV = nullptr;
   XyzTissue* dTis = nullptr;
+  XyzAux* dAux = nullptr;
   double* dF = nullptr;
   double* dTilePart = nullptr;
   double* dRyx = nullptr;
@@ -230,6 +231,7 @@ void free_buffers(nmc_handle* h) {
   drop_graph(h);
   free_ptr(h->dV);
   free_ptr(h->dTis);
+  free_ptr(h->dAux);
   free_ptr(h->dF);
   free_ptr(h->dTilePart);
   free_ptr(h->dRyx);
@@ -437,28 +439,8 @@ int enqueue_run(nmc_handle* h, uint64_t n) {
     a.hy = 0.5f * (float)rp.Ny;
     a.bmode = rp.boundaryflag;
     a.det_state = rp.det_state;
-    a.mcflag = rp.mcflag;
-    a.launchflag = rp.launchflag;
     a.zsurf = rp.zsurf;
-    a.n1 = rp.n1;
-    a.n2 = rp.n2;
-    if (rp.xd == 0.0f) {
-      a.rmin = 0.0f;
-      a.rmax = rp.det_radius;
-    } else {
-      a.rmin = rp.xd - rp.det_radius;
-      a.rmax = a.rmin + 2.0f * rp.det_radius;
-    }
-    a.cos_na = rp.na >= 90.0f ? -1.0f : (float)std::cos((double)rp.na * 3.14159265358979323846 / 180.0);
-    a.xs = rp.xs;
-    a.ys = rp.ys;
-    a.zs = rp.zs;
-    a.ux0 = rp.ux0;
-    a.uy0 = rp.uy0;
-    a.uz0 = rp.uz0;
-    a.radius = rp.radius;
-    a.waist = rp.waist;
-    a.zfocus = rp.zfocus;
+    a.aux = h->dAux;
     a.thr = e.threshold;
     a.chance = e.chance;
     a.max_events = e.max_events;
@@ -478,8 +460,6 @@ int enqueue_run(nmc_handle* h, uint64_t n) {
     a.tissue = h->dTis;
     a.F = h->dF;
     a.tile_part = h->dTilePart;
-    a.ryx = h->dRyx;
-    a.det2d = h->dDet;
     a.tot = h->dTot;
     a.state = h->dState;
     bool tt = h->sim == NMC_SIM_MCXYZ_TT;
@@ -751,6 +731,34 @@ int nmc_prepare(nmc_handle* h, const RunParams* rp, const TissueParams* tp, cons
   }
   HIP_TRY(hipMalloc(&h->dTis, NMC_NTISS * sizeof(XyzTissue)));
   HIP_TRY(hipMemcpy(h->dTis, tis.data(), NMC_NTISS * sizeof(XyzTissue), hipMemcpyHostToDevice));
+  XyzAux aux{};
+  aux.mcflag = rp->mcflag;
+  aux.launchflag = rp->launchflag;
+  aux.xs = rp->xs;
+  aux.ys = rp->ys;
+  aux.zs = rp->zs;
+  aux.ux0 = rp->ux0;
+  aux.uy0 = rp->uy0;
+  aux.uz0 = rp->uz0;
+  aux.radius = rp->radius;
+  aux.waist = rp->waist;
+  aux.zfocus = rp->zfocus;
+  aux.n1 = rp->n1;
+  aux.n2 = rp->n2;
+  if (rp->xd == 0.0f) {
+    aux.rmin = 0.0f;
+    aux.rmax = rp->det_radius;
+  } else {
+    aux.rmin = rp->xd - rp->det_radius;
+    aux.rmax = aux.rmin + 2.0f * rp->det_radius;
+  }
+  aux.cos_na = rp->na >= 90.0f ? -1.0f : (float)std::cos((double)rp->na * 3.14159265358979323846 / 180.0);
+  aux.Nx = rp->Nx;
+  aux.Ny = rp->Ny;
+  aux.ryx = h->dRyx;
+  aux.det2d = h->dDet;
+  HIP_TRY(hipMalloc(&h->dAux, sizeof(XyzAux)));
+  HIP_TRY(hipMemcpy(h->dAux, &aux, sizeof(XyzAux), hipMemcpyHostToDevice));
   NMC_TRY(resolve_xyz_tuning(h));
   h->prepared = true;
   NMC_TRY(nmc_reset_tallies(h));

@@ -36,12 +36,27 @@ struct RunState {
   unsigned long long first;         // photon id of queue index 0 (per item)
 };
 
+// Parameters of the rare paths of an mcxyz run: the source
+// (kernels.metal:366-463 launch modes) and the detector plane
+// (kernels.metal:556-630).  They live in device memory, not among the kernel
+// arguments, and are read by non-inlined functions, so the transport loop
+// does not hold them in scalar registers (with them there, the kernel spilled
+// 75 SGPRs into VGPR lanes and reloaded them inside the loop).
+struct XyzAux {
+  int mcflag, launchflag;
+  float xs, ys, zs, ux0, uy0, uz0, radius, waist, zfocus;
+  float n1, n2, rmin, rmax, cos_na;
+  int Nx, Ny;
+  double* ryx;                      // [Nx][Ny] detected weight
+  uint32_t* det2d;                  // [Nx][Ny] detected photons
+};
+
 struct XyzArgs {
   int Nx, Ny, Nz;
   float dx, dy, dz, hx, hy;
-  int bmode, det_state, mcflag, launchflag;
-  float zsurf, n1, n2, rmin, rmax, cos_na;
-  float xs, ys, zs, ux0, uy0, uz0, radius, waist, zfocus;
+  int bmode, det_state;
+  float zsurf;
+  const XyzAux* aux;
   float thr, chance;
   uint32_t max_events;
   uint32_t key0, key1;
@@ -55,8 +70,6 @@ struct XyzArgs {
   const XyzTissue* tissue;
   double* F;                        // may be null (NMC_FLAG_NO_FLUENCE)
   double* tile_part;                // [grid][tile voxels] block partials of the tile
-  double* ryx;
-  uint32_t* det2d;
   nmc_totals* tot;
   RunState* state;
 };
