@@ -194,7 +194,7 @@ __global__ __launch_bounds__(kLayBlock) void layered_kernel(LayArgs a) {
           p.x = p.y = p.z = 0.0f;
           p.ux = p.uy = 0.0f;
           p.uz = 1.0f;
-          p.ruz = 1.0f / p.uz;
+          p.ruz = rcp_rn(p.uz);
           p.l = 0;
           load_layer(p, rows);
           p.hits = 0;
@@ -292,7 +292,7 @@ __global__ __launch_bounds__(kLayBlock) void layered_kernel(LayArgs a) {
           p.ux = p.ux * ni_ / nt;
           p.uy = p.uy * ni_ / nt;
           p.uz = up ? -ca2 : ca2;
-          p.ruz = 1.0f / p.uz;
+          p.ruz = rcp_rn(p.uz);
         }
         p.l += up ? -1 : 1;
         load_layer(p, rows);
@@ -308,7 +308,7 @@ __global__ __launch_bounds__(kLayBlock) void layered_kernel(LayArgs a) {
       const LayRow& R = rows[p.row0 + p.l];
       float ct = R.phase ? tt_cos(R.tt, u01(p.b1)) : hg_cos(R.hg, u01(p.b1));
       spin(ct, u01(p.b2), p.ux, p.uy, p.uz);
-      p.ruz = 1.0f / p.uz;
+      p.ruz = rcp_rn(p.uz);
       p.Kcur++;
       p.n_int++;
       if (p.W < a.thr) {

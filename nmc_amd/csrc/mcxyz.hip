@@ -188,9 +188,9 @@ __device__ __forceinline__ void launch(Photon& p, const XyzArgs& a, const XyzTis
   p.mua = T[p.tis].mua;
   p.mus = T[p.tis].mus;
   p.inv_mus = T[p.tis].inv_mus;
-  p.rux = 1.0f / ux;
-  p.ruy = 1.0f / uy;
-  p.ruz = 1.0f / uz;
+  p.rux = rcp_rn(ux);
+  p.ruy = rcp_rn(uy);
+  p.ruz = rcp_rn(uz);
   p.ev = 1;
   p.events = 0;
   U4 b = philox(p.pid, 1u, 0u, a.key0, a.key1);
@@ -367,9 +367,9 @@ __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
       // SPIN (kernels.metal:695-730)
       float ct = TWO_TERM ? tt_cos(T[p.tis].tt, u01(p.b1)) : hg_cos(T[p.tis].hg, u01(p.b1));
       spin(ct, u01(p.b2), p.ux, p.uy, p.uz);
-      p.rux = 1.0f / p.ux;
-      p.ruy = 1.0f / p.uy;
-      p.ruz = 1.0f / p.uz;
+      p.rux = rcp_rn(p.ux);
+      p.ruy = rcp_rn(p.uy);
+      p.ruz = rcp_rn(p.uz);
       c_int++;
       // ROULETTE (kernels.metal:740-744)
       if (p.W < a.thr) {

@@ -4,10 +4,14 @@
 //                     addresses, the ceiling of the fluence flushes that miss
 //                     the LDS tile (one atomic per voxel visit, as in mcxyz);
 //   nmcp_copy         a plain streaming copy of known size, for scaling the
-//                     FETCH_SIZE / WRITE_SIZE counters to bytes.
+//                     FETCH_SIZE / WRITE_SIZE counters to bytes;
+//   nmcp_rcp_check    the transport kernels' fast reciprocal against IEEE
+//                     division over every binary32 input.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+
+#include "nmc_device.h"
 
 namespace {
 
@@ -37,6 +41,24 @@ __global__ void __launch_bounds__(256) copy_kernel(const float4* __restrict__ a,
 }
 
 constexpr int kGrid = 4096, kBlock = 256;
+
+// Counts the inputs x (all 2^32 bit patterns) whose rcp_rn(x) differs in bits
+// from 1.0f / x (two NaNs count as equal); keeps the smallest differing one.
+__global__ void __launch_bounds__(256) rcp_check_kernel(unsigned long long* bad,
+                                                        unsigned int* first_bad) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  unsigned long long n = 0;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < (1ull << 32); i += stride) {
+    const float x = __uint_as_float((uint32_t)i);
+    const float f = nmc::rcp_rn(x), q = 1.0f / x;
+    const bool same = __float_as_uint(f) == __float_as_uint(q) || (f != f && q != q);
+    if (!same) {
+      ++n;
+      atomicMin(first_bad, (uint32_t)i);
+    }
+  }
+  if (n) atomicAdd(bad, n);
+}
 
 }  // namespace
 
@@ -94,5 +116,32 @@ extern "C" int nmcp_copy(int64_t bytes, int reps, float* ms) {
   }
   hipFree(src);
   hipFree(dst);
+  return rc;
+}
+
+// Exhaustive check of the fast reciprocal; returns 0 and fills *mismatches
+// (inputs that differ from IEEE division) and *first (the smallest such bit
+// pattern, 0xFFFFFFFF if none).
+extern "C" int nmcp_rcp_check(unsigned long long* mismatches, unsigned int* first) {
+  if (!mismatches || !first) return -1;
+  unsigned long long* d_bad = nullptr;
+  unsigned int* d_first = nullptr;
+  if (hipMalloc(&d_bad, sizeof(*d_bad)) != hipSuccess) return -4;
+  if (hipMalloc(&d_first, sizeof(*d_first)) != hipSuccess) {
+    hipFree(d_bad);
+    return -4;
+  }
+  int rc = -3;
+  const unsigned int none = 0xFFFFFFFFu;
+  if (hipMemset(d_bad, 0, sizeof(*d_bad)) == hipSuccess &&
+      hipMemcpy(d_first, &none, sizeof(none), hipMemcpyHostToDevice) == hipSuccess) {
+    hipLaunchKernelGGL(rcp_check_kernel, dim3(kGrid), dim3(kBlock), 0, 0, d_bad, d_first);
+    if (hipGetLastError() == hipSuccess && hipDeviceSynchronize() == hipSuccess &&
+        hipMemcpy(mismatches, d_bad, sizeof(*d_bad), hipMemcpyDeviceToHost) == hipSuccess &&
+        hipMemcpy(first, d_first, sizeof(*first), hipMemcpyDeviceToHost) == hipSuccess)
+      rc = 0;
+  }
+  hipFree(d_bad);
+  hipFree(d_first);
   return rc;
 }
