@@ -150,45 +150,51 @@ def tile_leg(mc, hist, n: int, mode: int, steps: int = 2) -> dict:
                                "oracle (tests/test_gpu_parity.py)") if mode else
                               "every tally within 1e-9 relative of the oracle"}
 
-def sweep_leg(budget_scale: float) -> dict:
+def sweep_leg(budget_scale: float, reps: int = 3) -> dict:
     """128-wavelength sweep: one batched launch vs 128 plain launches (each
-    with its own configure, run and read-back), same photons."""
-    import numpy as np
-
+    with its own configure, run and read-back), same photons.  The pair is
+    measured `reps` times, alternating, and the median ratio reported (the
+    GPU's clock state drifts by a few per cent over a run)."""
     from nmc_amd import params as P
     from nmc_amd import workloads as W
     from nmc_amd.metal_mc import MetalMC
 
     w, items, _ = W.spectral_sweep(budget_scale=budget_scale)
     total = sum(i["n_photons"] for i in items)
-    log(f"[bench] sweep: 128 wavelengths, {total} photons")
-    with MetalMC(P.SIM_MCSUB) as mc:
-        mc.prepare_batch(w.rp, items, w.ext)
-        mc.synchronize()
-        t0 = time.perf_counter()
-        mc.run_batch()
-        tally, tots = mc.get_batch_results()
-        t_batch = time.perf_counter() - t0
-        k_batch = mc.last_kernel_ms()
-    t0 = time.perf_counter()
-    k_plain = 0.0
+    log(f"[bench] sweep: 128 wavelengths, {total} photons, {reps} x (batched, plain)")
+    pairs = []
     same = True
-    with MetalMC(P.SIM_MCSUB) as mc:
-        for j, it in enumerate(items):
-            w.rp.n1, w.rp.n2 = it["n1"], it["n2"]
-            w.tp.muav[1], w.tp.musv[1], w.tp.gv[1] = it["mua"], it["mus"], it["g"]
-            w.ext.stream_id = it["stream_id"]
-            mc.prepare_data(w.rp, w.tp, None, w.ext)
-            mc.send_compute_command(it["n_photons"])
-            res = mc.get_compute_results()
-            k_plain += mc.last_kernel_ms()
-            same &= all(res.totals[k] == tots[j][k] for k in P.TOTAL_INT_FIELDS)
-            same &= bool(np.allclose(res.rd_ra, tally[j, :w.ext.nr + 1], rtol=1e-9, atol=1e-12))
-    t_plain = time.perf_counter() - t0
-    return {"photons": total, "batched_ms": t_batch * 1e3, "batched_kernel_ms": k_batch,
-            "plain_128_ms": t_plain * 1e3, "plain_kernel_ms_sum": k_plain,
-            "speedup": t_plain / t_batch, "tallies_identical": bool(same),
-            "batched_photons_per_s": total / t_batch}
+    for _ in range(reps):
+        with MetalMC(P.SIM_MCSUB) as mc:
+            mc.prepare_batch(w.rp, items, w.ext)
+            mc.synchronize()
+            t0 = time.perf_counter()
+            mc.run_batch()
+            tally, tots = mc.get_batch_results()
+            t_batch = time.perf_counter() - t0
+            k_batch = mc.last_kernel_ms()
+        t0 = time.perf_counter()
+        k_plain = 0.0
+        with MetalMC(P.SIM_MCSUB) as mc:
+            for j, it in enumerate(items):
+                w.rp.n1, w.rp.n2 = it["n1"], it["n2"]
+                w.tp.muav[1], w.tp.musv[1], w.tp.gv[1] = it["mua"], it["mus"], it["g"]
+                w.ext.stream_id = it["stream_id"]
+                mc.prepare_data(w.rp, w.tp, None, w.ext)
+                mc.send_compute_command(it["n_photons"])
+                res = mc.get_compute_results()
+                k_plain += mc.last_kernel_ms()
+                same &= all(res.totals[k] == tots[j][k] for k in P.TOTAL_INT_FIELDS)
+                same &= bool(np.allclose(res.rd_ra, tally[j, :w.ext.nr + 1], rtol=1e-9,
+                                         atol=1e-12))
+        t_plain = time.perf_counter() - t0
+        pairs.append((t_plain / t_batch, t_batch * 1e3, k_batch, t_plain * 1e3, k_plain))
+    pairs.sort()
+    sp, t_batch_ms, k_batch, t_plain_ms, k_plain = pairs[len(pairs) // 2]
+    return {"photons": total, "batched_ms": t_batch_ms, "batched_kernel_ms": k_batch,
+            "plain_128_ms": t_plain_ms, "plain_kernel_ms_sum": k_plain,
+            "speedup": sp, "speedups": [p[0] for p in pairs], "reps": reps,
+            "tallies_identical": bool(same), "batched_photons_per_s": total / (t_batch_ms * 1e-3)}
 
 
 def gradient_leg(n_photons: int) -> dict:
