@@ -248,9 +248,8 @@ def main() -> int:
                     help="only run a plain copy of MIB MiB (scales PMC byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the photon-range "
-              "sharding over several GPUs is described in DESIGN.md but not run here.",
-              file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); sharding the "
+              "photon-id range over several GPUs is not run here.", file=sys.stderr)
         return 2
     if args.calibrate_copy:
         return calibrate_copy(args.calibrate_copy)

@@ -108,7 +108,7 @@ def test_pmc_matches_central_difference_for_mua():
             vals.append(o.rd[0, :40].sum() / s.n_photons)
         fd = (vals[0] - vals[1]) / (2 * h)
         # roulette decisions that flip under the perturbation add zero-mean
-        # noise; judge each layer against the largest component
+        # noise; each layer is checked against the largest component
         assert abs(fd - gmua[0, l]) < 0.01 * abs(gmua[0, l]) + 0.01 * scale, (l, fd, gmua[0, l])
 
 
