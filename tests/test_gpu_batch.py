@@ -130,6 +130,32 @@ def test_mcxyz_tuning_invariance(gpu, ti):
     assert_same_tallies(got, ref, f"tuning {ti}")
 
 
+SUB_TUNINGS = [
+    P.Tuning.make(block=256, refill_min=1),
+    P.Tuning.make(block=512, refill_min=8, chunk=64, mode=2),
+    P.Tuning.make(block=1024, refill_min=32, chunk=1000, mode=1),
+    P.Tuning.make(block=256, refill_min=4, blocks_per_cu=2, mode=2),
+]
+
+
+@pytest.mark.parametrize("ti", range(len(SUB_TUNINGS)))
+def test_mcsub_tuning_invariance(gpu, ti):
+    """Block size (256/512/1024 lanes), refill batch, queue chunk and tally
+    placement (LDS or global replicas) never change an mcsub result."""
+    w = W.mcsub_semi_infinite()
+    _say(f"[tuning] mcsub {SUB_TUNINGS[ti].as_dict()}")
+    ref = O.run(w, 20000, threads=8)
+    with MetalMC(w.sim) as mc:
+        mc.prepare_data(w.rp, w.tp, None, w.ext)
+        mc.set_tuning(SUB_TUNINGS[ti])
+        mc.set_num_runs_seed(1, 0)
+        mc.send_compute_command(20000)
+        got = mc.get_compute_results()
+        eff = mc.get_tuning()
+    assert eff.block == SUB_TUNINGS[ti].block
+    assert_same_tallies(got, ref, f"mcsub tuning {ti}")
+
+
 def test_default_tile_is_used(gpu):
     w = W.skin_vessel(n_photons=1000)
     with MetalMC(w.sim) as mc:
