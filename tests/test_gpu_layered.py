@@ -129,3 +129,34 @@ def test_autograd_wrapper(gpu):
         gmua, gmus = mc.backward(np.tile(w.numpy(), (1, 1)))
     np.testing.assert_allclose(mua.grad.numpy(), gmua[:, :5], rtol=1e-12)
     np.testing.assert_allclose(mus.grad.numpy(), gmus[:, :5], rtol=1e-12)
+
+
+def test_layer_count_extremes(gpu):
+    """A single semi-infinite layer, and a stack of NMC_MAX_LAYERS layers
+    (thin layers with alternating indices, HG and two-term mixed, a finite
+    last layer so photons also leave through the bottom), batched together,
+    with uneven budgets: tallies, gradients and records equal the oracle's."""
+    rng = np.random.default_rng(11)
+    one = W.make_stack([0.0], [0.5], [80.0], [1.37], 3000, stream_id=0)
+    m = P.MAX_LAYERS
+    deep = W.make_stack(rng.uniform(0.005, 0.03, m), rng.uniform(0.2, 5.0, m),
+                        rng.uniform(20.0, 200.0, m), rng.uniform(1.33, 1.5, m), 2000,
+                        stream_id=1, n_below=1.33)
+    for l in range(0, m, 2):
+        deep.layer[l].phase = 1
+        deep.layer[l].gf, deep.layer[l].af, deep.layer[l].gb = 0.9, 0.5, 0.3
+        deep.layer[l].ab, deep.layer[l].cc = 0.5, 0.8
+    cfg = P.LayeredConfig.default(nr=60, dr=0.01)
+    stacks = [one, deep]
+    _say(f"[layered] 1 and {m} layers")
+    ref = O.run_layered(cfg, stacks, threads=4)
+    with LayeredMC(cfg, stacks) as mc:
+        mc.run()
+        res = mc.results()
+        g = np.random.default_rng(1).normal(size=(2, cfg.nr + 1))
+        gmua, gmus = mc.backward(g)
+    _compare(res, ref, 2)
+    assert res[2][1]["escapes_bottom"] > 0
+    rmua, rmus = O.layered_backward(ref, g)
+    np.testing.assert_allclose(gmua, rmua, rtol=1e-9, atol=1e-15)
+    np.testing.assert_allclose(gmus, rmus, rtol=1e-9, atol=1e-15)
