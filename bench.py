@@ -340,7 +340,8 @@ def main() -> int:
                    "dermis/blood cylinder), flat collimated beam r=0.06 cm, n=1, 1e8 photons/step",
                    "photons_per_step": n, "autotuned": args.autotune,
                    "lds_tile": ("FP32 sums within a workgroup, FP64 across workgroups and "
-                                "outside the tile" if eff.mode else "FP64"),
+                                "outside the tile (the reference adds every deposit into an "
+                                "FP32 global array)" if eff.mode else "FP64"),
                    "tile_box": list(eff.tile)},
         "roofline": roof,
         "rd": rd,
