@@ -57,6 +57,43 @@ def test_skin_vessel_1e8_properties(gpu):
     assert abs(rd - ref.mean()) < 0.005 * ref.mean()
 
 
+def test_skin_vessel_1e8_fp32_tile_properties(gpu):
+    """The benchmark's configuration: FP32 LDS tile (sums within a workgroup
+    in FP32, FP64 above), 1e8 photons."""
+    from nmc_amd import params as P
+    from nmc_amd.metal_mc import MetalMC
+
+    w = W.skin_vessel()
+    n = 100_000_000
+    _say(f"[fullscale] mcxyz skin-vessel, FP32 tile, n={n}")
+    with MetalMC(w.sim) as mc:
+        mc.prepare_data(w.rp, w.tp, w.v, w.ext)
+        mc.set_tuning(P.Tuning.make(mode=1))
+        assert mc.get_tuning().mode == 1
+        mc.set_num_runs_seed(1, 0)
+        mc.send_compute_command(n)
+        got = mc.get_compute_results()
+    t = got.totals
+    assert t["photons"] == n
+    assert abs(energy_balance(t) - 1.0) < 1e-6
+    # FP32 partial sums per workgroup: the grid total matches the FP64
+    # absorbed tally to FP32 accuracy
+    assert got.fluence.sum() == pytest.approx(t["absorbed"], rel=1e-5)
+    assert got.ryx.sum() == pytest.approx(t["rd"], rel=1e-9)
+    ryx = got.ryx.reshape(w.rp.Nx, w.rp.Ny)
+    lo, hi = ryx[:, : w.rp.Ny // 2].sum(), ryx[:, w.rp.Ny // 2:].sum()
+    assert abs(lo - hi) < 0.01 * (lo + hi)
+    F = got.fluence.reshape(w.rp.Nz, w.rp.Nx, w.rp.Ny)
+    a, b = F[:, :, : w.rp.Ny // 2].sum(), F[:, :, w.rp.Ny // 2:][:, :, ::-1].sum()
+    assert abs(a - b) < 0.01 * (a + b)
+    rd = t["rd"] / n
+    _say("[fullscale] oracle: 32 independent batches of 100000 photons")
+    ref = _oracle_rd(w, 32, 100_000, first=1 << 40)
+    sem = ref.std(ddof=1) / np.sqrt(len(ref))
+    assert abs(rd - ref.mean()) < 4.0 * sem + 1e-12, (rd, ref.mean(), sem)
+    assert abs(rd - ref.mean()) < 0.005 * ref.mean()
+
+
 def test_mcsub_1e8_properties(gpu):
     w = W.mcsub_semi_infinite()
     n = 100_000_000
