@@ -35,8 +35,11 @@ struct Photon {
 
 __device__ __forceinline__ int clampi(int v, int n) { return v < 0 ? 0 : (v >= n ? n - 1 : v); }
 
-__device__ __forceinline__ int64_t vidx(const XyzArgs& a, int ix, int iy, int iz) {
-  return (int64_t)iz * a.Ny * a.Nx + (int64_t)ix * a.Ny + iy;
+// Linear voxel index iz*Ny*Nx + ix*Ny + iy (kernels.metal:482) of an in-grid
+// voxel, in 32-bit arithmetic: nmc_prepare rejects grids above 2e9 voxels
+// (the reference indexes with a 32-bit int), so every partial sum fits.
+__device__ __forceinline__ uint32_t vidx(const XyzArgs& a, int ix, int iy, int iz) {
+  return ((uint32_t)iz * (uint32_t)a.Nx + (uint32_t)ix) * (uint32_t)a.Ny + (uint32_t)iy;
 }
 
 __device__ __forceinline__ void lds_addi(nmc_totals* t, int k, unsigned long long v) {
