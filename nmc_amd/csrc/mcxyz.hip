@@ -13,11 +13,14 @@
 // depend only on its id, never on the lane or the order of execution.
 //
 // Fluence: deposits are summed per voxel visit in a register (FP64) and added
-// when the photon leaves the voxel - into the workgroup's LDS tile (FP64) if
-// the voxel lies inside the tile box, else with an FP64 global atomic.  The
-// tiles are written out as block partials and summed over blocks in a fixed
-// order.  Rare events (detections, escapes, roulette) go to LDS totals;
-// frequent counters stay in registers until the workgroup ends.
+// when the photon leaves the voxel - into the workgroup's LDS tile (FP64, or
+// FP32 with twice the voxels: TileT) if the voxel lies inside the tile box,
+// else with an FP64 global atomic.  The tiles are written out as FP64 block
+// partials and summed over blocks in a fixed order.  Rare events (detections,
+// escapes, roulette) go to LDS totals; frequent counters stay in registers
+// until the workgroup ends.  The source and detector-plane code is kept out
+// of line (source_photon, detector_plane) with its parameters in device
+// memory, so the loop's scalar registers hold only what the loop uses.
 #include "nmc_internal.h"
 
 namespace nmc {
