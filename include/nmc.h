@@ -321,7 +321,9 @@ int nmc_layered_records(nmc_handle* h, uint32_t* out, uint64_t max_records, uint
  * grad_rd[j][ir] = dLoss/d(rd[j][ir] / n_photons_j), returns
  * grad_mua[j][l] = sum over records of grad_rd * w * (-L_l) / n_j and
  * grad_mus[j][l] = sum of grad_rd * w * (K_l / mus_l - L_l) / n_j, where L_l and
- * K_l are the photon's path length and scattering count in layer l. */
+ * K_l are the photon's path length and scattering count in layer l.  The
+ * records must come from the current properties: after nmc_layered_update the
+ * call fails with NMC_ERR_STATE until nmc_layered_run has run again. */
 int nmc_layered_backward(nmc_handle* h, const double* grad_rd, double* grad_mua,
                          double* grad_mus);
 

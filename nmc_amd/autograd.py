@@ -14,6 +14,7 @@ from __future__ import annotations
 import numpy as np
 import torch
 
+from . import params as P
 from .layered import LayeredMC
 
 
@@ -21,18 +22,23 @@ class LayeredRd(torch.autograd.Function):
     @staticmethod
     def forward(ctx, mua: torch.Tensor, mus: torch.Tensor, runner: LayeredMC):
         nl = mua.shape[1]
-        full_mua = np.zeros((runner.n_stacks, 8))
-        full_mus = np.zeros((runner.n_stacks, 8))
+        full_mua = np.zeros((runner.n_stacks, P.MAX_LAYERS))
+        full_mus = np.zeros((runner.n_stacks, P.MAX_LAYERS))
         full_mua[:, :nl] = mua.detach().cpu().double().numpy()
         full_mus[:, :nl] = mus.detach().cpu().double().numpy()
         runner.set_properties(full_mua, full_mus)
         runner.run()
         ctx.runner = runner
+        ctx.epoch = runner.epoch
         ctx.nl = nl
         return torch.from_numpy(runner.rd_per_photon()).to(mua.device)
 
     @staticmethod
     def backward(ctx, grad_rd: torch.Tensor):
+        if ctx.runner.epoch != ctx.epoch:
+            raise RuntimeError("the runner has run or changed properties since this forward; "
+                               "its path records belong to a later call (use one runner per "
+                               "forward that is differentiated)")
         gmua, gmus = ctx.runner.backward(grad_rd.detach().cpu().double().numpy())
         nl = ctx.nl
         dev = grad_rd.device

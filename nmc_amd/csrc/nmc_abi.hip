@@ -201,6 +201,7 @@ struct nmc_handle {
   uint32_t* dRec = nullptr;
   uint64_t rec_cap = 0;
   int rec_nl = 0;
+  bool rec_valid = false;      // records belong to the properties now uploaded
   double* dGrad = nullptr;     // grad_rd [n][nr+1] | inv_n [n] | grad_mua, grad_mus [n][MAXL]
   float* dMus = nullptr;
   int* dNl = nullptr;
@@ -210,7 +211,8 @@ struct nmc_handle {
   bool graph_mode = false;
   hipGraphExec_t graph_exec = nullptr;
   uint64_t graph_n = 0;
-  uint32_t graph_key = 0;
+  uint64_t graph_seed = 0;     // the Philox key the captured launch arguments hold
+  uint32_t graph_stream = 0;
 };
 
 namespace {
@@ -483,8 +485,8 @@ int run_async(nmc_handle* h, uint64_t n) {
   HIP_TRY(hipSetDevice(h->device));
   HIP_TRY(hipEventRecord(h->e0, h->stream));
   if (h->graph_mode) {
-    uint32_t key = key0_of(h) ^ (key1_of(h, h->ext.stream_id) * 0x9E3779B9u);
-    if (!h->graph_exec || h->graph_n != n || h->graph_key != key) {
+    if (!h->graph_exec || h->graph_n != n || h->graph_seed != h->ext.seed ||
+        h->graph_stream != h->ext.stream_id) {
       drop_graph(h);
       hipGraph_t g = nullptr;
       HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
@@ -499,7 +501,8 @@ int run_async(nmc_handle* h, uint64_t n) {
       (void)hipGraphDestroy(g);
       HIP_TRY(ie);
       h->graph_n = n;
-      h->graph_key = key;
+      h->graph_seed = h->ext.seed;
+      h->graph_stream = h->ext.stream_id;
     }
     HIP_TRY(hipGraphLaunch(h->graph_exec, h->stream));
   } else {
@@ -861,6 +864,9 @@ int nmc_run_async(nmc_handle* h, uint64_t n_photons) {
   if (h->sim == NMC_SIM_MCSUB) {
     if (h->batch) return fail(NMC_ERR_STATE, "handle holds a batch; use nmc_run_batch");
     if (h->items[0].n_photons != n_photons) {
+      // a run still in flight reads the old budget
+      HIP_TRY(hipSetDevice(h->device));
+      HIP_TRY(hipStreamSynchronize(h->stream));
       h->items[0].n_photons = n_photons;
       h->prefix[1] = n_photons;
       HIP_TRY(hipMemcpy(h->dPrefix, h->prefix.data(), 2 * sizeof(uint64_t), hipMemcpyHostToDevice));
@@ -1088,6 +1094,7 @@ int nmc_layered_prepare(nmc_handle* h, const nmc_layered_config* cfg, const nmc_
   HIP_TRY(hipMalloc(&h->dGrad, ((size_t)n * (cfg->nr + 1) + n + 2 * (size_t)n * NMC_MAX_LAYERS) * sizeof(double)));
   HIP_TRY(hipMalloc(&h->dRecCount, 2 * sizeof(unsigned long long)));
   h->rec_cap = 0;
+  h->rec_valid = false;
   h->lay_lds = layered_lds_bytes(n, h->lay_rows, 0);
   if (h->lay_lds + 1024 > h->lds_max)
     return fail(NMC_ERR_ARG, "%d stacks / %d layers need %zu B of LDS (%zu available)", n, h->lay_rows,
@@ -1145,6 +1152,7 @@ int nmc_layered_update(nmc_handle* h, const nmc_stack* stacks, int n_stacks) {
   HIP_TRY(hipSetDevice(h->device));
   HIP_TRY(hipStreamSynchronize(h->stream));
   h->stacks.assign(stacks, stacks + n_stacks);
+  h->rec_valid = false;
   return upload_stacks(h);
 }
 
@@ -1152,6 +1160,7 @@ int nmc_layered_run(nmc_handle* h) {
   if (!h) return fail(NMC_ERR_ARG, "null handle");
   if (h->sim != NMC_SIM_LAYERED || !h->prepared) return fail(NMC_ERR_STATE, "nmc_layered_prepare has not succeeded");
   HIP_TRY(hipSetDevice(h->device));
+  h->rec_valid = false;
   int n = (int)h->stacks.size();
   hipStream_t s = h->stream;
   const int rd_len = n * (h->lcfg.nr + 1);
@@ -1193,6 +1202,7 @@ int nmc_layered_run(nmc_handle* h) {
   HIP_TRY(hipEventRecord(h->e1, s));
   h->timed = true;
   HIP_TRY(hipStreamSynchronize(s));
+  h->rec_valid = true;
   return ok();
 }
 
@@ -1246,6 +1256,8 @@ int nmc_layered_backward(nmc_handle* h, const double* grad_rd, double* grad_mua,
   if (!h || !grad_rd) return fail(NMC_ERR_ARG, "null argument");
   if (h->sim != NMC_SIM_LAYERED || !h->prepared) return fail(NMC_ERR_STATE, "nmc_layered_prepare has not succeeded");
   if (!h->lcfg.records) return fail(NMC_ERR_STATE, "records were not enabled at prepare");
+  if (!h->rec_valid)
+    return fail(NMC_ERR_STATE, "no path records for the current properties: nmc_layered_run first");
   HIP_TRY(hipSetDevice(h->device));
   hipStream_t s = h->stream;
   size_t n = h->stacks.size(), nb = n * (h->lcfg.nr + 1);

@@ -25,6 +25,9 @@ class LayeredMC:
         if tuning is not None:
             check(lib().nmc_set_tuning(self._h, C.byref(tuning)))
         self.cfg = cfg
+        # bumped by every change of properties and every run: a backward pass
+        # is valid only for the records of the run it follows
+        self.epoch = 0
         self._stacks = (P.Stack * len(stacks))(*stacks)
         check(lib().nmc_layered_prepare(self._h, C.byref(cfg), self._stacks, len(stacks)))
 
@@ -43,12 +46,14 @@ class LayeredMC:
             for l in range(s.n_layers):
                 s.layer[l].mua = float(mua[j, l])
                 s.layer[l].mus = float(mus[j, l])
+        self.epoch += 1
         check(lib().nmc_layered_update(self._h, self._stacks, len(self._stacks)))
 
     def set_seed(self, seed: int, first_photon: int = 0) -> None:
         check(lib().nmc_set_seed(self._h, int(seed), int(first_photon)))
 
     def run(self) -> None:
+        self.epoch += 1
         check(lib().nmc_layered_run(self._h))
 
     def last_kernel_ms(self) -> float:
@@ -96,7 +101,7 @@ class LayeredMC:
         """Rd(r) per launched photon, [n_stacks, nr+1] (the backward's variable)."""
         rd = self.results()[0]
         budgets = np.array([s.n_photons for s in self._stacks], np.float64)
-        return rd / budgets[:, None]
+        return rd / np.maximum(budgets, 1.0)[:, None]
 
     def backward(self, grad_rd: np.ndarray):
         """grad_rd = dLoss/d(rd_per_photon) -> (dLoss/dmua, dLoss/dmus) [n, MAX_LAYERS]."""

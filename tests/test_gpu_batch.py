@@ -93,6 +93,21 @@ def test_plain_mcsub_graph_replay(gpu, placement):
     assert_same_tallies(g, ref, "graph")
 
 
+def test_mcsub_async_budget_change(gpu):
+    """Back-to-back asynchronous runs of different sizes: the second run's
+    budget must not reach the first, still in flight."""
+    w = W.mcsub_semi_infinite()
+    _say("[async] mcsub 30000 + 10000 photons, no wait between")
+    with MetalMC(w.sim) as mc:
+        mc.prepare_data(w.rp, w.tp, None, w.ext)
+        mc.send_compute_command_async(30000)
+        mc.send_compute_command_async(10000)
+        mc.synchronize()
+        g = mc.get_compute_results()
+    ref = O.run(w, 40000, threads=8)
+    assert_same_tallies(g, ref, "async-budget")
+
+
 def test_mcxyz_graph_replay(gpu):
     w = W.skin_vessel()
     _say("[graph] mcxyz 2 replays x 10000")
@@ -104,6 +119,29 @@ def test_mcxyz_graph_replay(gpu):
         g = mc.get_compute_results()
     ref = O.run(w, 20000, threads=8)
     assert_same_tallies(g, ref, "graph-xyz")
+
+
+def test_mcxyz_graph_seed_change(gpu):
+    """A new seed between replays must reach the captured launch.  The two
+    seeds differ in both key words yet agree in key0 ^ (key1 * 0x9E3779B9),
+    so a graph cache keyed on a 32-bit mix of the key would replay the first
+    seed's stream."""
+    w = W.skin_vessel()
+    seed_a = 5
+    seed_b = (1 << 32) | (5 ^ 0x9E3779B9)
+    _say("[graph] mcxyz seed change 2 x 8000")
+    with MetalMC(w.sim) as mc:
+        mc.prepare_data(w.rp, w.tp, w.v, w.ext)
+        mc.set_graph_mode(True)
+        mc.set_num_runs_seed(seed_a, 0)
+        mc.send_compute_command(8000)
+        mc.reset_tallies()
+        mc.set_num_runs_seed(seed_b, 0)
+        mc.send_compute_command(8000)
+        g = mc.get_compute_results()
+    w.ext.seed = seed_b
+    ref = O.run(w, 8000, threads=8)
+    assert_same_tallies(g, ref, "graph-seed")
 
 
 TUNINGS = [

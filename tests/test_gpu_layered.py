@@ -131,6 +131,49 @@ def test_autograd_wrapper(gpu):
     np.testing.assert_allclose(mus.grad.numpy(), gmus[:, :5], rtol=1e-12)
 
 
+def test_backward_needs_records_of_current_properties(gpu):
+    """After new properties the stored records describe the old ones: the
+    backward pass refuses until the next run, then works again."""
+    from nmc_amd._lib import NmcError
+
+    cfg, stacks = W.multilayer_5(n_photons=3000)
+    g = np.ones((1, cfg.nr + 1))
+    with LayeredMC(cfg, stacks) as mc:
+        with pytest.raises(NmcError) as e:
+            mc.backward(g)                          # nothing has run yet
+        assert e.value.code == P.ERR_STATE
+        mc.run()
+        mc.backward(g)
+        mua = np.array([[l.mua for l in stacks[0].layer]]) * 1.1
+        mus = np.array([[l.mus for l in stacks[0].layer]])
+        mc.set_properties(mua, mus)
+        with pytest.raises(NmcError) as e:
+            mc.backward(g)
+        assert e.value.code == P.ERR_STATE
+        mc.run()
+        gmua, _ = mc.backward(g)
+    assert np.all(np.isfinite(gmua))
+
+
+def test_autograd_rejects_a_superseded_forward(gpu):
+    """Two forwards through one runner: the first one's backward would read
+    the second one's records, so it raises instead."""
+    import torch
+
+    from nmc_amd.autograd import layered_rd
+
+    cfg, stacks = W.multilayer_5(n_photons=2000)
+    mua = torch.tensor([[l.mua for l in stacks[0].layer][:5]], dtype=torch.float64, requires_grad=True)
+    mus = torch.tensor([[l.mus for l in stacks[0].layer][:5]], dtype=torch.float64, requires_grad=True)
+    with LayeredMC(cfg, stacks) as mc:
+        rd1 = layered_rd(mua, mus, mc)
+        rd2 = layered_rd(mua * 1.05, mus, mc)
+        with pytest.raises(RuntimeError, match="later call"):
+            rd1.sum().backward()
+        rd2.sum().backward()
+    assert torch.isfinite(mua.grad).all()
+
+
 def test_layer_count_extremes(gpu):
     """A single semi-infinite layer, and a stack of NMC_MAX_LAYERS layers
     (thin layers with alternating indices, HG and two-term mixed, a finite
