@@ -9,11 +9,14 @@ For an mcxyz workload it
      block size x refill batch)
      and keeps the fastest.
 Results are cached in a JSON file keyed by a hash of the workload, so a
-repeated configuration is tuned once.  For batched mcsub and layered handles
-the search is over block size / refill batch / tally placement only.
+repeated configuration is tuned once.  For a batched mcsub launch
+(`autotune_mcsub_batch`) the search is over block size x tally placement (LDS
+or global replicas) x refill batch, each trial timed on the batch itself.
+Layered handles fix their grid at prepare and use the library's defaults.
 
-    from nmc_amd.autotune import autotune_mcxyz
+    from nmc_amd.autotune import autotune_mcxyz, autotune_mcsub_batch
     tuning = autotune_mcxyz(workload)           # -> params.Tuning
+    tuning = autotune_mcsub_batch(rp, items, ext)
 """
 from __future__ import annotations
 
@@ -157,8 +160,68 @@ def autotune_mcxyz(w, n_trial: int = 20_000_000, device: int = 0, use_cache: boo
                 print(f"[autotune] {mc.get_tuning().as_dict()} {ms:.2f} ms", flush=True)
             if ms < best_ms:
                 best, best_ms = mc.get_tuning(), ms
+    if best is None:
+        raise RuntimeError(f"no mcxyz launch shape fits this workload: {log}")
     res = {"tuning": best.as_dict(), "ms": best_ms, "n_trial": n_trial, "trials": log,
            "time": time.time()}
+    res["tuning"].pop("blocks_per_cu", None)
+    if use_cache:
+        _store_cache(key, res)
+    return P.Tuning.make(**res["tuning"])
+
+
+def _item_bytes(items) -> bytes:
+    arr = (P.WorkItem * len(items))()
+    for i, it in enumerate(items):
+        if isinstance(it, dict):
+            for k, v in it.items():
+                setattr(arr[i], k, v)
+        else:
+            arr[i] = it
+    return bytes(arr)
+
+
+def autotune_mcsub_batch(rp: P.RunParams, items, ext: P.ExtConfig | None = None, device: int = 0,
+                         use_cache: bool = True, verbose: bool = False) -> P.Tuning:
+    """Fastest (block size, tally placement, refill batch) for one batched
+    mcsub launch of `items`; every trial transports the same photons (seed
+    reset before each), so only the launch shape differs."""
+    ext = ext if ext is not None else P.ExtConfig.default()
+    h = hashlib.sha1()
+    h.update(bytes(rp))
+    h.update(bytes(ext))
+    h.update(_item_bytes(items))
+    h.update(b"mcsub-batch-v1")
+    key = h.hexdigest()
+    if use_cache:
+        hit = _load_cache().get(key)
+        if hit:
+            return P.Tuning.make(**hit["tuning"])
+    trials = [P.Tuning.make(block=b, mode=m, refill_min=r)
+              for b in (1024, 512, 256) for m in (0, 2) for r in (1, 4)]
+    best, best_ms, log = None, float("inf"), []
+    with MetalMC(P.SIM_MCSUB, device) as mc:
+        mc.prepare_batch(rp, items, ext)
+        mc.set_num_runs_seed(ext.seed, 0)
+        mc.run_batch()                               # warm-up
+        for t in trials:
+            try:
+                mc.set_tuning(t)
+            except Exception as e:   # e.g. tallies that do not fit the block's LDS share
+                log.append({"tuning": t.as_dict(), "error": str(e)})
+                continue
+            mc.set_num_runs_seed(ext.seed, 0)
+            mc.run_batch()
+            ms = mc.last_kernel_ms()
+            eff = mc.get_tuning()
+            log.append({"tuning": eff.as_dict(), "ms": ms})
+            if verbose:
+                print(f"[autotune] {eff.as_dict()} {ms:.2f} ms", flush=True)
+            if ms < best_ms:
+                best, best_ms = eff, ms
+    if best is None:
+        raise RuntimeError(f"no mcsub launch shape fits this batch: {log}")
+    res = {"tuning": best.as_dict(), "ms": best_ms, "trials": log, "time": time.time()}
     res["tuning"].pop("blocks_per_cu", None)
     if use_cache:
         _store_cache(key, res)

@@ -124,7 +124,7 @@ def main(argv: list[str] | None = None) -> int:
     a = ap.parse_args(argv)
     try:
         run(a.name, int(a.photons), a.seed, a.device, a.file_beam, a.nr, a.nz, a.dr, a.dz)
-    except NmcError as e:
+    except (NmcError, OSError, ValueError) as e:   # library errors, unreadable or malformed files
         print(f"nmc: {e}", file=sys.stderr)
         return 1
     return 0

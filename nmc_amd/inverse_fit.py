@@ -41,14 +41,20 @@ def make_target(runner: LayeredMC, seed: int, noise: float = 0.01) -> np.ndarray
 def fit(n_inst: int = 256, steps: int = 50, budget_scale: float = 1.0, lr: float = 0.05,
         seed: int = 5, init_spread: float = 0.4, device: int = 0, verbose: bool = False,
         plain_compare: bool = True) -> dict:
+    cfg, hidden = W.inverse_fit_instances(n_inst, seed, budget_scale)
+    nl = hidden[0].n_layers
+    true_mua, true_mus = _props(hidden, nl)
+    with LayeredMC(cfg, hidden, device=device) as runner:
+        return _fit(runner, cfg, hidden, nl, true_mua, true_mus, steps, lr, seed, init_spread,
+                    device, verbose, plain_compare)
+
+
+def _fit(runner, cfg, hidden, nl, true_mua, true_mus, steps, lr, seed, init_spread, device,
+         verbose, plain_compare) -> dict:
     import torch
 
     from .autograd import layered_rd
 
-    cfg, hidden = W.inverse_fit_instances(n_inst, seed, budget_scale)
-    nl = hidden[0].n_layers
-    true_mua, true_mus = _props(hidden, nl)
-    runner = LayeredMC(cfg, hidden, device=device)
     target = make_target(runner, seed)
     rng = np.random.default_rng(seed + 2000)
     start_mua = true_mua * np.exp(rng.uniform(-init_spread, init_spread, true_mua.shape))
@@ -69,9 +75,9 @@ def fit(n_inst: int = 256, steps: int = 50, budget_scale: float = 1.0, lr: float
         loss.backward()
         kernel_ms += runner.last_kernel_ms()
         opt.step()
-        history.append(float(loss))
+        history.append(loss.item())
         if verbose:
-            print(f"step {step} loss {float(loss):.4g}", flush=True)
+            print(f"step {step} loss {history[-1]:.4g}", flush=True)
     wall = time.perf_counter() - t0
     fit_mua, fit_mus = np.exp(la.detach().numpy()), np.exp(ls.detach().numpy())
     err_a = np.abs(fit_mua - true_mua) / true_mua
@@ -79,7 +85,7 @@ def fit(n_inst: int = 256, steps: int = 50, budget_scale: float = 1.0, lr: float
     start_err_a = np.abs(start_mua - true_mua) / true_mua
     start_err_s = np.abs(start_mus - true_mus) / true_mus
     out = {
-        "instances": n_inst, "layers": nl, "steps": steps,
+        "instances": len(hidden), "layers": nl, "steps": steps,
         "photons_per_step": int(sum(s.n_photons for s in hidden)),
         "loss_first": history[0], "loss_last": history[-1],
         "median_rel_err_mua_per_layer": np.median(err_a, axis=0).tolist(),
@@ -91,7 +97,6 @@ def fit(n_inst: int = 256, steps: int = 50, budget_scale: float = 1.0, lr: float
     if plain_compare:
         out.update(plain_step_time(cfg, hidden, device))
         out["batched_speedup_per_step"] = out["plain_wall_s_per_step"] / out["wall_s_per_step"]
-    runner.close()
     return out
 
 

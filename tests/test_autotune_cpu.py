@@ -21,3 +21,23 @@ def test_box_search_finds_the_blob():
         if min(zz, xx, yy) < 0 or zz + nz > 40 or xx + nx > 60 or yy + ny > 60:
             continue
         assert hist[zz:zz + nz, xx:xx + nx, yy:yy + ny].sum() / hist.sum() <= cov + 1e-12
+
+
+def test_batch_cache_key_covers_every_item_field():
+    """The batched-mcsub tuner keys its cache on the items' bytes: dicts and
+    WorkItem structs of the same media agree, and any field change differs."""
+    from nmc_amd import params as P
+    from nmc_amd import workloads as W
+    from nmc_amd.autotune import _item_bytes
+
+    _, items, _ = W.spectral_sweep(n_wavelengths=4, budget_scale=0.001)
+    structs = []
+    for it in items:
+        s = P.WorkItem()
+        for k, v in it.items():
+            setattr(s, k, v)
+        structs.append(s)
+    assert _item_bytes(items) == _item_bytes(structs)
+    changed = [dict(it) for it in items]
+    changed[2]["n_photons"] += 1
+    assert _item_bytes(changed) != _item_bytes(items)

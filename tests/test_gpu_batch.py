@@ -201,3 +201,29 @@ def test_default_tile_is_used(gpu):
         t = mc.get_tuning()
     _say(f"[tuning] default {t.as_dict()}")
     assert t.tile[3] * t.tile[4] * t.tile[5] > 1000
+
+
+def test_mcsub_batch_autotuner(gpu, tmp_path, monkeypatch):
+    """The batched-mcsub tuner returns a launch shape the library accepts,
+    caches it, and the tuned launch gives the default launch's tallies."""
+    from nmc_amd import autotune as A
+
+    monkeypatch.setattr(A, "CACHE", str(tmp_path / "tuning.json"))
+    w, items, _ = W.spectral_sweep(n_wavelengths=16, budget_scale=0.001)
+    _say(f"[autotune] mcsub batch, 16 items, {sum(i['n_photons'] for i in items)} photons")
+    t = A.autotune_mcsub_batch(w.rp, items, w.ext)
+    assert t.block in (256, 512, 1024) and t.mode in (1, 2), t.as_dict()
+    assert A.autotune_mcsub_batch(w.rp, items, w.ext).as_dict() == t.as_dict()   # cached
+    out = []
+    for tuning in (None, t):
+        with MetalMC(P.SIM_MCSUB) as mc:
+            mc.prepare_batch(w.rp, items, w.ext)
+            if tuning is not None:
+                mc.set_tuning(tuning)
+            mc.run_batch()
+            out.append(mc.get_batch_results())
+    (ta, sa), (tb, sb) = out
+    for j in range(len(items)):
+        for k in P.TOTAL_INT_FIELDS:
+            assert sa[j][k] == sb[j][k], (j, k)
+    np.testing.assert_allclose(ta, tb, rtol=RTOL, atol=1e-12)
