@@ -52,15 +52,32 @@ __device__ __forceinline__ void lds_addf(nmc_totals* t, int k, double v) {
   unsafeAtomicAdd(&((double*)t)[k], v);
 }
 
-// Adds the deposits of the current voxel visit to the tile or the global grid.
-template <typename TileT>
-__device__ __forceinline__ void flush_visit(Photon& p, const XyzArgs& a, TileT* tile) {
-  if (p.inside && a.F && (a.count_visits || p.vdep != 0.0)) {
-    double v = a.count_visits ? 1.0 : p.vdep;
-    unsigned lx = (unsigned)(p.ix - a.tile_x0), ly = (unsigned)(p.iy - a.tile_y0),
-             lz = (unsigned)(p.iz - a.tile_z0);
-    if (lx < (unsigned)a.tile_nx && ly < (unsigned)a.tile_ny && lz < (unsigned)a.tile_nz)
-      unsafeAtomicAdd(&tile[((int)lz * a.tile_nx + (int)lx) * a.tile_ny + (int)ly], (TileT)v);
+// A wave-uniform value kept in a vector register.  The transport loop uses
+// every scalar register the hardware gives a wave, so the compiler would
+// otherwise park such values in lanes of a vector register and fetch them
+// back (one v_readlane each) every time the voxel-flush code runs; a copy per
+// lane is free here, as the LDS tile already limits the CU to one workgroup.
+__device__ __forceinline__ uint32_t in_vgpr(uint32_t s) {
+  uint32_t v;
+  asm volatile("v_mov_b32 %0, %1" : "=v"(v) : "s"(s));
+  return v;
+}
+
+// The LDS tile box, in vector registers (in_vgpr).
+struct TileBox {
+  uint32_t x0, y0, z0, nx, ny, nz;
+};
+
+// Adds the deposits of the current voxel visit to the tile or the global grid
+// (COUNT: adds 1 per visit instead, the tile-placement pilot's visit count).
+template <bool COUNT, typename TileT>
+__device__ __forceinline__ void flush_visit(Photon& p, const XyzArgs& a, const TileBox& tb,
+                                            TileT* tile) {
+  if (p.inside && (COUNT || p.vdep != 0.0)) {
+    double v = COUNT ? 1.0 : p.vdep;
+    uint32_t lx = (uint32_t)p.ix - tb.x0, ly = (uint32_t)p.iy - tb.y0, lz = (uint32_t)p.iz - tb.z0;
+    if (lx < tb.nx && ly < tb.ny && lz < tb.nz)
+      unsafeAtomicAdd(&tile[(lz * tb.nx + lx) * tb.ny + ly], (TileT)v);
     else
       unsafeAtomicAdd(&a.F[vidx(a, p.ix, p.iy, p.iz)], v);
   }
@@ -205,7 +222,7 @@ __device__ __forceinline__ void launch(Photon& p, const XyzArgs& a, const XyzTis
   p.alive = true;
 }
 
-template <bool TWO_TERM, int BLOCK, typename TileT>
+template <bool TWO_TERM, int BLOCK, typename TileT, bool COUNT>
 __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
   extern __shared__ double tile_mem[];
   TileT* tile = (TileT*)tile_mem;  // [tile_nz][tile_nx][tile_ny]
@@ -218,6 +235,17 @@ __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
   if (threadIdx.x < TI_COUNT) (&st.photons)[threadIdx.x] = 0ull;
   __syncthreads();
   const uint64_t first = a.state->first;
+  const TileBox tb{in_vgpr((uint32_t)a.tile_x0), in_vgpr((uint32_t)a.tile_y0),
+                   in_vgpr((uint32_t)a.tile_z0), in_vgpr((uint32_t)a.tile_nx),
+                   in_vgpr((uint32_t)a.tile_ny), in_vgpr((uint32_t)a.tile_nz)};
+  // grid size, boundary and detector modes, likewise in vector registers; no
+  // fluence tally (NMC_FLAG_NO_FLUENCE): voxel visits are not flushed
+  const int gnx = (int)in_vgpr((uint32_t)a.Nx), gny = (int)in_vgpr((uint32_t)a.Ny),
+            gnz = (int)in_vgpr((uint32_t)a.Nz);
+  const uint32_t bmode = in_vgpr((uint32_t)a.bmode);
+  const bool det_refl = in_vgpr(a.det_state == NMC_DET_REFLECTANCE ? 1u : 0u) != 0u;
+  const float zsurf = __uint_as_float(in_vgpr(__float_as_uint(a.zsurf)));
+  const bool tally_f = in_vgpr(a.F != nullptr ? 1u : 0u) != 0u;
 
   const int lane = lane_id();
   Photon p;
@@ -291,10 +319,10 @@ __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
       s_face = tz;
     }
     float s_surf = kInf;
-    if (a.det_state == NMC_DET_REFLECTANCE) {
-      if (p.uz < 0.0f && p.z > a.zsurf) s_surf = fmaxf((a.zsurf - p.z) * p.ruz, 0.0f);
+    if (det_refl) {
+      if (p.uz < 0.0f && p.z > zsurf) s_surf = fmaxf((zsurf - p.z) * p.ruz, 0.0f);
     } else {
-      if (p.uz > 0.0f && p.z < a.zsurf) s_surf = fmaxf((a.zsurf - p.z) * p.ruz, 0.0f);
+      if (p.uz > 0.0f && p.z < zsurf) s_surf = fmaxf((zsurf - p.z) * p.ruz, 0.0f);
     }
     int kind;
     float s;
@@ -332,7 +360,7 @@ __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
       if (!(p.sleft > 0.0f)) p.sleft = 0.0f;
       if (kind == 1) {
         // detector plane (kernels.metal:556-630)
-        p.z = a.zsurf;
+        p.z = zsurf;
         lds_addi(&st, TI_SURFACE_HITS, 1);
         const DetectorHit d = detector_plane(p.x, p.y, p.uz, p.W, p.ix, p.iy, a.aux);
         if (d.where == 1) {
@@ -347,16 +375,16 @@ __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
         if (!(p.W > 0.0f)) alive = false;
       } else {
         // voxel face (kernels.metal:641-679)
-        flush_visit(p, a, tile);
+        if (tally_f) flush_visit<COUNT>(p, a, tb, tile);
         if (axis == 0) p.ix += p.ux > 0.0f ? 1 : -1;
         else if (axis == 1) p.iy += p.uy > 0.0f ? 1 : -1;
         else p.iz += p.uz > 0.0f ? 1 : -1;
-        p.inside = p.ix >= 0 && p.ix < a.Nx && p.iy >= 0 && p.iy < a.Ny && p.iz >= 0 && p.iz < a.Nz;
-        if (!p.inside && (a.bmode == 1 || (a.bmode == 2 && p.iz < 0))) {
+        p.inside = p.ix >= 0 && p.ix < gnx && p.iy >= 0 && p.iy < gny && p.iz >= 0 && p.iz < gnz;
+        if (!p.inside && (bmode == 1u || (bmode == 2u && p.iz < 0))) {
           tally_escape(&st, a, p.iz, p.W);
           alive = false;
         } else {
-          p.tis = a.V[vidx(a, clampi(p.ix, a.Nx), clampi(p.iy, a.Ny), clampi(p.iz, a.Nz))];
+          p.tis = a.V[vidx(a, clampi(p.ix, gnx), clampi(p.iy, gny), clampi(p.iz, gnz))];
           p.mua = T[p.tis].mua;
           p.mus = T[p.tis].mus;
           p.inv_mus = T[p.tis].inv_mus;
@@ -400,7 +428,7 @@ __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
       }
     }
     if (!alive) {
-      flush_visit(p, a, tile);
+      if (tally_f) flush_visit<COUNT>(p, a, tb, tile);
       p.alive = false;
     }
   }
@@ -436,55 +464,73 @@ __global__ void tile_reduce_kernel(const double* __restrict__ part, int grid, Xy
   }
 }
 
-template <bool TT, typename TileT>
+template <bool TT, typename TileT, bool COUNT>
 hipError_t launch_block(const XyzArgs& a, int block, int grid, size_t lds, hipStream_t s) {
   switch (block) {
     case 256:
-      hipLaunchKernelGGL((mcxyz_kernel<TT, 256, TileT>), dim3(grid), dim3(256), lds, s, a);
+      hipLaunchKernelGGL((mcxyz_kernel<TT, 256, TileT, COUNT>), dim3(grid), dim3(256), lds, s, a);
       break;
     case 512:
-      hipLaunchKernelGGL((mcxyz_kernel<TT, 512, TileT>), dim3(grid), dim3(512), lds, s, a);
+      hipLaunchKernelGGL((mcxyz_kernel<TT, 512, TileT, COUNT>), dim3(grid), dim3(512), lds, s, a);
       break;
     default:
-      hipLaunchKernelGGL((mcxyz_kernel<TT, 1024, TileT>), dim3(grid), dim3(1024), lds, s, a);
+      hipLaunchKernelGGL((mcxyz_kernel<TT, 1024, TileT, COUNT>), dim3(grid), dim3(1024), lds, s, a);
       break;
   }
   return hipGetLastError();
 }
 
-template <bool TT, typename TileT>
+template <bool TT, typename TileT, bool COUNT>
 const void* kernel_of(int block) {
-  return block == 256   ? (const void*)mcxyz_kernel<TT, 256, TileT>
-         : block == 512 ? (const void*)mcxyz_kernel<TT, 512, TileT>
-                        : (const void*)mcxyz_kernel<TT, 1024, TileT>;
+  return block == 256   ? (const void*)mcxyz_kernel<TT, 256, TileT, COUNT>
+         : block == 512 ? (const void*)mcxyz_kernel<TT, 512, TileT, COUNT>
+                        : (const void*)mcxyz_kernel<TT, 1024, TileT, COUNT>;
 }
 
+template <bool COUNT>
 const void* kernel_for(bool two_term, bool tile32, int block) {
-  if (two_term) return tile32 ? kernel_of<true, float>(block) : kernel_of<true, double>(block);
-  return tile32 ? kernel_of<false, float>(block) : kernel_of<false, double>(block);
+  if (two_term)
+    return tile32 ? kernel_of<true, float, COUNT>(block) : kernel_of<true, double, COUNT>(block);
+  return tile32 ? kernel_of<false, float, COUNT>(block) : kernel_of<false, double, COUNT>(block);
+}
+
+template <bool COUNT>
+hipError_t launch_variant(const XyzArgs& a, bool two_term, bool tile32, int block, int grid,
+                          size_t lds_bytes, hipStream_t s) {
+  if (two_term)
+    return tile32 ? launch_block<true, float, COUNT>(a, block, grid, lds_bytes, s)
+                  : launch_block<true, double, COUNT>(a, block, grid, lds_bytes, s);
+  return tile32 ? launch_block<false, float, COUNT>(a, block, grid, lds_bytes, s)
+                : launch_block<false, double, COUNT>(a, block, grid, lds_bytes, s);
 }
 
 }  // namespace
 
+// The visit-counting variant (a.count_visits) is a kernel of its own, so the
+// transport loop carries no per-visit test of the mode.
 hipError_t launch_mcxyz(const XyzArgs& a, bool two_term, bool tile32, int block, int grid,
                         size_t lds_bytes, hipStream_t s) {
-  if (two_term)
-    return tile32 ? launch_block<true, float>(a, block, grid, lds_bytes, s)
-                  : launch_block<true, double>(a, block, grid, lds_bytes, s);
-  return tile32 ? launch_block<false, float>(a, block, grid, lds_bytes, s)
-                : launch_block<false, double>(a, block, grid, lds_bytes, s);
+  return a.count_visits ? launch_variant<true>(a, two_term, tile32, block, grid, lds_bytes, s)
+                        : launch_variant<false>(a, two_term, tile32, block, grid, lds_bytes, s);
 }
 
 hipError_t mcxyz_configure(bool two_term, bool tile32, int block, size_t lds_bytes) {
-  return hipFuncSetAttribute(kernel_for(two_term, tile32, block),
+  hipError_t e = hipFuncSetAttribute(kernel_for<false>(two_term, tile32, block),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+  if (e != hipSuccess) return e;
+  return hipFuncSetAttribute(kernel_for<true>(two_term, tile32, block),
                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
 }
 
+// Workgroups per CU that both variants reach.
 int mcxyz_blocks_per_cu(bool two_term, bool tile32, int block, size_t lds_bytes) {
-  int nb = 0;
+  int nb = 0, nc = 0;
   hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      &nb, kernel_for(two_term, tile32, block), block, lds_bytes);
-  return e == hipSuccess ? nb : 0;
+      &nb, kernel_for<false>(two_term, tile32, block), block, lds_bytes);
+  if (e == hipSuccess)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nc, kernel_for<true>(two_term, tile32, block),
+                                                     block, lds_bytes);
+  return e == hipSuccess ? (nb < nc ? nb : nc) : 0;
 }
 
 hipError_t launch_tile_reduce(const double* part, int grid, const XyzArgs& a, hipStream_t s) {
