@@ -63,23 +63,38 @@ __device__ __forceinline__ uint32_t in_vgpr(uint32_t s) {
   return v;
 }
 
-// The LDS tile box, in vector registers (in_vgpr).
+// A wave-uniform pointer in a pair of vector registers (see in_vgpr); the
+// atomics and loads that use it take a 64-bit vector address anyway.
+template <typename T>
+__device__ __forceinline__ T* ptr_in_vgpr(T* p) {
+  const uint64_t u = (uint64_t)p;
+  return (T*)(((uint64_t)in_vgpr((uint32_t)(u >> 32)) << 32) | in_vgpr((uint32_t)u));
+}
+
+// The LDS tile box and the fluence array, in vector registers (in_vgpr).
 struct TileBox {
   uint32_t x0, y0, z0, nx, ny, nz;
+  double* F;
 };
 
+// Fluence tally of a kernel variant: none (NMC_FLAG_NO_FLUENCE), the
+// deposited weight, or the number of voxel visits (the tile-placement pilot).
+// Each is a kernel of its own, so the loop carries no per-visit mode test.
+enum : int { kFluNone = 0, kFluDeposit = 1, kFluCount = 2 };
+
 // Adds the deposits of the current voxel visit to the tile or the global grid
-// (COUNT: adds 1 per visit instead, the tile-placement pilot's visit count).
-template <bool COUNT, typename TileT>
+// (kFluCount: adds 1 per visit instead).
+template <int FMODE, typename TileT>
 __device__ __forceinline__ void flush_visit(Photon& p, const XyzArgs& a, const TileBox& tb,
                                             TileT* tile) {
+  constexpr bool COUNT = FMODE == kFluCount;
   if (p.inside && (COUNT || p.vdep != 0.0)) {
     double v = COUNT ? 1.0 : p.vdep;
     uint32_t lx = (uint32_t)p.ix - tb.x0, ly = (uint32_t)p.iy - tb.y0, lz = (uint32_t)p.iz - tb.z0;
     if (lx < tb.nx && ly < tb.ny && lz < tb.nz)
       unsafeAtomicAdd(&tile[(lz * tb.nx + lx) * tb.ny + ly], (TileT)v);
     else
-      unsafeAtomicAdd(&a.F[vidx(a, p.ix, p.iy, p.iz)], v);
+      unsafeAtomicAdd(&tb.F[vidx(a, p.ix, p.iy, p.iz)], v);
   }
   p.vdep = 0.0;
 }
@@ -222,7 +237,7 @@ __device__ __forceinline__ void launch(Photon& p, const XyzArgs& a, const XyzTis
   p.alive = true;
 }
 
-template <bool TWO_TERM, int BLOCK, typename TileT, bool COUNT>
+template <bool TWO_TERM, int BLOCK, typename TileT, int FMODE>
 __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
   extern __shared__ double tile_mem[];
   TileT* tile = (TileT*)tile_mem;  // [tile_nz][tile_nx][tile_ny]
@@ -237,15 +252,23 @@ __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
   const uint64_t first = a.state->first;
   const TileBox tb{in_vgpr((uint32_t)a.tile_x0), in_vgpr((uint32_t)a.tile_y0),
                    in_vgpr((uint32_t)a.tile_z0), in_vgpr((uint32_t)a.tile_nx),
-                   in_vgpr((uint32_t)a.tile_ny), in_vgpr((uint32_t)a.tile_nz)};
-  // grid size, boundary and detector modes, likewise in vector registers; no
-  // fluence tally (NMC_FLAG_NO_FLUENCE): voxel visits are not flushed
+                   in_vgpr((uint32_t)a.tile_ny), in_vgpr((uint32_t)a.tile_nz),
+                   FMODE == kFluNone ? nullptr : ptr_in_vgpr(a.F)};
+  const uint8_t* const vmap = ptr_in_vgpr(a.V);
+  // grid size, boundary and detector modes, likewise in vector registers, and
+  // used in forms that leave no loop-invariant lane mask for the compiler to
+  // keep in scalar registers: the detector plane's side as a sign (-1: the
+  // reflectance plane, crossed going up; +1: the transmittance plane, crossed
+  // going down; multiplying by it is exact), and the escape rule of the
+  // boundary flag as a per-lane select (escapes through the top face, and
+  // through any face)
   const int gnx = (int)in_vgpr((uint32_t)a.Nx), gny = (int)in_vgpr((uint32_t)a.Ny),
             gnz = (int)in_vgpr((uint32_t)a.Nz);
-  const uint32_t bmode = in_vgpr((uint32_t)a.bmode);
-  const bool det_refl = in_vgpr(a.det_state == NMC_DET_REFLECTANCE ? 1u : 0u) != 0u;
+  const uint32_t esc_top = in_vgpr(a.bmode == 1 || a.bmode == 2 ? 1u : 0u);
+  const uint32_t esc_any = in_vgpr(a.bmode == 1 ? 1u : 0u);
+  const float dsign = __uint_as_float(
+      in_vgpr(__float_as_uint(a.det_state == NMC_DET_REFLECTANCE ? -1.0f : 1.0f)));
   const float zsurf = __uint_as_float(in_vgpr(__float_as_uint(a.zsurf)));
-  const bool tally_f = in_vgpr(a.F != nullptr ? 1u : 0u) != 0u;
 
   const int lane = lane_id();
   Photon p;
@@ -319,11 +342,8 @@ __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
       s_face = tz;
     }
     float s_surf = kInf;
-    if (det_refl) {
-      if (p.uz < 0.0f && p.z > zsurf) s_surf = fmaxf((zsurf - p.z) * p.ruz, 0.0f);
-    } else {
-      if (p.uz > 0.0f && p.z < zsurf) s_surf = fmaxf((zsurf - p.z) * p.ruz, 0.0f);
-    }
+    if (p.uz * dsign > 0.0f && (zsurf - p.z) * dsign > 0.0f)
+      s_surf = fmaxf((zsurf - p.z) * p.ruz, 0.0f);
     int kind;
     float s;
     if (s_int <= s_face && s_int <= s_surf) {
@@ -375,16 +395,16 @@ __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
         if (!(p.W > 0.0f)) alive = false;
       } else {
         // voxel face (kernels.metal:641-679)
-        if (tally_f) flush_visit<COUNT>(p, a, tb, tile);
+        if (FMODE != kFluNone) flush_visit<FMODE>(p, a, tb, tile);
         if (axis == 0) p.ix += p.ux > 0.0f ? 1 : -1;
         else if (axis == 1) p.iy += p.uy > 0.0f ? 1 : -1;
         else p.iz += p.uz > 0.0f ? 1 : -1;
         p.inside = p.ix >= 0 && p.ix < gnx && p.iy >= 0 && p.iy < gny && p.iz >= 0 && p.iz < gnz;
-        if (!p.inside && (bmode == 1u || (bmode == 2u && p.iz < 0))) {
+        if (!p.inside && (p.iz < 0 ? esc_top : esc_any) != 0u) {
           tally_escape(&st, a, p.iz, p.W);
           alive = false;
         } else {
-          p.tis = a.V[vidx(a, clampi(p.ix, gnx), clampi(p.iy, gny), clampi(p.iz, gnz))];
+          p.tis = vmap[vidx(a, clampi(p.ix, gnx), clampi(p.iy, gny), clampi(p.iz, gnz))];
           p.mua = T[p.tis].mua;
           p.mus = T[p.tis].mus;
           p.inv_mus = T[p.tis].inv_mus;
@@ -428,7 +448,7 @@ __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
       }
     }
     if (!alive) {
-      if (tally_f) flush_visit<COUNT>(p, a, tb, tile);
+      if (FMODE != kFluNone) flush_visit<FMODE>(p, a, tb, tile);
       p.alive = false;
     }
   }
@@ -464,73 +484,82 @@ __global__ void tile_reduce_kernel(const double* __restrict__ part, int grid, Xy
   }
 }
 
-template <bool TT, typename TileT, bool COUNT>
+template <bool TT, typename TileT, int FMODE>
 hipError_t launch_block(const XyzArgs& a, int block, int grid, size_t lds, hipStream_t s) {
   switch (block) {
     case 256:
-      hipLaunchKernelGGL((mcxyz_kernel<TT, 256, TileT, COUNT>), dim3(grid), dim3(256), lds, s, a);
+      hipLaunchKernelGGL((mcxyz_kernel<TT, 256, TileT, FMODE>), dim3(grid), dim3(256), lds, s, a);
       break;
     case 512:
-      hipLaunchKernelGGL((mcxyz_kernel<TT, 512, TileT, COUNT>), dim3(grid), dim3(512), lds, s, a);
+      hipLaunchKernelGGL((mcxyz_kernel<TT, 512, TileT, FMODE>), dim3(grid), dim3(512), lds, s, a);
       break;
     default:
-      hipLaunchKernelGGL((mcxyz_kernel<TT, 1024, TileT, COUNT>), dim3(grid), dim3(1024), lds, s, a);
+      hipLaunchKernelGGL((mcxyz_kernel<TT, 1024, TileT, FMODE>), dim3(grid), dim3(1024), lds, s, a);
       break;
   }
   return hipGetLastError();
 }
 
-template <bool TT, typename TileT, bool COUNT>
+template <bool TT, typename TileT, int FMODE>
 const void* kernel_of(int block) {
-  return block == 256   ? (const void*)mcxyz_kernel<TT, 256, TileT, COUNT>
-         : block == 512 ? (const void*)mcxyz_kernel<TT, 512, TileT, COUNT>
-                        : (const void*)mcxyz_kernel<TT, 1024, TileT, COUNT>;
+  return block == 256   ? (const void*)mcxyz_kernel<TT, 256, TileT, FMODE>
+         : block == 512 ? (const void*)mcxyz_kernel<TT, 512, TileT, FMODE>
+                        : (const void*)mcxyz_kernel<TT, 1024, TileT, FMODE>;
 }
 
-template <bool COUNT>
+template <int FMODE>
 const void* kernel_for(bool two_term, bool tile32, int block) {
   if (two_term)
-    return tile32 ? kernel_of<true, float, COUNT>(block) : kernel_of<true, double, COUNT>(block);
-  return tile32 ? kernel_of<false, float, COUNT>(block) : kernel_of<false, double, COUNT>(block);
+    return tile32 ? kernel_of<true, float, FMODE>(block) : kernel_of<true, double, FMODE>(block);
+  return tile32 ? kernel_of<false, float, FMODE>(block) : kernel_of<false, double, FMODE>(block);
 }
 
-template <bool COUNT>
+template <int FMODE>
 hipError_t launch_variant(const XyzArgs& a, bool two_term, bool tile32, int block, int grid,
                           size_t lds_bytes, hipStream_t s) {
   if (two_term)
-    return tile32 ? launch_block<true, float, COUNT>(a, block, grid, lds_bytes, s)
-                  : launch_block<true, double, COUNT>(a, block, grid, lds_bytes, s);
-  return tile32 ? launch_block<false, float, COUNT>(a, block, grid, lds_bytes, s)
-                : launch_block<false, double, COUNT>(a, block, grid, lds_bytes, s);
+    return tile32 ? launch_block<true, float, FMODE>(a, block, grid, lds_bytes, s)
+                  : launch_block<true, double, FMODE>(a, block, grid, lds_bytes, s);
+  return tile32 ? launch_block<false, float, FMODE>(a, block, grid, lds_bytes, s)
+                : launch_block<false, double, FMODE>(a, block, grid, lds_bytes, s);
+}
+
+const void* kernel_for_mode(int fmode, bool two_term, bool tile32, int block) {
+  return fmode == kFluNone    ? kernel_for<kFluNone>(two_term, tile32, block)
+         : fmode == kFluCount ? kernel_for<kFluCount>(two_term, tile32, block)
+                              : kernel_for<kFluDeposit>(two_term, tile32, block);
 }
 
 }  // namespace
 
-// The visit-counting variant (a.count_visits) is a kernel of its own, so the
-// transport loop carries no per-visit test of the mode.
 hipError_t launch_mcxyz(const XyzArgs& a, bool two_term, bool tile32, int block, int grid,
                         size_t lds_bytes, hipStream_t s) {
-  return a.count_visits ? launch_variant<true>(a, two_term, tile32, block, grid, lds_bytes, s)
-                        : launch_variant<false>(a, two_term, tile32, block, grid, lds_bytes, s);
+  if (!a.F) return launch_variant<kFluNone>(a, two_term, tile32, block, grid, lds_bytes, s);
+  if (a.count_visits)
+    return launch_variant<kFluCount>(a, two_term, tile32, block, grid, lds_bytes, s);
+  return launch_variant<kFluDeposit>(a, two_term, tile32, block, grid, lds_bytes, s);
 }
 
 hipError_t mcxyz_configure(bool two_term, bool tile32, int block, size_t lds_bytes) {
-  hipError_t e = hipFuncSetAttribute(kernel_for<false>(two_term, tile32, block),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
-  if (e != hipSuccess) return e;
-  return hipFuncSetAttribute(kernel_for<true>(two_term, tile32, block),
-                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+  for (int m = kFluNone; m <= kFluCount; ++m) {
+    hipError_t e = hipFuncSetAttribute(kernel_for_mode(m, two_term, tile32, block),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
-// Workgroups per CU that both variants reach.
+// Workgroups per CU that every fluence variant reaches.
 int mcxyz_blocks_per_cu(bool two_term, bool tile32, int block, size_t lds_bytes) {
-  int nb = 0, nc = 0;
-  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      &nb, kernel_for<false>(two_term, tile32, block), block, lds_bytes);
-  if (e == hipSuccess)
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nc, kernel_for<true>(two_term, tile32, block),
-                                                     block, lds_bytes);
-  return e == hipSuccess ? (nb < nc ? nb : nc) : 0;
+  int best = 0;
+  for (int m = kFluNone; m <= kFluCount; ++m) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel_for_mode(m, two_term, tile32, block),
+                                                     block, lds_bytes) != hipSuccess)
+      return 0;
+    best = m == kFluNone || nb < best ? nb : best;
+  }
+  return best;
 }
 
 hipError_t launch_tile_reduce(const double* part, int grid, const XyzArgs& a, hipStream_t s) {
