@@ -19,12 +19,48 @@ constexpr float kInf = __builtin_inff();
 // sequence).  Magnitudes outside [2^-125, 2^125] (zero, subnormals, huge
 // values, inf, NaN) take the division.  Equality with 1.0f / x for all 2^32
 // inputs is checked on the GPU (nmcp_rcp_check, tests/test_gpu_math.py).
-__device__ __forceinline__ float rcp_rn(float x) {
-  const float ax = fabsf(x);
-  if (!(ax >= 0x1p-125f && ax <= 0x1p125f)) return 1.0f / x;
+__device__ __forceinline__ float rcp_rn_mid(float x) {
   const float r = __builtin_amdgcn_rcpf(x);
   const float e = fmaf(-x, r, 1.0f);
   return fmaf(e, r, r);
+}
+
+__device__ __forceinline__ float rcp_rn(float x) {
+  const float ax = fabsf(x);
+  if (!(ax >= 0x1p-125f && ax <= 0x1p125f)) return 1.0f / x;
+  return rcp_rn_mid(x);
+}
+
+// a / b rounded to nearest, bit for bit what IEEE division gives, by
+// Markstein's correction (P. Markstein, IBM J. Res. Dev. 34(1):111-119, 1990):
+// with y = RN(1/b) (rcp_rn_mid) and q = RN(a y) within one ulp of a/b, the
+// residual r = a - b q is exact and RN(q + r y) = RN(a/b).  6 instructions
+// instead of the 11 of the division sequence, but no range handling: only for
+// operands with |b| in [2^-125, 2^125], and a = 0 or a, a/b and the residual
+// normal.  Each call site states why its operands qualify.  Checked against
+// the division on the GPU (nmcp_div_check, tests/test_gpu_math.py):
+// exhaustively on log_pos's operands, on every Henyey-Greenstein divisor of
+// 256 anisotropies, and on 2^32 pairs each of spin's operands and of general
+// operands in [2^-60, 2^60].
+__device__ __forceinline__ float div_rn(float a, float b) {
+  const float y = rcp_rn_mid(b);
+  const float q = a * y;
+  const float r = fmaf(-b, q, a);
+  return fmaf(r, y, q);
+}
+
+// sqrt(x) rounded to nearest, bit for bit sqrtf, for x = 0 or x in
+// [2^-100, 2^126]: the hardware square root (within 1 ulp) corrected to the
+// nearest by the signs of the residuals x - s' s of its two neighbours s'
+// (9 instructions instead of 15: no rescaling of tiny inputs, no special
+// values).  Checked against sqrtf on the GPU for every input of that domain
+// (nmcp_sqrt_check, tests/test_gpu_math.py).
+__device__ __forceinline__ float sqrt_rn(float x) {
+  const float s = __builtin_amdgcn_sqrtf(x);
+  const float sm = __uint_as_float(__float_as_uint(s) - 1u);
+  const float sp = __uint_as_float(__float_as_uint(s) + 1u);
+  const float t = fmaf(-sm, s, x) <= 0.0f ? sm : s;
+  return fmaf(-sp, s, x) > 0.0f ? sp : t;
 }
 
 // ---------------------------------------------------------------------------
@@ -111,7 +147,8 @@ __device__ __forceinline__ float log_pos(float x) {
     k += 1;
   }
   float f = m - 1.0f;
-  float s = f / (2.0f + f);
+  // f = 0 or |f| in [2^-24, 0.42], 2 + f in [1.7, 2.5]: div_rn's domain
+  float s = div_rn(f, 2.0f + f);
   float z = s * s;
   float R = z * fmaf(z, fmaf(z, fmaf(z, 0x1.c71c72p-3f, 0x1.24924ap-2f), 0x1.99999ap-2f),
                      0x1.555556p-1f);
@@ -244,7 +281,9 @@ __device__ __forceinline__ float hg_cos(const HGCoef& h, float u) {
   } else if (h.mode == 2) {
     ct = 1.0f;
   } else {
-    float t = h.b / (h.c + h.d * u);
+    // b = 1 - g^2 in [0, 1], c + d u in [1 - |g|, 1 + |g|] (|g| < 1 here: at
+    // least 2^-24 as floats), quotient in [1 - |g|, 1 + |g|]: div_rn's domain
+    float t = div_rn(h.b, h.c + h.d * u);
     ct = (h.a - t * t) * h.e;
   }
   return fminf(fmaxf(ct, -1.0f), 1.0f);
@@ -287,15 +326,19 @@ __device__ __forceinline__ void spin(float cost, float u_psi, float& ux, float& 
   sincos2pi(u_psi, &sp, &cp);
   float x = ux, y = uy, z = uz;
   if (fabsf(z) > 1.0f - 1.0e-6f) {
-    float sint = sqrtf(1.0f - cost * cost);
+    // 1 - cos^2 is 0 or at least 2^-24 (cos^2 <= 1 is a float): sqrt_rn's domain
+    float sint = sqrt_rn(1.0f - cost * cost);
     ux = sint * cp;
     uy = sint * sp;
     uz = z >= 0.0f ? cost : -cost;
   } else {
     // st = sin(theta) / sqrt(ux^2 + uy^2) as one square root; the uz term
     // sin(theta) * sqrt(ux^2 + uy^2) is st * (ux^2 + uy^2)
+    // r2 = 1 - uz^2 up to rounding, at least ~2e-6 here; 1 - cos^2 is 0 or in
+    // [2^-24, 1]: both in div_rn's domain, and the quotient (0 or at least
+    // 2^-24) in sqrt_rn's
     float r2 = x * x + y * y;
-    float st = sqrtf((1.0f - cost * cost) / r2);
+    float st = sqrt_rn(div_rn(1.0f - cost * cost, r2));
     ux = st * (x * z * cp - y * sp) + x * cost;
     uy = st * (y * z * cp + x * sp) + y * cost;
     uz = -(st * cp * r2) + z * cost;

@@ -6,7 +6,11 @@
 //   nmcp_copy         a plain streaming copy of known size, for scaling the
 //                     FETCH_SIZE / WRITE_SIZE counters to bytes;
 //   nmcp_rcp_check    the transport kernels' fast reciprocal against IEEE
-//                     division over every binary32 input.
+//                     division over every binary32 input;
+//   nmcp_div_check    their short division against IEEE division on the
+//                     operand domains of its call sites and on general pairs;
+//   nmcp_sqrt_check   their short square root against sqrtf over every input
+//                     of its domain.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -58,6 +62,87 @@ __global__ void __launch_bounds__(256) rcp_check_kernel(unsigned long long* bad,
     }
   }
   if (n) atomicAdd(bad, n);
+}
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+// Operands (a, b) of case i of div_rn's check domain `dom`:
+//   0  log_pos: every mantissa m of [sqrt2/2, sqrt2] as log_pos reduces it,
+//      (m - 1, 2 + (m - 1))                                    i < 2^23
+//   1  Henyey-Greenstein: g = -1 + (k + 1/2)/128, k = i >> 24 < 256, u = (i mod
+//      2^24) 2^-24: (1 - g^2, (1 - g) + 2g u)                    i < 2^32
+//   2  spin: (1 - c^2, 1 - w^2), c uniform in [-1, 1], w uniform in
+//      [-(1 - 1e-6), 1 - 1e-6] (hashed)                       i < 2^32
+//   3  general: sign, exponent in [-60, 60] and mantissa of a and b hashed
+//                                                              i < 2^32
+__device__ __forceinline__ void div_operands(int dom, uint64_t i, float* a, float* b) {
+  if (dom == 0) {
+    float m = __uint_as_float(0x3f800000u | (uint32_t)i);
+    if (m > 0x1.6a09e6p+0f) m = m * 0.5f;
+    const float f = m - 1.0f;
+    *a = f;
+    *b = 2.0f + f;
+  } else if (dom == 1) {
+    const float g = -1.0f + ((float)(i >> 24) + 0.5f) * 0x1p-7f;
+    const float u = (float)(uint32_t)(i & 0xFFFFFFu) * 0x1p-24f;
+    *a = 1.0f - g * g;
+    *b = (1.0f - g) + (2.0f * g) * u;
+  } else if (dom == 2) {
+    const uint64_t h = mix64(i);
+    const float c = 2.0f * ((float)(uint32_t)(h >> 40) * 0x1p-24f) - 1.0f;
+    const float w = (1.0f - 1.0e-6f) * (2.0f * ((float)(uint32_t)(h & 0xFFFFFFu) * 0x1p-24f) - 1.0f);
+    *a = 1.0f - c * c;
+    *b = 1.0f - w * w;
+  } else {
+    const uint64_t h = mix64(i);
+    const uint32_t ea = 67u + (uint32_t)((h >> 23) % 121u), eb = 67u + (uint32_t)((h >> 46) % 121u);
+    *a = __uint_as_float(((uint32_t)(h >> 63) << 31) | (ea << 23) | (uint32_t)(h & 0x7FFFFFu));
+    const uint64_t h2 = mix64(h);
+    *b = __uint_as_float(((uint32_t)(h2 >> 63) << 31) | (eb << 23) | (uint32_t)(h2 & 0x7FFFFFu));
+  }
+}
+
+// Counts the cases of domain `dom` whose div_rn(a, b) differs in bits from
+// a / b; keeps the smallest differing case index.
+__global__ void __launch_bounds__(256) div_check_kernel(int dom, uint64_t n, unsigned long long* bad,
+                                                        unsigned long long* first_bad) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  unsigned long long cnt = 0;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += stride) {
+    float a, b;
+    div_operands(dom, i, &a, &b);
+    const float f = nmc::div_rn(a, b), q = a / b;
+    if (__float_as_uint(f) != __float_as_uint(q)) {
+      ++cnt;
+      atomicMin(first_bad, (unsigned long long)i);
+    }
+  }
+  if (cnt) atomicAdd(bad, cnt);
+}
+
+// Counts the inputs x = +0 or x in [2^-100, 2^126] (every such bit pattern)
+// whose sqrt_rn(x) differs in bits from sqrtf(x); keeps the smallest.
+__global__ void __launch_bounds__(256) sqrt_check_kernel(unsigned long long* bad,
+                                                         unsigned int* first_bad) {
+  const uint32_t lo = __float_as_uint(0x1p-100f), hi = __float_as_uint(0x1p126f);
+  const uint64_t n = (uint64_t)(hi - lo) + 2u;   // [lo, hi] and +0
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  unsigned long long cnt = 0;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += stride) {
+    const uint32_t bits = i + 1 == n ? 0u : lo + (uint32_t)i;
+    const float x = __uint_as_float(bits);
+    const float f = nmc::sqrt_rn(x), q = sqrtf(x);
+    if (__float_as_uint(f) != __float_as_uint(q)) {
+      ++cnt;
+      atomicMin(first_bad, bits);
+    }
+  }
+  if (cnt) atomicAdd(bad, cnt);
 }
 
 }  // namespace
@@ -136,6 +221,59 @@ extern "C" int nmcp_rcp_check(unsigned long long* mismatches, unsigned int* firs
   if (hipMemset(d_bad, 0, sizeof(*d_bad)) == hipSuccess &&
       hipMemcpy(d_first, &none, sizeof(none), hipMemcpyHostToDevice) == hipSuccess) {
     hipLaunchKernelGGL(rcp_check_kernel, dim3(kGrid), dim3(kBlock), 0, 0, d_bad, d_first);
+    if (hipGetLastError() == hipSuccess && hipDeviceSynchronize() == hipSuccess &&
+        hipMemcpy(mismatches, d_bad, sizeof(*d_bad), hipMemcpyDeviceToHost) == hipSuccess &&
+        hipMemcpy(first, d_first, sizeof(*first), hipMemcpyDeviceToHost) == hipSuccess)
+      rc = 0;
+  }
+  hipFree(d_bad);
+  hipFree(d_first);
+  return rc;
+}
+
+// div_rn against IEEE division on check domain `domain` (0..3, see
+// div_operands); returns 0 and fills *cases, *mismatches and *first (the
+// smallest differing case index, ~0 if none).
+extern "C" int nmcp_div_check(int domain, unsigned long long* cases, unsigned long long* mismatches,
+                              unsigned long long* first) {
+  if (domain < 0 || domain > 3 || !cases || !mismatches || !first) return -1;
+  const uint64_t n = domain == 0 ? (1ull << 23) : (1ull << 32);
+  unsigned long long* d = nullptr;   // [mismatches, first]
+  if (hipMalloc(&d, 2 * sizeof(*d)) != hipSuccess) return -4;
+  int rc = -3;
+  const unsigned long long init[2] = {0ull, ~0ull};
+  if (hipMemcpy(d, init, sizeof(init), hipMemcpyHostToDevice) == hipSuccess) {
+    hipLaunchKernelGGL(div_check_kernel, dim3(kGrid), dim3(kBlock), 0, 0, domain, n, d, d + 1);
+    unsigned long long out[2];
+    if (hipGetLastError() == hipSuccess && hipDeviceSynchronize() == hipSuccess &&
+        hipMemcpy(out, d, sizeof(out), hipMemcpyDeviceToHost) == hipSuccess) {
+      *cases = n;
+      *mismatches = out[0];
+      *first = out[1];
+      rc = 0;
+    }
+  }
+  hipFree(d);
+  return rc;
+}
+
+// sqrt_rn against sqrtf on every input of its domain; returns 0 and fills
+// *mismatches and *first (the smallest differing bit pattern, 0xFFFFFFFF if
+// none).
+extern "C" int nmcp_sqrt_check(unsigned long long* mismatches, unsigned int* first) {
+  if (!mismatches || !first) return -1;
+  unsigned long long* d_bad = nullptr;
+  unsigned int* d_first = nullptr;
+  if (hipMalloc(&d_bad, sizeof(*d_bad)) != hipSuccess) return -4;
+  if (hipMalloc(&d_first, sizeof(*d_first)) != hipSuccess) {
+    hipFree(d_bad);
+    return -4;
+  }
+  int rc = -3;
+  const unsigned int none = 0xFFFFFFFFu;
+  if (hipMemset(d_bad, 0, sizeof(*d_bad)) == hipSuccess &&
+      hipMemcpy(d_first, &none, sizeof(none), hipMemcpyHostToDevice) == hipSuccess) {
+    hipLaunchKernelGGL(sqrt_check_kernel, dim3(kGrid), dim3(kBlock), 0, 0, d_bad, d_first);
     if (hipGetLastError() == hipSuccess && hipDeviceSynchronize() == hipSuccess &&
         hipMemcpy(mismatches, d_bad, sizeof(*d_bad), hipMemcpyDeviceToHost) == hipSuccess &&
         hipMemcpy(first, d_first, sizeof(*first), hipMemcpyDeviceToHost) == hipSuccess)

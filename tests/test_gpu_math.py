@@ -1,6 +1,12 @@
-"""GPU: the transport kernels' fast reciprocal equals IEEE division on every
-binary32 input, so replacing 1.0f / x by it keeps GPU trajectories identical
-to the oracle's (which divides)."""
+"""GPU: the transport kernels' short reciprocal, division and square root give
+the IEEE results bit for bit, so using them keeps GPU trajectories identical
+to the oracle's (which divides and takes square roots the IEEE way).
+
+- reciprocal: every binary32 input;
+- division: every operand pair of log_pos, every Henyey-Greenstein divisor of
+  256 anisotropies, 2^32 pairs shaped like the rotation's, 2^32 general pairs
+  with exponents in [-60, 60];
+- square root: every input in [2^-100, 2^126], and +0."""
 from __future__ import annotations
 
 import ctypes as C
@@ -19,4 +25,23 @@ def test_rcp_rn_equals_division_for_all_inputs(gpu):
     L.nmcp_rcp_check.argtypes = [C.POINTER(C.c_ulonglong), C.POINTER(C.c_uint)]
     bad, first = C.c_ulonglong(), C.c_uint()
     assert L.nmcp_rcp_check(C.byref(bad), C.byref(first)) == 0
+    assert bad.value == 0, f"{bad.value} inputs differ, first 0x{first.value:08x}"
+
+
+@pytest.mark.parametrize("domain,name", [(0, "log_pos operands"), (1, "Henyey-Greenstein divisors"),
+                                         (2, "rotation operands"), (3, "general pairs")])
+def test_div_rn_equals_division(gpu, domain, name):
+    L = C.CDLL(PROBE_LIB)
+    L.nmcp_div_check.argtypes = [C.c_int] + [C.POINTER(C.c_ulonglong)] * 3
+    cases, bad, first = C.c_ulonglong(), C.c_ulonglong(), C.c_ulonglong()
+    assert L.nmcp_div_check(domain, C.byref(cases), C.byref(bad), C.byref(first)) == 0
+    assert cases.value >= 1 << 23
+    assert bad.value == 0, f"{name}: {bad.value} of {cases.value} differ, first case {first.value}"
+
+
+def test_sqrt_rn_equals_sqrtf_on_its_domain(gpu):
+    L = C.CDLL(PROBE_LIB)
+    L.nmcp_sqrt_check.argtypes = [C.POINTER(C.c_ulonglong), C.POINTER(C.c_uint)]
+    bad, first = C.c_ulonglong(), C.c_uint()
+    assert L.nmcp_sqrt_check(C.byref(bad), C.byref(first)) == 0
     assert bad.value == 0, f"{bad.value} inputs differ, first 0x{first.value:08x}"
