@@ -70,6 +70,13 @@ __device__ __forceinline__ float sqrt_rn(float x) {
 // ---------------------------------------------------------------------------
 struct U4 { uint32_t x, y, z, w; };
 
+// a ^ b ^ c in one instruction: CDNA4's three-input bitwise operation
+// (v_bitop3_b32) with the truth table of XOR over its operand patterns
+// 0xF0, 0xCC, 0xAA, i.e. 0x96; the compiler does not fuse the two XORs itself.
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
+  return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+}
+
 __device__ __forceinline__ U4 philox(uint64_t pid, uint32_t ev, uint32_t tag, uint32_t k0,
                                      uint32_t k1) {
   uint32_t c0 = (uint32_t)pid, c1 = (uint32_t)(pid >> 32), c2 = ev, c3 = tag;
@@ -77,8 +84,8 @@ __device__ __forceinline__ U4 philox(uint64_t pid, uint32_t ev, uint32_t tag, ui
   for (int r = 0; r < 10; ++r) {
     uint64_t p0 = (uint64_t)0xD2511F53u * c0;
     uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
-    uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
-    uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+    uint32_t n0 = xor3((uint32_t)(p1 >> 32), c1, k0);
+    uint32_t n2 = xor3((uint32_t)(p0 >> 32), c3, k1);
     c1 = (uint32_t)p1;
     c3 = (uint32_t)p0;
     c0 = n0;
@@ -109,8 +116,8 @@ __device__ __forceinline__ U4 philox(uint64_t pid, uint32_t ev, uint32_t tag, ui
   for (int r = 0; r < 10; ++r) {
     uint64_t p0 = (uint64_t)0xD2511F53u * c0;
     uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
-    uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
-    uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ ks.k1[r];
+    uint32_t n0 = xor3((uint32_t)(p1 >> 32), c1, k0);
+    uint32_t n2 = xor3((uint32_t)(p0 >> 32), c3, ks.k1[r]);
     c1 = (uint32_t)p1;
     c3 = (uint32_t)p0;
     c0 = n0;
