@@ -78,9 +78,13 @@ __device__ __forceinline__ void fold_layer(LPhoton& p, double* absorbed) {
 // last call (all lanes of the wave call it together).  Slots come from a
 // wave-private run [next, end) of kRecChunk slots reserved with one global
 // atomic; a flush that overruns the run continues in a fresh one.
+// rec_nl and rec are the record width and buffer in vector registers
+// (in_vgpr); the per-layer slot tests below then cost a compare each instead
+// of a scalar-register mask kept across the loop.
 __device__ __forceinline__ void flush_records(LPhoton& p, const LayArgs& a, int lane,
                                               uint64_t& next, uint64_t& end,
-                                              uint32_t& written) {
+                                              uint32_t& written, uint32_t rec_nl_v,
+                                              uint32_t* rec) {
   uint64_t m = ballot(p.rec_pending);
   if (!m) return;
   uint64_t c = (uint64_t)__popcll(m);
@@ -94,16 +98,16 @@ __device__ __forceinline__ void flush_records(LPhoton& p, const LayArgs& a, int 
     uint64_t rank = (uint64_t)__popcll(m & lanemask_lt());
     uint64_t k = rank < rem ? next + rank : fresh + (rank - rem);
     if (k < a.cap) {
-      const int rs = 3 + 2 * a.rec_nl;
-      uint32_t* r = a.rec + k * (uint64_t)rs;
+      const uint32_t nl = fresh_vgpr(rec_nl_v);
+      uint32_t* r = rec + k * (uint64_t)(3u + 2u * nl);
       r[0] = (uint32_t)p.item;
       r[1] = (uint32_t)p.rec_ir;
       r[2] = __float_as_uint(p.W);
 #pragma unroll
-      for (int i = 0; i < NMC_MAX_LAYERS; ++i) {
-        if (i < a.rec_nl) {
+      for (uint32_t i = 0; i < NMC_MAX_LAYERS; ++i) {
+        if (i < nl) {
           r[3 + i] = __float_as_uint(p.L[i]);
-          r[3 + a.rec_nl + i] = p.K[i];
+          r[3 + nl + i] = p.K[i];
         }
       }
       written++;
@@ -118,19 +122,27 @@ __device__ __forceinline__ void flush_records(LPhoton& p, const LayArgs& a, int 
   }
 }
 
+// RD_LDS: Rd(r) in an LDS histogram (else the block's global replica);
+// RECORDS: photons leaving through the top write path records.  Both are
+// kernel variants, so the loop holds no mask for them.
+template <bool RD_LDS, bool RECORDS>
 __global__ __launch_bounds__(kLayBlock) void layered_kernel(LayArgs a) {
   extern __shared__ double smem[];
-  // [items] absorbed[MAXL] | [items] totals | [items][nr+1] Rd (rd_lds only) |
+  // [items] absorbed[MAXL] | [items] totals | [items][nr+1] Rd (RD_LDS only) |
   // [items] headers | [items+1] prefix | [rows]
   const int ni = a.n_items;
-  const int rd_len = a.rd_lds ? ni * (a.nr + 1) : 0;
+  const int rd_len = RD_LDS ? ni * (a.nr + 1) : 0;
   double* absorbed = smem;
   nmc_totals* st = (nmc_totals*)(absorbed + ni * NMC_MAX_LAYERS);
   double* rd_tile = (double*)(st + ni);
   LayStackDev* hdr = (LayStackDev*)(rd_tile + rd_len);
   uint64_t* prefix = (uint64_t*)(hdr + ni);
   LayRow* rows = (LayRow*)(prefix + ni + 1);
-  double* rd_rep = a.rd + (int64_t)(blockIdx.x % (unsigned)a.rd_reps) * ni * (a.nr + 1);
+  double* rd_rep = RD_LDS ? nullptr
+                          : ptr_in_vgpr(a.rd + (int64_t)(blockIdx.x % (unsigned)a.rd_reps) * ni * (a.nr + 1));
+  const uint32_t rd_row = in_vgpr((uint32_t)(a.nr + 1));
+  const uint32_t rec_nl_v = in_vgpr((uint32_t)a.rec_nl);
+  uint32_t* const rec_v = RECORDS ? ptr_in_vgpr(a.rec) : nullptr;
   for (int i = threadIdx.x; i < ni * (NMC_MAX_LAYERS + TF_COUNT + TI_COUNT) + rd_len; i += kLayBlock)
     smem[i] = 0.0;
   for (int i = threadIdx.x; i < ni; i += kLayBlock) hdr[i] = a.stacks[i];
@@ -155,7 +167,7 @@ __global__ __launch_bounds__(kLayBlock) void layered_kernel(LayArgs a) {
   uint32_t rec_written = 0;
 
   for (;;) {
-    if (a.records) flush_records(p, a, lane, rec_next, rec_end, rec_written);
+    if (RECORDS) flush_records(p, a, lane, rec_next, rec_end, rec_written, rec_nl_v, rec_v);
     uint64_t idle = ballot(!p.alive && !finished);
     if (idle && ((uint32_t)__popcll(idle) >= a.refill_min || ballot(p.alive) == 0ull)) {
       while (idle) {
@@ -270,13 +282,13 @@ __global__ __launch_bounds__(kLayBlock) void layered_kernel(LayArgs a) {
         // the bins near r = 0 take a large share of all exits: an LDS
         // histogram (or a per-XCD replica) keeps them from serialising the
         // whole chip on a few global addresses
-        if (a.rd_lds)
-          unsafeAtomicAdd(&rd_tile[p.item * (a.nr + 1) + ir], (double)p.W);
+        if (RD_LDS)
+          unsafeAtomicAdd(&rd_tile[(uint32_t)p.item * rd_row + (uint32_t)ir], (double)p.W);
         else
-          unsafeAtomicAdd(&rd_rep[(int64_t)p.item * (a.nr + 1) + ir], (double)p.W);
+          unsafeAtomicAdd(&rd_rep[(uint32_t)p.item * rd_row + (uint32_t)ir], (double)p.W);
         addf(T, TF_RD, (double)p.W);
         addi(T, TI_DETECTIONS, 1);
-        if (a.records) {
+        if (RECORDS) {
           p.rec_pending = true;
           p.rec_ir = ir;
         }
@@ -342,7 +354,7 @@ __global__ __launch_bounds__(kLayBlock) void layered_kernel(LayArgs a) {
     }
   }
 
-  if (a.records) {
+  if (RECORDS) {
     // the unused tail of the wave's slot run is marked as holes; the count of
     // written records goes out with one atomic per wave
     const int rs = 3 + 2 * a.rec_nl;
@@ -364,7 +376,7 @@ __global__ __launch_bounds__(kLayBlock) void layered_kernel(LayArgs a) {
   __syncthreads();
   double* part = a.part + (int64_t)blockIdx.x * per;
   for (int i = threadIdx.x; i < per; i += kLayBlock) part[i] = smem[i];
-  if (a.rd_lds) {
+  if (RD_LDS) {
     double* rp = a.rd + (int64_t)blockIdx.x * rd_len;
     for (int i = threadIdx.x; i < rd_len; i += kLayBlock) rp[i] = rd_tile[i];
   }
@@ -437,20 +449,44 @@ size_t layered_lds_bytes(int n_items, int n_rows, int rd_len) {
          (size_t)n_rows * sizeof(LayRow);
 }
 
+namespace {
+const void* layered_variant(int v) {
+  switch (v) {
+    case 0: return (const void*)layered_kernel<false, false>;
+    case 1: return (const void*)layered_kernel<false, true>;
+    case 2: return (const void*)layered_kernel<true, false>;
+    default: return (const void*)layered_kernel<true, true>;
+  }
+}
+}  // namespace
+
 hipError_t layered_configure(size_t lds_bytes) {
-  return hipFuncSetAttribute((const void*)layered_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                             (int)lds_bytes);
+  for (int v = 0; v < 4; ++v) {
+    hipError_t e = hipFuncSetAttribute(layered_variant(v), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)lds_bytes);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
+// Workgroups per CU that every variant reaches.
 int layered_blocks_per_cu(size_t lds_bytes) {
-  int nb = 0;
-  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, layered_kernel, kLayBlock, lds_bytes);
-  return e == hipSuccess ? nb : 0;
+  int best = 0;
+  for (int v = 0; v < 4; ++v) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, layered_variant(v), kLayBlock, lds_bytes) !=
+        hipSuccess)
+      return 0;
+    best = v == 0 || nb < best ? nb : best;
+  }
+  return best;
 }
 
 hipError_t launch_layered(const LayArgs& a, int grid, size_t lds_bytes, hipStream_t s) {
-  hipLaunchKernelGGL(layered_kernel, dim3(grid), dim3(kLayBlock), lds_bytes, s, a);
-  return hipGetLastError();
+  const int v = (a.rd_lds ? 2 : 0) + (a.records ? 1 : 0);
+  LayArgs arg = a;
+  void* args[] = {&arg};
+  return hipLaunchKernel(layered_variant(v), dim3(grid), dim3(kLayBlock), args, lds_bytes, s);
 }
 
 hipError_t launch_layered_backward(const LayBackArgs& b, hipStream_t s) {

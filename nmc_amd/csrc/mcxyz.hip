@@ -52,25 +52,6 @@ __device__ __forceinline__ void lds_addf(nmc_totals* t, int k, double v) {
   unsafeAtomicAdd(&((double*)t)[k], v);
 }
 
-// A wave-uniform value kept in a vector register.  The transport loop uses
-// every scalar register the hardware gives a wave, so the compiler would
-// otherwise park such values in lanes of a vector register and fetch them
-// back (one v_readlane each) every time the voxel-flush code runs; a copy per
-// lane is free here, as the LDS tile already limits the CU to one workgroup.
-__device__ __forceinline__ uint32_t in_vgpr(uint32_t s) {
-  uint32_t v;
-  asm volatile("v_mov_b32 %0, %1" : "=v"(v) : "s"(s));
-  return v;
-}
-
-// A wave-uniform pointer in a pair of vector registers (see in_vgpr); the
-// atomics and loads that use it take a 64-bit vector address anyway.
-template <typename T>
-__device__ __forceinline__ T* ptr_in_vgpr(T* p) {
-  const uint64_t u = (uint64_t)p;
-  return (T*)(((uint64_t)in_vgpr((uint32_t)(u >> 32)) << 32) | in_vgpr((uint32_t)u));
-}
-
 // The LDS tile box and the fluence array, in vector registers (in_vgpr).
 struct TileBox {
   uint32_t x0, y0, z0, nx, ny, nz;

@@ -352,6 +352,34 @@ __device__ __forceinline__ void spin(float cost, float u_psi, float& ux, float& 
   }
 }
 
+// A wave-uniform value kept in a vector register.  The transport loops use
+// every scalar register the hardware gives a wave, so the compiler would
+// otherwise park loop-invariant values in lanes of a vector register and
+// fetch each back with a v_readlane (a VALU instruction) wherever the loop
+// uses it; a copy per lane costs one v_mov at kernel start and a vector
+// register, of which these kernels have spare ones at their occupancy.
+__device__ __forceinline__ uint32_t in_vgpr(uint32_t s) {
+  uint32_t v;
+  asm volatile("v_mov_b32 %0, %1" : "=v"(v) : "s"(s));
+  return v;
+}
+
+// A wave-uniform pointer in a pair of vector registers (see in_vgpr); the
+// atomics and loads that use it take a 64-bit vector address anyway.
+template <typename T>
+__device__ __forceinline__ T* ptr_in_vgpr(T* p) {
+  const uint64_t u = (uint64_t)p;
+  return (T*)(((uint64_t)in_vgpr((uint32_t)(u >> 32)) << 32) | in_vgpr((uint32_t)u));
+}
+
+// The same value, redefined for the compiler at this point: comparisons with
+// it are evaluated where they are used instead of being hoisted out of the
+// loop as lane masks, which would occupy (or spill) scalar registers.
+__device__ __forceinline__ uint32_t fresh_vgpr(uint32_t v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+
 // ---------------------------------------------------------------------------
 // Wave helpers (wave64).
 // ---------------------------------------------------------------------------
