@@ -14,6 +14,10 @@
 // Launch model as mcxyz.hip: persistent wave64 workgroups, one photon per
 // lane, batched lane refills from a device queue spanning every stack's
 // budget (queue index -> stack by binary search over budget prefix sums).
+// The Rd(r) placement (LDS histogram or global replica) and the path records
+// are kernel variants, and the record width and the Rd / record pointers sit
+// in vector registers (in_vgpr), so the loop keeps no masks or parked values
+// in scalar registers.
 #include <algorithm>
 
 #include "nmc_internal.h"

@@ -20,7 +20,11 @@
 // escapes, roulette) go to LDS totals; frequent counters stay in registers
 // until the workgroup ends.  The source and detector-plane code is kept out
 // of line (source_photon, detector_plane) with its parameters in device
-// memory, so the loop's scalar registers hold only what the loop uses.
+// memory, the loop-invariant values the loop tests or indexes with (tile box,
+// grid size, boundary and detector modes, fluence and voxel-map pointers) sit
+// in vector registers (in_vgpr), and the fluence mode (none, deposit, visit
+// count) is a kernel variant: the scalar registers, which the Philox key
+// schedule largely fills, then hold no parked values the loop must fetch.
 #include "nmc_internal.h"
 
 namespace nmc {
