@@ -143,7 +143,7 @@ def tile_leg(mc, hist, n: int, mode: int, steps: int = 2) -> dict:
         ms.append(mc.last_kernel_ms())
     avg = sum(ms) / len(ms)
     elem = "float" if mode else "double"
-    return {"kernel": f"mcxyz_kernel<false, 1024, {elem}>", "tile": list(box),
+    return {"kernel": f"mcxyz_kernel<false, 1024, {elem}, 1>", "tile": list(box),
             "lds_tile_fraction": box_coverage(hist, box), "kernel_ms": avg,
             "photons_per_s": n / (avg * 1e-3),
             "fluence_parity": ("integer tallies bit-exact; fluence within 2e-5 relative of the "
@@ -313,7 +313,9 @@ def main() -> int:
 
     roof = {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_T,
             "unit": "Tlane-op/s", "frac": achieved / VALU_PEAK_T, "traffic": None,
-            "kernel": f"mcxyz_kernel<false, {eff.block}, {'float' if eff.mode else 'double'}>",
+            # template arguments: two-term phase function, block, tile element,
+            # fluence mode (1 = deposits), as in a kernel trace
+            "kernel": f"mcxyz_kernel<false, {eff.block}, {'float' if eff.mode else 'double'}, 1>",
             "kernel_ms": avg_ms,
             "ops_per_photon": ops / tot["photons"],
             "events_per_photon": {"same": tot["substeps_same"] / tot["photons"],
