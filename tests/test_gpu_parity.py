@@ -90,6 +90,18 @@ def test_visit_counts_bit_exact(gpu):
     assert got.fluence.sum() == t["substeps_cross"] - t["surface_hits"] + ends_inside
 
 
+@pytest.mark.parametrize("two_term", [False, True])
+def test_no_fluence_variant(gpu, two_term):
+    """NMC_FLAG_NO_FLUENCE runs the kernel variant that flushes no voxel
+    visits: every other tally (totals, R(x,y), detections) is the oracle's."""
+    w = W.tiny_voxel_case("two_term", n_photons=8000) if two_term else W.skin_vessel()
+    _say(f"[parity] no-fluence variant two_term={two_term} n=8000")
+    ref = O.run(w, 8000, threads=8, fluence=False)
+    got = simulate(w, 8000, fluence=False)
+    assert got.fluence is None
+    assert_same_tallies(got, ref, "no-fluence")
+
+
 def test_fp32_tile(gpu):
     """FP32 LDS tile (tuning.mode 1): integer tallies bit-exact; fluence within
     FP32 summation error of the oracle (sums of one workgroup's visits in FP32,
