@@ -211,7 +211,9 @@ typedef struct nmc_tuning {
                                per-item tallies fit), 1 LDS, 2 global atomics;
                                mcxyz: LDS tile element, 0 FP64, 1 FP32 (twice
                                the voxels; sums within a workgroup in FP32,
-                               across workgroups in FP64);
+                               across workgroups in FP64; a run of more than
+                               1e8 photons becomes launches of at most 1e8,
+                               the same photons);
                                layered: Rd(r) placement, read at
                                nmc_layered_prepare, 0 or 1 LDS histogram when
                                it fits without lowering occupancy, 2 global

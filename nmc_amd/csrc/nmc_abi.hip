@@ -377,11 +377,14 @@ uint32_t pick_chunk(uint64_t n, int waves, int req) {
 uint32_t key0_of(const nmc_handle* h) { return (uint32_t)h->ext.seed; }
 uint32_t key1_of(const nmc_handle* h, uint32_t stream) { return (uint32_t)(h->ext.seed >> 32) + stream; }
 
+// Photons per mcxyz launch with the FP32 LDS tile (tuning.mode 1).
+constexpr uint64_t kFp32TileLaunch = 100000000ull;
+
 // Enqueues one run (no events, no capture handling).
 int enqueue_run(nmc_handle* h, uint64_t n) {
   hipStream_t s = h->stream;
-  HIP_TRY(launch_state_begin(h->dState, s));
   if (h->sim == NMC_SIM_MCSUB) {
+    HIP_TRY(launch_state_begin(h->dState, s));
     const RunParams& rp = h->rp;
     const nmc_ext_config& e = h->ext;
     SubArgs a{};
@@ -428,6 +431,19 @@ int enqueue_run(nmc_handle* h, uint64_t n) {
     for (int j = 0; j < h->n_items; ++j) adv = std::max<uint64_t>(adv, h->prefix[j + 1] - h->prefix[j]);
     HIP_TRY(launch_state_advance(h->dState, adv, s));
   } else {
+    // With the FP32 tile each workgroup sums its share of a launch's deposits
+    // in FP32; runs are split into launches of at most kFp32TileLaunch
+    // photons (each flushes its tile into the FP64 array), which bounds
+    // those sums to the size the FP32 tile's accuracy was measured at.  The
+    // launches take consecutive photon ids, so the photons are the same.
+    if (h->tune.mode == 1 && n > kFp32TileLaunch) {
+      for (uint64_t done = 0; done < n; done += kFp32TileLaunch) {
+        const uint64_t part = std::min<uint64_t>(kFp32TileLaunch, n - done);
+        NMC_TRY(enqueue_run(h, part));
+      }
+      return ok();
+    }
+    HIP_TRY(launch_state_begin(h->dState, s));
     const RunParams& rp = h->rp;
     const nmc_ext_config& e = h->ext;
     XyzArgs a{};

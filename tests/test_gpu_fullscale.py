@@ -9,9 +9,10 @@ import sys
 import numpy as np
 import pytest
 
+from nmc_amd import params as P
 from nmc_amd import workloads as W
 from nmc_amd.layered import LayeredMC
-from nmc_amd.metal_mc import simulate
+from nmc_amd.metal_mc import MetalMC, simulate
 from oracle import oracle as O
 
 from parity_util import energy_balance
@@ -124,3 +125,28 @@ def test_multilayer_1e7_properties(gpu):
     assert rd.sum() == pytest.approx(t["rd"], rel=1e-9)
     assert ab.sum() == pytest.approx(t["absorbed"], rel=1e-9)
     assert nrec == t["detections"]
+
+
+def test_fp32_tile_long_run_is_split(gpu):
+    """A run above 1e8 photons with the FP32 tile goes out as launches of at
+    most 1e8 photons: the same photons as one FP64-tile launch (integer
+    tallies equal), fluence within the FP32 tile's bound."""
+    w = W.skin_vessel()
+    n = 100_000_000 + 3_000_000
+    _say(f"[fullscale] fp32 tile, one run of {n} photons")
+    out = []
+    for mode in (1, 0):
+        with MetalMC(w.sim) as mc:
+            mc.prepare_data(w.rp, w.tp, w.v, w.ext)
+            mc.set_tuning(P.Tuning.make(mode=mode))
+            mc.set_num_runs_seed(1, 0)
+            mc.send_compute_command(n)
+            out.append(mc.get_compute_results())
+    f32, f64 = out
+    for k in P.TOTAL_INT_FIELDS:
+        assert f32.totals[k] == f64.totals[k], k
+    assert f32.totals["photons"] == n
+    assert np.array_equal(f32.det2d, f64.det2d)
+    # FP32 sums of one workgroup's share of a <= 1e8-photon launch
+    np.testing.assert_allclose(f32.fluence, f64.fluence, rtol=1e-4, atol=1e-12 * f64.fluence.max())
+    assert abs(f32.fluence.sum() - f64.fluence.sum()) <= 1e-6 * f64.fluence.sum()
