@@ -218,12 +218,22 @@ typedef struct nmc_tuning {
                                nmc_layered_prepare, 0 or 1 LDS histogram when
                                it fits without lowering occupancy, 2 global
                                replicas (effective value reported as 1 / 2)   */
-    int32_t reserved[5];
+    int32_t split;          /* mcxyz with the FP32 tile: photons per workgroup
+                               in one launch (0: 390625, i.e. 1e8 photons on
+                               one 1024-lane workgroup per CU of 256 CUs); a
+                               longer run becomes consecutive launches of the
+                               same photons, each flushing its tile in FP64  */
+    int32_t reserved[4];
 } nmc_tuning;
 
+/* A rejected request (NMC_ERR_ARG, or an allocation failure) leaves the
+ * handle's previous configuration and buffers unchanged. */
 int nmc_set_tuning(nmc_handle* h, const nmc_tuning* t);
 /* Effective values after defaults and clamping (tile resolved to a box). */
 int nmc_get_tuning(nmc_handle* h, nmc_tuning* t);
+/* Transport-kernel launches of the last run (no reference counterpart; more
+ * than one when an FP32-tile mcxyz run is split by tuning.split). */
+int nmc_last_launch_count(nmc_handle* h, uint32_t* n);
 
 /*
  * Batched mcsub: many media (e.g. wavelengths), each with its own optical

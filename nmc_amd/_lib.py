@@ -50,6 +50,7 @@ def lib():
         "nmc_version": (C.c_char_p, []),
         "nmc_set_tuning": (C.c_int, [H, C.POINTER(P.Tuning)]),
         "nmc_get_tuning": (C.c_int, [H, C.POINTER(P.Tuning)]),
+        "nmc_last_launch_count": (C.c_int, [H, C.POINTER(C.c_uint32)]),
         "nmc_prepare_batch": (C.c_int, [H, rp, ext, C.POINTER(P.WorkItem), C.c_int]),
         "nmc_run_batch": (C.c_int, [H]),
         "nmc_run_batch_async": (C.c_int, [H]),
@@ -75,7 +76,8 @@ def lib():
 EXPORTED = ("nmc_ext_config_init", "nmc_validate", "nmc_create", "nmc_prepare", "nmc_set_seed",
             "nmc_run", "nmc_run_async", "nmc_synchronize", "nmc_last_kernel_ms",
             "nmc_get_results", "nmc_reset_tallies", "nmc_destroy", "nmc_last_error",
-            "nmc_version", "nmc_set_tuning", "nmc_get_tuning", "nmc_prepare_batch",
+            "nmc_version", "nmc_set_tuning", "nmc_get_tuning", "nmc_last_launch_count",
+            "nmc_prepare_batch",
             "nmc_run_batch", "nmc_run_batch_async", "nmc_get_batch_results",
             "nmc_set_graph_mode", "nmc_layered_prepare", "nmc_layered_update", "nmc_layered_run",
             "nmc_layered_results", "nmc_layered_backward", "nmc_layered_records")

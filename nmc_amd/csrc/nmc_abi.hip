@@ -213,6 +213,8 @@ struct nmc_handle {
   uint64_t graph_n = 0;
   uint64_t graph_seed = 0;     // the Philox key the captured launch arguments hold
   uint32_t graph_stream = 0;
+  uint32_t graph_launches = 0; // transport launches the captured graph holds
+  uint32_t launches = 0;       // transport launches of the last run
 };
 
 namespace {
@@ -264,6 +266,14 @@ int64_t nvox(const RunParams* rp) { return (int64_t)rp->Nx * rp->Ny * rp->Nz; }
 int64_t per_item(const nmc_handle* h) { return h->n_ra + h->n_rz; }
 
 // ---- mcxyz launch configuration -------------------------------------------
+// Photons per workgroup in one launch with the FP32 LDS tile (tuning.split
+// default): 1e8 photons on the default grid of one 1024-lane workgroup per CU
+// of a 256-CU device, the launch size the FP32 tile's accuracy was measured at.
+constexpr int32_t kFp32TileSplit = 390625;
+
+// Resolves h->tune_req for an mcxyz handle.  Transactional: every check runs
+// and every new buffer is allocated before the handle changes, so a rejected
+// request leaves the previous configuration (and its buffers) in force.
 int resolve_xyz_tuning(nmc_handle* h) {
   const nmc_tuning& r = h->tune_req;
   nmc_tuning t{};
@@ -273,11 +283,13 @@ int resolve_xyz_tuning(nmc_handle* h) {
   t.refill_min = r.refill_min ? r.refill_min : 16;
   if (t.refill_min < 1 || t.refill_min > 64) return fail(NMC_ERR_ARG, "tuning.refill_min not in 1..64");
   t.chunk = r.chunk;
+  if (r.split < 0) return fail(NMC_ERR_ARG, "tuning.split must be >= 0");
   bool tt = h->sim == NMC_SIM_MCXYZ_TT;
   const RunParams& rp = h->rp;
   if (r.mode < 0 || r.mode > 1) return fail(NMC_ERR_ARG, "tuning.mode (tile element) not 0 or 1");
   t.mode = r.mode;
   const bool tile32 = t.mode == 1;
+  t.split = tile32 ? (r.split ? r.split : kFp32TileSplit) : 0;
   const size_t elem = tile32 ? sizeof(float) : sizeof(double);
   const size_t static_lds = sizeof(XyzTissue) * NMC_NTISS + sizeof(nmc_totals) + 256;
   int64_t budget = h->lds_max > static_lds ? (int64_t)((h->lds_max - static_lds) / elem) : 0;
@@ -312,20 +324,28 @@ int resolve_xyz_tuning(nmc_handle* h) {
     if (tile_n > budget) return fail(NMC_ERR_ARG, "tuning.tile needs %lld voxels, LDS holds %lld",
                                      (long long)tile_n, (long long)budget);
   }
-  h->xyz_lds = (size_t)tile_n * elem;
-  HIP_TRY(mcxyz_configure(tt, tile32, t.block, h->xyz_lds));
-  int occ = mcxyz_blocks_per_cu(tt, tile32, t.block, h->xyz_lds);
-  if (occ <= 0) return fail(NMC_ERR_ARG, "kernel does not fit on a CU with block %d, %zu B LDS", t.block, h->xyz_lds);
+  const size_t lds = (size_t)tile_n * elem;
+  // the kernels' dynamic-LDS limit is set to the whole budget, so no other
+  // handle's configuration can fall below what it was resolved for
+  HIP_TRY(mcxyz_configure(tt, tile32, t.block, (size_t)budget * elem));
+  int occ = mcxyz_blocks_per_cu(tt, tile32, t.block, lds);
+  if (occ <= 0) return fail(NMC_ERR_ARG, "kernel does not fit on a CU with block %d, %zu B LDS", t.block, lds);
   t.blocks_per_cu = r.blocks_per_cu > 0 ? std::min(r.blocks_per_cu, occ) : occ;
-  h->xyz_grid = h->num_cu * t.blocks_per_cu;
-  h->tune = t;
-  free_ptr(h->dTilePart);
-  if (tile_n > 0) HIP_TRY(hipMalloc(&h->dTilePart, (size_t)h->xyz_grid * tile_n * sizeof(double)));
+  const int grid = h->num_cu * t.blocks_per_cu;
+  double* part = nullptr;
+  if (tile_n > 0) HIP_TRY(hipMalloc(&part, (size_t)grid * tile_n * sizeof(double)));
+  // commit: the cached graph holds the old buffers, so it goes first
   drop_graph(h);
+  free_ptr(h->dTilePart);
+  h->dTilePart = part;
+  h->xyz_lds = lds;
+  h->xyz_grid = grid;
+  h->tune = t;
   return ok();
 }
 
 // ---- mcsub launch configuration -------------------------------------------
+// Transactional like resolve_xyz_tuning.
 int resolve_sub_tuning(nmc_handle* h) {
   const nmc_tuning& r = h->tune_req;
   nmc_tuning t{};
@@ -335,36 +355,52 @@ int resolve_sub_tuning(nmc_handle* h) {
   if (mode < 0 || mode > 2) return fail(NMC_ERR_ARG, "tuning.mode (tally placement) not in 0..2");
   bool want_lds = mode != 2;
   size_t lds_full = mcsub_lds_bytes(h->n_items, per_item(h)) + 512;
-  h->sub_lds = want_lds && lds_full <= h->lds_max;
-  h->sub_lds_bytes = mcsub_lds_bytes(h->n_items, h->sub_lds ? per_item(h) : 0);
-  if (h->sub_lds_bytes + 512 > h->lds_max)
-    return fail(NMC_ERR_ARG, "%d work items need %zu B of LDS", h->n_items, h->sub_lds_bytes);
+  const bool sub_lds = want_lds && lds_full <= h->lds_max;
+  const size_t lds = mcsub_lds_bytes(h->n_items, sub_lds ? per_item(h) : 0);
+  if (lds + 512 > h->lds_max)
+    return fail(NMC_ERR_ARG, "%d work items need %zu B of LDS", h->n_items, lds);
   t.block = r.block ? r.block : 1024;
   if (t.block != 256 && t.block != 512 && t.block != 1024)
     return fail(NMC_ERR_ARG, "mcsub block must be 256, 512 or 1024");
   t.refill_min = r.refill_min ? r.refill_min : 1;  // mcsub's launch is cheap: refill at once
   if (t.refill_min < 1 || t.refill_min > 64) return fail(NMC_ERR_ARG, "tuning.refill_min not in 1..64");
   t.chunk = r.chunk;
-  t.mode = h->sub_lds ? 1 : 2;
-  HIP_TRY(mcsub_configure(t.block, h->sub_lds_bytes, h->sub_lds));
-  int occ = mcsub_blocks_per_cu(t.block, h->sub_lds_bytes, h->sub_lds);
+  t.mode = sub_lds ? 1 : 2;
+  HIP_TRY(mcsub_configure(t.block, h->lds_max - 512, sub_lds));
+  int occ = mcsub_blocks_per_cu(t.block, lds, sub_lds);
   if (occ <= 0) return fail(NMC_ERR_ARG, "mcsub kernel does not fit on a CU");
   t.blocks_per_cu = r.blocks_per_cu > 0 ? std::min(r.blocks_per_cu, occ) : occ;
-  h->sub_grid = h->num_cu * t.blocks_per_cu;
-  h->tune = t;
-  free_ptr(h->dPart);
-  size_t stride = (size_t)h->n_items * ((h->sub_lds ? per_item(h) : 0) + TF_COUNT + TI_COUNT);
-  HIP_TRY(hipMalloc(&h->dPart, (size_t)h->sub_grid * stride * sizeof(double)));
+  const int grid = h->num_cu * t.blocks_per_cu;
+  size_t stride = (size_t)h->n_items * ((sub_lds ? per_item(h) : 0) + TF_COUNT + TI_COUNT);
+  double* part = nullptr;
+  double* rep = nullptr;
+  int reps = 1;
+  HIP_TRY(hipMalloc(&part, (size_t)grid * stride * sizeof(double)));
   // global-mode tallies go to per-XCD replicas (the bins near the beam axis
   // take a large share of all adds); zeroed here, then by each reduction
-  free_ptr(h->dRep);
-  if (!h->sub_lds) {
-    h->sub_reps = replica_count((int64_t)h->n_items * per_item(h), h->sub_grid);
-    size_t rep_n = (size_t)h->sub_reps * h->n_items * per_item(h);
-    HIP_TRY(hipMalloc(&h->dRep, rep_n * sizeof(double)));
-    HIP_TRY(hipMemsetAsync(h->dRep, 0, rep_n * sizeof(double), h->stream));
+  if (!sub_lds) {
+    reps = replica_count((int64_t)h->n_items * per_item(h), grid);
+    size_t rep_n = (size_t)reps * h->n_items * per_item(h);
+    hipError_t e = hipMalloc(&rep, rep_n * sizeof(double));
+    if (e == hipSuccess) e = hipMemsetAsync(rep, 0, rep_n * sizeof(double), h->stream);
+    if (e != hipSuccess) {
+      free_ptr(part);
+      free_ptr(rep);
+      return fail(e == hipErrorOutOfMemory ? NMC_ERR_NOMEM : NMC_ERR_DEVICE,
+                  "mcsub tally replicas: %s", hipGetErrorString(e));
+    }
   }
+  // commit: the cached graph holds the old buffers, so it goes first
   drop_graph(h);
+  free_ptr(h->dPart);
+  free_ptr(h->dRep);
+  h->dPart = part;
+  h->dRep = rep;
+  h->sub_reps = reps;
+  h->sub_lds = sub_lds;
+  h->sub_lds_bytes = lds;
+  h->sub_grid = grid;
+  h->tune = t;
   return ok();
 }
 
@@ -376,9 +412,6 @@ uint32_t pick_chunk(uint64_t n, int waves, int req) {
 
 uint32_t key0_of(const nmc_handle* h) { return (uint32_t)h->ext.seed; }
 uint32_t key1_of(const nmc_handle* h, uint32_t stream) { return (uint32_t)(h->ext.seed >> 32) + stream; }
-
-// Photons per mcxyz launch with the FP32 LDS tile (tuning.mode 1).
-constexpr uint64_t kFp32TileLaunch = 100000000ull;
 
 // Enqueues one run (no events, no capture handling).
 int enqueue_run(nmc_handle* h, uint64_t n) {
@@ -418,6 +451,7 @@ int enqueue_run(nmc_handle* h, uint64_t n) {
     a.part = h->dPart;
     a.state = h->dState;
     HIP_TRY(launch_mcsub(a, h->tune.block, h->sub_grid, h->sub_lds_bytes, h->sub_lds, s));
+    h->launches++;
     {
       int64_t lds_per = h->sub_lds ? per_item(h) : 0;
       int64_t stride = (int64_t)h->n_items * (lds_per + TF_COUNT + TI_COUNT);
@@ -432,17 +466,20 @@ int enqueue_run(nmc_handle* h, uint64_t n) {
     HIP_TRY(launch_state_advance(h->dState, adv, s));
   } else {
     // With the FP32 tile each workgroup sums its share of a launch's deposits
-    // in FP32; runs are split into launches of at most kFp32TileLaunch
-    // photons (each flushes its tile into the FP64 array), which bounds
-    // those sums to the size the FP32 tile's accuracy was measured at.  The
-    // launches take consecutive photon ids, so the photons are the same.
-    if (h->tune.mode == 1 && n > kFp32TileLaunch) {
-      for (uint64_t done = 0; done < n; done += kFp32TileLaunch) {
-        const uint64_t part = std::min<uint64_t>(kFp32TileLaunch, n - done);
+    // in FP32; runs are split into launches of at most tuning.split photons
+    // per workgroup (each launch flushes its tile into the FP64 array), which
+    // bounds those sums to the size the FP32 tile's accuracy was measured at,
+    // whatever the grid.  The launches take consecutive photon ids, so the
+    // photons are the same.
+    const uint64_t cap = h->tune.mode == 1 ? (uint64_t)h->tune.split * (uint64_t)h->xyz_grid : 0;
+    if (cap > 0 && n > cap) {
+      for (uint64_t done = 0; done < n; done += cap) {
+        const uint64_t part = std::min<uint64_t>(cap, n - done);
         NMC_TRY(enqueue_run(h, part));
       }
       return ok();
     }
+    h->launches++;
     HIP_TRY(launch_state_begin(h->dState, s));
     const RunParams& rp = h->rp;
     const nmc_ext_config& e = h->ext;
@@ -500,6 +537,7 @@ int write_first(nmc_handle* h) {
 int run_async(nmc_handle* h, uint64_t n) {
   HIP_TRY(hipSetDevice(h->device));
   HIP_TRY(hipEventRecord(h->e0, h->stream));
+  h->launches = 0;
   if (h->graph_mode) {
     if (!h->graph_exec || h->graph_n != n || h->graph_seed != h->ext.seed ||
         h->graph_stream != h->ext.stream_id) {
@@ -519,8 +557,10 @@ int run_async(nmc_handle* h, uint64_t n) {
       h->graph_n = n;
       h->graph_seed = h->ext.seed;
       h->graph_stream = h->ext.stream_id;
+      h->graph_launches = h->launches;
     }
     HIP_TRY(hipGraphLaunch(h->graph_exec, h->stream));
+    h->launches = h->graph_launches;
   } else {
     NMC_TRY(enqueue_run(h, n));
   }
@@ -614,6 +654,15 @@ int nmc_validate(int sim, const RunParams* rp, const TissueParams* tp, const uin
     return fail(NMC_ERR_ARG, "boundaryflag %d not in 0..2", rp->boundaryflag);
   if (sim == NMC_SIM_MCSUB) {
     if (rp->mcflag < 0 || rp->mcflag > 2) return fail(NMC_ERR_ARG, "mcsub mcflag %d not in 0..2", rp->mcflag);
+    // the medium is the half-space z > 0 with its surface at z = 0 for every
+    // source; zs places only the isotropic point source (the reference uses
+    // its compile-time zs_mcsub = 0, definitions.h:47, both as the launch
+    // depth and as the escape plane, kernels.metal:113,131,152,190)
+    if ((rp->mcflag == 0 || rp->mcflag == 1) && rp->zs != 0.0f)
+      return fail(NMC_ERR_ARG, "mcsub beams (mcflag 0, 1) enter at the surface z = 0: zs must be 0");
+    if (rp->mcflag == 2 && !(rp->zs >= 0.0f && std::isfinite(rp->zs) && std::isfinite(rp->xs) &&
+                             std::isfinite(rp->ys)))
+      return fail(NMC_ERR_ARG, "mcsub point source: zs must be finite and >= 0 (inside the medium)");
     if (e.nr <= 0 || e.nz <= 0 || e.na <= 0 || e.na > 64 || !(e.dr > 0.0f) || !(e.dz > 0.0f))
       return fail(NMC_ERR_ARG, "mcsub bins invalid (nr,nz >= 1, 1 <= na <= 64, dr,dz > 0)");
     float mua = tp->muav[1], mus = tp->musv[1], g = tp->gv[1];
@@ -843,6 +892,7 @@ int nmc_set_seed(nmc_handle* h, uint64_t seed, uint64_t first_photon) {
 int nmc_set_tuning(nmc_handle* h, const nmc_tuning* t) {
   if (!h || !t) return fail(NMC_ERR_ARG, "null argument");
   if (t->refill_min < 0 || t->refill_min > 64) return fail(NMC_ERR_ARG, "tuning.refill_min not in 1..64");
+  const nmc_tuning prev = h->tune_req;
   h->tune_req = *t;
   if (!h->prepared) return ok();
   HIP_TRY(hipSetDevice(h->device));
@@ -854,7 +904,18 @@ int nmc_set_tuning(nmc_handle* h, const nmc_tuning* t) {
     h->tune.chunk = t->chunk;
     return ok();
   }
-  return h->sim == NMC_SIM_MCSUB ? resolve_sub_tuning(h) : resolve_xyz_tuning(h);
+  // a rejected request changes nothing: the resolution is transactional and
+  // the request it replaced stays the one a later nmc_prepare resolves
+  int st = h->sim == NMC_SIM_MCSUB ? resolve_sub_tuning(h) : resolve_xyz_tuning(h);
+  if (st != NMC_OK) h->tune_req = prev;
+  return st;
+}
+
+int nmc_last_launch_count(nmc_handle* h, uint32_t* n) {
+  if (!h || !n) return fail(NMC_ERR_ARG, "null argument");
+  if (!h->timed) return fail(NMC_ERR_STATE, "no run has been timed");
+  *n = h->launches;
+  return ok();
 }
 
 int nmc_get_tuning(nmc_handle* h, nmc_tuning* t) {
@@ -1211,6 +1272,7 @@ int nmc_layered_run(nmc_handle* h) {
   a.rec = h->dRec;
   a.state = h->dState;
   HIP_TRY(launch_layered(a, h->lay_grid, h->lay_lds, s));
+  h->launches = 1;
   HIP_TRY(launch_reduce_partials(h->dLPart, h->lay_grid, (int64_t)n * (NMC_MAX_LAYERS + TF_COUNT + TI_COUNT),
                                  h->dLOut, n, NMC_MAX_LAYERS, s));
   HIP_TRY(launch_reduce_partials(h->dLRdPart, h->lay_rd_count, rd_len, h->dLRd, n, h->lcfg.nr + 1, s,

@@ -328,11 +328,17 @@ __device__ __forceinline__ float tt_cos(const TTCoef& t, float u) {
 }
 
 // Rotate direction (ux,uy,uz) by polar cosine `cost` and azimuth 2*pi*u_psi.
+// The pole formula is taken where the reference takes it: its test
+// 1 - |uz| <= 1e-12 in float arithmetic (kernels.metal:713) holds exactly
+// when |uz| >= 1.  It is also taken when ux^2 + uy^2 < 2^-100 (only a
+// direction that is not quite unit length gets there with |uz| < 1), which
+// keeps the general formula's quotient below 2^100.
 __device__ __forceinline__ void spin(float cost, float u_psi, float& ux, float& uy, float& uz) {
   float sp, cp;
   sincos2pi(u_psi, &sp, &cp);
   float x = ux, y = uy, z = uz;
-  if (fabsf(z) > 1.0f - 1.0e-6f) {
+  float r2 = x * x + y * y;
+  if (fabsf(z) >= 1.0f || !(r2 >= 0x1p-100f)) {
     // 1 - cos^2 is 0 or at least 2^-24 (cos^2 <= 1 is a float): sqrt_rn's domain
     float sint = sqrt_rn(1.0f - cost * cost);
     ux = sint * cp;
@@ -341,10 +347,10 @@ __device__ __forceinline__ void spin(float cost, float u_psi, float& ux, float& 
   } else {
     // st = sin(theta) / sqrt(ux^2 + uy^2) as one square root; the uz term
     // sin(theta) * sqrt(ux^2 + uy^2) is st * (ux^2 + uy^2)
-    // r2 = 1 - uz^2 up to rounding, at least ~2e-6 here; 1 - cos^2 is 0 or in
-    // [2^-24, 1]: both in div_rn's domain, and the quotient (0 or at least
-    // 2^-24) in sqrt_rn's
-    float r2 = x * x + y * y;
+    // r2 = 1 - uz^2 up to rounding (about 1.2e-7 or more for a unit vector
+    // with |uz| < 1), at least 2^-100 here; 1 - cos^2 is 0 or in [2^-24, 1]:
+    // the residual of the quotient stays normal (div_rn's domain), and the
+    // quotient (0 or at least 2^-24) is in sqrt_rn's
     float st = sqrt_rn(div_rn(1.0f - cost * cost, r2));
     ux = st * (x * z * cp - y * sp) + x * cost;
     uy = st * (y * z * cp + x * sp) + y * cost;

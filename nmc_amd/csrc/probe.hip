@@ -93,11 +93,20 @@ __device__ __forceinline__ void div_operands(int dom, uint64_t i, float* a, floa
     *a = 1.0f - g * g;
     *b = (1.0f - g) + (2.0f * g) * u;
   } else if (dom == 2) {
+    // the rotation's 1 - cos^2 over ux^2 + uy^2: half the pairs take the
+    // latter as 1 - w^2 of a unit vector (|w| < 1, down to about 1.2e-7), half
+    // as any magnitude in [2^-100, 1) (the spin's lower bound for it)
     const uint64_t h = mix64(i);
     const float c = 2.0f * ((float)(uint32_t)(h >> 40) * 0x1p-24f) - 1.0f;
-    const float w = (1.0f - 1.0e-6f) * (2.0f * ((float)(uint32_t)(h & 0xFFFFFFu) * 0x1p-24f) - 1.0f);
     *a = 1.0f - c * c;
-    *b = 1.0f - w * w;
+    if (h & 1ull) {
+      const float w = 2.0f * ((float)(uint32_t)((h >> 1) & 0xFFFFFFu) * 0x1p-24f) - 1.0f;
+      const float r2 = 1.0f - w * w;
+      *b = r2 >= 0x1p-100f ? r2 : 0x1p-100f;
+    } else {
+      const uint32_t e = 27u + (uint32_t)((h >> 1) % 100u);
+      *b = __uint_as_float((e << 23) | (uint32_t)((h >> 8) & 0x7FFFFFu));
+    }
   } else {
     const uint64_t h = mix64(i);
     const uint32_t ea = 67u + (uint32_t)((h >> 23) % 121u), eb = 67u + (uint32_t)((h >> 46) % 121u);

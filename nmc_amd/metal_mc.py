@@ -61,6 +61,12 @@ class MetalMC:
         check(lib().nmc_last_kernel_ms(self._h, C.byref(ms)))
         return ms.value
 
+    def last_launch_count(self) -> int:
+        """Transport-kernel launches of the last run."""
+        n = C.c_uint32()
+        check(lib().nmc_last_launch_count(self._h, C.byref(n)))
+        return n.value
+
     # -- results --------------------------------------------------------
     def get_compute_results(self, fluence: bool = True) -> P.TallyArrays:
         if self.rp is None:

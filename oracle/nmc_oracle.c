@@ -334,13 +334,17 @@ float nmco_tt_cos(float gf, float af, float gb, float ab, float cc, float u)
 }
 
 /* New direction after deflection cos(theta) = cost and azimuth 2*pi*u_psi
- * (kernels.metal:705-730; sin(psi) is computed directly). */
+ * (kernels.metal:705-730; sin(psi) is computed directly).  Pole formula where
+ * the reference's float test 1 - |uz| <= 1e-12 (kernels.metal:713) holds,
+ * i.e. |uz| >= 1, or where ux^2 + uy^2 < 2^-100 (a direction slightly off
+ * unit length), which bounds the general formula's quotient. */
 void nmco_spin(float cost, float u_psi, float* ux, float* uy, float* uz)
 {
     float sp, cp;
     nmco_sincos2pi(u_psi, &sp, &cp);
     float x = *ux, y = *uy, z = *uz;
-    if (fabsf(z) > 1.0f - 1.0e-6f) {
+    float r2 = x * x + y * y;
+    if (fabsf(z) >= 1.0f || !(r2 >= 0x1p-100f)) {
         float sint = sqrtf(1.0f - cost * cost);
         *ux = sint * cp;
         *uy = sint * sp;
@@ -348,7 +352,6 @@ void nmco_spin(float cost, float u_psi, float* ux, float* uy, float* uz)
     } else {
         /* st = sin(theta) / sqrt(ux^2 + uy^2) as one square root; the uz term
          * sin(theta) * sqrt(ux^2 + uy^2) is st * (ux^2 + uy^2) */
-        float r2 = x * x + y * y;
         float st = sqrtf((1.0f - cost * cost) / r2);
         *ux = st * (x * z * cp - y * sp) + x * cost;
         *uy = st * (y * z * cp + x * sp) + y * cost;

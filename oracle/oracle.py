@@ -20,16 +20,9 @@ LIB_PATH = os.path.join(HERE, "libnmc_oracle.so")
 _lib = None
 
 
-REF_SOURCES = "/root/reference/NMC/NMCCode"
-
-
 def build() -> str:
-    """Builds the oracle library and, where the reference sources are present,
-    oracle/_ref/ref_mcio from the reference's own file routines."""
+    """Builds the oracle library (always recompiles)."""
     subprocess.run(["make", "-s", "-B", "-C", HERE], check=True)
-    if os.path.isfile(os.path.join(REF_SOURCES, "mcio.h")):
-        subprocess.run(["make", "-s", "-B", "-C", HERE, "ref", f"REF_SRC={REF_SOURCES}"],
-                       check=True)
     return LIB_PATH
 
 

@@ -117,6 +117,24 @@ def test_mcsub_validation_errors():
     _expect(P.ERR_ARG, lambda: validate(w.sim, w.rp, tp, None, w.ext))
 
 
+def test_mcsub_source_depth_rule():
+    """The mcsub medium's surface is z = 0 for every source: a beam (mcflag 0,
+    1) must enter there, a point source (mcflag 2) sits at zs >= 0 inside."""
+    w = W.mcsub_semi_infinite()
+    for mcflag in (0, 1):
+        rp = P.RunParams.from_buffer_copy(w.rp)
+        rp.mcflag, rp.zs = mcflag, 0.05
+        assert "zs" in _expect(P.ERR_ARG, lambda: validate(w.sim, rp, w.tp, None, w.ext))
+        rp.zs = 0.0
+        validate(w.sim, rp, w.tp, None, w.ext)
+    rp = P.RunParams.from_buffer_copy(w.rp)
+    rp.mcflag, rp.zs = 2, 0.05
+    validate(w.sim, rp, w.tp, None, w.ext)
+    for bad in (-0.01, float("inf"), float("nan")):
+        rp.zs = bad
+        _expect(P.ERR_ARG, lambda: validate(w.sim, rp, w.tp, None, w.ext))
+
+
 def test_create_without_device_fails_cleanly():
     h = C.c_void_p()
     if os.environ.get("HIP_VISIBLE_DEVICES") is None:

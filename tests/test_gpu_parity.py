@@ -68,7 +68,8 @@ def test_mcsub_launch_modes(gpu, mcflag):
     w = W.mcsub_semi_infinite()
     w.rp.mcflag = mcflag
     w.rp.radius, w.rp.waist, w.rp.zfocus = 0.02, 0.01, 0.5
-    w.rp.zs = 0.05
+    # beams enter at the surface z = 0; the point source sits inside at zs
+    w.rp.zs = 0.05 if mcflag == 2 else 0.0
     _say(f"[parity] mcsub mcflag={mcflag}")
     ref = O.run(w, 5000, threads=4)
     got = simulate(w, 5000)

@@ -1,17 +1,14 @@
 """File contract (nmc_amd/mcio.py) on the CPU.
 
-Where oracle/_ref/ref_mcio was built from the reference's own mcio.h
-(NMC/NMCCode/mcio.h), the reader, the run-parameter echo, `_props.m` and
-`mcOUT0.dat` are compared with it byte for byte; the other checks pin the
-formats against the printf strings and the normalisations against closed
-forms.
+The reference's file routines (NMC/NMCCode/mcio.h) cannot be compiled here:
+definitions.h needs Objective-C's YES/NO, which only the absent Foundation
+headers define.  The checks therefore pin the formats against the printf
+strings of mcio.h / main.m and the normalisations against closed forms.
 """
 from __future__ import annotations
 
-import ctypes as C
 import math
 import os
-import subprocess
 
 import numpy as np
 import pytest
@@ -22,9 +19,6 @@ from nmc_amd import workloads as W
 from nmc_amd._lib import NmcError
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-REF = os.path.join(ROOT, "oracle", "_ref", "ref_mcio")
-needs_ref = pytest.mark.skipif(not os.path.exists(REF),
-                               reason="oracle/_ref/ref_mcio not built (needs the reference sources)")
 
 
 def notebook_mci(path_name: str, rp: P.RunParams, tp: P.TissueParams, props=None) -> None:
@@ -195,89 +189,6 @@ def test_mcsub_tables_closed_form():
                                rd_ra.reshape(nr + 1, na).sum(1), rtol=1e-6)
     np.testing.assert_allclose(F[1:, 1:] * (2 * math.pi * r * dr * dz * n * mua)[:, None],
                                a_rz.reshape(nr + 1, nz + 1), rtol=1e-6)
-
-
-# ---------------------------------------------------------------------------
-# against the reference's own routines (oracle/_ref/ref_mcio)
-
-def _ref_read(tmp_path, stem: str):
-    r = subprocess.run([REF, "read", stem], cwd=tmp_path, capture_output=True, text=True,
-                       timeout=60)
-    assert r.returncode == 0, r.stderr
-    raw = (tmp_path / f"{stem}_parsed.bin").read_bytes()
-    rp = P.RunParams.from_buffer_copy(raw[:C.sizeof(P.RunParams)])
-    tp = P.TissueParams.from_buffer_copy(raw[C.sizeof(P.RunParams):])
-    return rp, tp, r.stdout
-
-
-@needs_ref
-@pytest.mark.parametrize("style", ["notebook", "exact"])
-@pytest.mark.parametrize("kernel", [P.SIM_MCXYZ, P.SIM_MCXYZ_TT, P.SIM_MCSUB, 3])
-def test_reader_echo_and_props_match_reference(tmp_path, kernel, style):
-    rp, tp, v = small_case(kernel)
-    rp.zfocus = math.inf if kernel != P.SIM_MCSUB else 1.0
-    rp.launchflag = 1 if kernel == 3 else 0
-    rp.ux0, rp.uy0, rp.uz0 = 0.1, -0.2, 0.97
-    stem = "case"
-    name = str(tmp_path / stem)
-    (notebook_mci if style == "notebook" else mcio.write_mci)(name, rp, tp)
-    mcio.write_tissue(name, v)
-    rrp, rtp, echo = _ref_read(tmp_path, stem)
-    ref_props = (tmp_path / f"{stem}_props.m").read_text()
-    os.remove(tmp_path / f"{stem}_props.m")
-
-    mrp, mtp = mcio.read_run_params(name)
-    assert struct_bytes(mrp, mtp) == struct_bytes(rrp, rtp)
-    mv = mcio.import_tissue(name, mrp)
-    assert echo == f"name = {stem}\n" + mcio.run_parameters_text(mrp, mtp, mv)
-    mcio.save_optical_properties(mrp, mtp, name)
-    assert (tmp_path / f"{stem}_props.m").read_text() == ref_props
-
-
-@needs_ref
-def test_long_line_matches_reference(tmp_path):
-    rp, tp, v = small_case()
-    stem = "long"
-    name = str(tmp_path / stem)
-    mcio.write_mci(name, rp, tp)
-    lines = open(name + "_H.mci").read().split("\n")
-    lines[0] = "1" + "0" * 29 + "7" + "0" * 8 + "5"
-    lines[5] = "   0.00500000000000000000000000000001"     # dy, 37 characters
-    open(name + "_H.mci", "w").write("\n".join(lines))
-    # the spilled reads shift every later field; the file still has enough lines
-    open(name + "_H.mci", "a").write("0\n0\n")
-    mrp, mtp = mcio.read_run_params(name)
-    mcio.write_tissue(name, np.ones(max(mrp.Nx * mrp.Ny * mrp.Nz, 1), np.uint8))
-    rrp, rtp, _ = _ref_read(tmp_path, stem)
-    assert struct_bytes(mrp, mtp) == struct_bytes(rrp, rtp)
-
-
-@needs_ref
-@pytest.mark.parametrize("nr,nz", [(6, 6), (101, 101)])
-def test_mcout_matches_reference(tmp_path, nr, nz):
-    rng = np.random.default_rng(nr)
-    J = np.zeros(nr + 1, np.float32)
-    J[1:] = rng.random(nr) * 10.0 ** rng.integers(-8, 3, nr)
-    F2 = np.zeros((nr + 1, nz + 1), np.float32)
-    F2[1:, 1:] = rng.random((nr, nz)) * 10.0 ** rng.integers(-9, 2, (nr, nz))
-    F2[3, 2] = 0.0
-    d = [0.0200628, 0.7711, 0.2088, 1.0, 100.0, 0.9, 1.33, 1.0, 0.0, 0.1, 0.0, 0.0, 0.0,
-         0.002, 0.002, 1e6]
-    flat = np.zeros((nr + 1) * nr + nz + 1, np.float32)
-    for ir in range(1, nr + 1):
-        for iz in range(1, nz + 1):
-            flat[ir * nr + iz] = F2[ir, iz]
-    blob = tmp_path / "in.bin"
-    blob.write_bytes(np.array(d, np.float64).tobytes() + np.array([0, nr, nz], np.int32).tobytes()
-                     + J.tobytes() + flat.tobytes())
-    r = subprocess.run([REF, "savefile", str(blob)], cwd=tmp_path, capture_output=True,
-                       timeout=60)
-    assert r.returncode == 0
-    ref = (tmp_path / "mcOUT0.dat").read_text()
-    mine = mcio.save_mcsub_file(str(tmp_path / "mine.dat"), J, F2, d[0], d[1], d[2], d[3], d[4],
-                                d[5], d[6], d[7], 0, d[8], d[9], d[10], d[11], d[12], d[13],
-                                d[14], d[15])
-    assert open(mine).read() == ref
 
 
 def test_cli_rejects_bad_inputs_before_any_device_work(tmp_path, capsys):
