@@ -339,6 +339,36 @@ int nmc_layered_records(nmc_handle* h, uint32_t* out, uint64_t max_records, uint
 int nmc_layered_backward(nmc_handle* h, const double* grad_rd, double* grad_mua,
                          double* grad_mus);
 
+/*
+ * Allocation-free mcxyz run.  No counterpart in the reference: for callers
+ * that cannot allocate device memory (serving loops, harnesses that forbid
+ * hipMalloc, code that must not allocate between captures).  Transports
+ * photons [first_photon, first_photon + n_photons) - the photons that
+ * nmc_prepare + nmc_set_seed(h, ext->seed, first_photon) + nmc_run(h, n) give
+ * on a handle of the same kind - and ADDS the tallies into the caller's
+ * device arrays, as getComputeResults adds into caller-zeroed arrays
+ * (MetalMC.m:521): d_fluence Nz*Nx*Ny doubles (NULL or NMC_FLAG_NO_FLUENCE:
+ * no fluence tally), d_ryx Nx*Ny doubles, d_det2d Nx*Ny u32, d_totals one
+ * nmc_totals.  rp, tp, ext (may be NULL) and tuning (may be NULL) are host
+ * structs; d_v is the voxel map in device memory (Nx*Ny*Nz bytes).  Runs on
+ * the current device, on `stream` (a hipStream_t; NULL: the null stream).
+ * The call checks the voxel map's tissue ids on the device and synchronises
+ * `stream` once for that, then enqueues the run and returns; it calls no
+ * hipMalloc or hipFree.  Scratch (tissue table, source block, queue state,
+ * FP64 block partials of the LDS tile) lives in one static device array of
+ * NMC_NOALLOC_SCRATCH_BYTES per GPU, kept until the process ends: a tile whose
+ * block partials (workgroups x tile voxels) exceed NMC_NOALLOC_MAX_PARTIALS
+ * doubles is rejected with NMC_ERR_ARG (the automatic tile shrinks to fit).
+ * Every call on a GPU shares that array: calls that overlap on one GPU (on
+ * different streams, unsynchronised) race.
+ */
+#define NMC_NOALLOC_MAX_PARTIALS (2u * 1024u * 1024u)
+#define NMC_NOALLOC_SCRATCH_BYTES (4096u + 8u * NMC_NOALLOC_MAX_PARTIALS)
+int nmc_mcxyz_run_noalloc(int sim_type, const RunParams* rp, const TissueParams* tp,
+                          const nmc_ext_config* ext, const nmc_tuning* tuning, const uint8_t* d_v,
+                          uint64_t first_photon, uint64_t n_photons, double* d_fluence,
+                          double* d_ryx, uint32_t* d_det2d, nmc_totals* d_totals, void* stream);
+
 /* Message of the last failing call on this host thread ("" if none). */
 const char* nmc_last_error(void);
 

@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -271,11 +272,15 @@ int64_t per_item(const nmc_handle* h) { return h->n_ra + h->n_rz; }
 // of a 256-CU device, the launch size the FP32 tile's accuracy was measured at.
 constexpr int32_t kFp32TileSplit = 390625;
 
-// Resolves h->tune_req for an mcxyz handle.  Transactional: every check runs
-// and every new buffer is allocated before the handle changes, so a rejected
-// request leaves the previous configuration (and its buffers) in force.
-int resolve_xyz_tuning(nmc_handle* h) {
-  const nmc_tuning& r = h->tune_req;
+}  // namespace
+
+namespace nmc {
+
+// Resolves a tuning request for an mcxyz run (no side effects on any handle).
+// part_cap >= 0 bounds the tile's block partials (grid x tile voxels doubles):
+// the automatic box shrinks to fit it and an explicit box beyond it fails.
+int resolve_xyz_launch(const RunParams& rp, int sim, bool has_fluence, const nmc_tuning& r,
+                       size_t lds_max, int num_cu, int64_t part_cap, XyzLaunch* out) {
   nmc_tuning t{};
   t.block = r.block ? r.block : 1024;
   if (t.block != 256 && t.block != 512 && t.block != 1024)
@@ -284,16 +289,24 @@ int resolve_xyz_tuning(nmc_handle* h) {
   if (t.refill_min < 1 || t.refill_min > 64) return fail(NMC_ERR_ARG, "tuning.refill_min not in 1..64");
   t.chunk = r.chunk;
   if (r.split < 0) return fail(NMC_ERR_ARG, "tuning.split must be >= 0");
-  bool tt = h->sim == NMC_SIM_MCXYZ_TT;
-  const RunParams& rp = h->rp;
+  bool tt = sim == NMC_SIM_MCXYZ_TT;
   if (r.mode < 0 || r.mode > 1) return fail(NMC_ERR_ARG, "tuning.mode (tile element) not 0 or 1");
   t.mode = r.mode;
   const bool tile32 = t.mode == 1;
   t.split = tile32 ? (r.split ? r.split : kFp32TileSplit) : 0;
   const size_t elem = tile32 ? sizeof(float) : sizeof(double);
   const size_t static_lds = sizeof(XyzTissue) * NMC_NTISS + sizeof(nmc_totals) + 256;
-  int64_t budget = h->lds_max > static_lds ? (int64_t)((h->lds_max - static_lds) / elem) : 0;
-  if (!h->dF || r.tile[3] < 0) {
+  const int64_t lds_budget = lds_max > static_lds ? (int64_t)((lds_max - static_lds) / elem) : 0;
+  // the kernels' dynamic-LDS limit is set to the whole budget, so no other
+  // configuration can fall below what it was resolved for
+  HIP_TRY(mcxyz_configure(tt, tile32, t.block, (size_t)lds_budget * elem));
+  int64_t budget = lds_budget;
+  if (part_cap >= 0) {
+    const int occ0 = mcxyz_blocks_per_cu(tt, tile32, t.block, 0);
+    const int64_t most_blocks = (int64_t)num_cu * std::max(1, occ0);
+    budget = std::min<int64_t>(budget, part_cap / most_blocks);
+  }
+  if (!has_fluence || r.tile[3] < 0 || (r.tile[3] == 0 && budget < 1)) {
     for (int i = 0; i < 6; ++i) t.tile[i] = 0;
   } else if (r.tile[3] > 0) {
     for (int i = 0; i < 6; ++i) t.tile[i] = r.tile[i];
@@ -321,26 +334,144 @@ int resolve_xyz_tuning(nmc_handle* h) {
     if (t.tile[0] < 0 || t.tile[1] < 0 || t.tile[2] < 0 || t.tile[0] + t.tile[3] > rp.Nx ||
         t.tile[1] + t.tile[4] > rp.Ny || t.tile[2] + t.tile[5] > rp.Nz)
       return fail(NMC_ERR_ARG, "tuning.tile box outside the grid");
-    if (tile_n > budget) return fail(NMC_ERR_ARG, "tuning.tile needs %lld voxels, LDS holds %lld",
-                                     (long long)tile_n, (long long)budget);
+    if (tile_n > lds_budget) return fail(NMC_ERR_ARG, "tuning.tile needs %lld voxels, LDS holds %lld",
+                                         (long long)tile_n, (long long)lds_budget);
   }
   const size_t lds = (size_t)tile_n * elem;
-  // the kernels' dynamic-LDS limit is set to the whole budget, so no other
-  // handle's configuration can fall below what it was resolved for
-  HIP_TRY(mcxyz_configure(tt, tile32, t.block, (size_t)budget * elem));
   int occ = mcxyz_blocks_per_cu(tt, tile32, t.block, lds);
   if (occ <= 0) return fail(NMC_ERR_ARG, "kernel does not fit on a CU with block %d, %zu B LDS", t.block, lds);
   t.blocks_per_cu = r.blocks_per_cu > 0 ? std::min(r.blocks_per_cu, occ) : occ;
-  const int grid = h->num_cu * t.blocks_per_cu;
+  const int grid = num_cu * t.blocks_per_cu;
+  if (part_cap >= 0 && (int64_t)grid * tile_n > part_cap)
+    return fail(NMC_ERR_ARG, "tuning.tile: %d workgroups x %lld voxels of FP64 block partials exceed "
+                "the %lld doubles the scratch array holds", grid, (long long)tile_n, (long long)part_cap);
+  out->t = t;
+  out->lds = lds;
+  out->grid = grid;
+  out->tile_n = tile_n;
+  return ok();
+}
+
+uint32_t pick_chunk(uint64_t n, int waves, int req) {
+  if (req > 0) return (uint32_t)req;
+  uint64_t c = n / ((uint64_t)waves * 8ull);
+  return (uint32_t)std::max<uint64_t>(64, std::min<uint64_t>(1024, c));
+}
+
+// Launch arguments of an mcxyz run that follow from the configuration alone
+// (the device pointers, photon count and queue chunk are set by the caller).
+XyzArgs xyz_args(const RunParams& rp, const nmc_ext_config& e, int sim, uint32_t key0, uint32_t key1) {
+  (void)sim;
+  XyzArgs a{};
+  a.Nx = rp.Nx;
+  a.Ny = rp.Ny;
+  a.Nz = rp.Nz;
+  a.dx = rp.dx;
+  a.dy = rp.dy;
+  a.dz = rp.dz;
+  a.hx = 0.5f * (float)rp.Nx;
+  a.hy = 0.5f * (float)rp.Ny;
+  a.bmode = rp.boundaryflag;
+  a.det_state = rp.det_state;
+  a.zsurf = rp.zsurf;
+  a.thr = e.threshold;
+  a.chance = e.chance;
+  a.max_events = e.max_events;
+  a.key0 = key0;
+  a.key1 = key1;
+  a.count_visits = (e.flags & NMC_FLAG_COUNT_VISITS) != 0;
+  return a;
+}
+
+// Enqueues photons [first, first + n) of an mcxyz run on stream s, `first`
+// being a.state->first: [reset queue counter, transport kernel, tile
+// reduction, advance first] per launch.  With the FP32 tile each workgroup
+// sums its share of a launch's deposits in FP32; runs are split into
+// launches of at most tuning.split photons per workgroup (each launch
+// flushes its tile into the FP64 array), which bounds those sums to the size
+// the FP32 tile's accuracy was measured at, whatever the grid.  The launches
+// take consecutive photon ids, so the photons are the same.
+int enqueue_xyz(XyzArgs a, bool two_term, const XyzLaunch& L, uint64_t n, hipStream_t s,
+                uint32_t* launches) {
+  a.tile_x0 = L.t.tile[0];
+  a.tile_y0 = L.t.tile[1];
+  a.tile_z0 = L.t.tile[2];
+  a.tile_nx = L.t.tile[3];
+  a.tile_ny = L.t.tile[4];
+  a.tile_nz = L.t.tile[5];
+  a.refill_min = (uint32_t)L.t.refill_min;
+  const uint64_t cap = L.t.mode == 1 ? (uint64_t)L.t.split * (uint64_t)L.grid : 0;
+  const uint64_t step = cap > 0 ? cap : n;
+  for (uint64_t done = 0; done < n; done += step) {
+    const uint64_t part = std::min<uint64_t>(step, n - done);
+    a.n = part;
+    a.chunk = pick_chunk(part, L.grid * (L.t.block / 64), L.t.chunk);
+    HIP_TRY(launch_state_begin(a.state, s));
+    HIP_TRY(launch_mcxyz(a, two_term, L.t.mode == 1, L.t.block, L.grid, L.lds, s));
+    HIP_TRY(launch_tile_reduce(a.tile_part, L.grid, a, s));
+    HIP_TRY(launch_state_advance(a.state, part, s));
+    ++*launches;
+  }
+  return ok();
+}
+
+// Device tables of an mcxyz configuration: the tissue rows (index 0..18) and
+// the source / detector block (its ryx and det2d pointers left null).
+void xyz_tables(const RunParams& rp, const TissueParams& tp, int sim, XyzTissue* tis, XyzAux* aux) {
+  for (int t = 0; t < NMC_NTISS; ++t) {
+    std::memset(&tis[t], 0, sizeof(XyzTissue));
+    tis[t].mua = tp.muav[t];
+    tis[t].mus = tp.musv[t];
+    tis[t].inv_mus = inv_pos(tp.musv[t]);
+    tis[t].g = tp.gv[t];
+    tis[t].hg = hg_prepare(tp.gv[t]);
+    if (sim == NMC_SIM_MCXYZ_TT) tt_prepare(tp.gf[t], tp.af[t], tp.gb[t], tp.ab[t], tp.CC[t], &tis[t].tt);
+  }
+  std::memset(aux, 0, sizeof(*aux));
+  aux->mcflag = rp.mcflag;
+  aux->launchflag = rp.launchflag;
+  aux->xs = rp.xs;
+  aux->ys = rp.ys;
+  aux->zs = rp.zs;
+  aux->ux0 = rp.ux0;
+  aux->uy0 = rp.uy0;
+  aux->uz0 = rp.uz0;
+  aux->radius = rp.radius;
+  aux->waist = rp.waist;
+  aux->zfocus = rp.zfocus;
+  aux->n1 = rp.n1;
+  aux->n2 = rp.n2;
+  if (rp.xd == 0.0f) {
+    aux->rmin = 0.0f;
+    aux->rmax = rp.det_radius;
+  } else {
+    aux->rmin = rp.xd - rp.det_radius;
+    aux->rmax = aux->rmin + 2.0f * rp.det_radius;
+  }
+  aux->cos_na = rp.na >= 90.0f ? -1.0f : (float)std::cos((double)rp.na * 3.14159265358979323846 / 180.0);
+  aux->Nx = rp.Nx;
+  aux->Ny = rp.Ny;
+}
+
+}  // namespace nmc
+
+namespace {
+
+// Resolves h->tune_req for an mcxyz handle.  Transactional: every check runs
+// and every new buffer is allocated before the handle changes, so a rejected
+// request leaves the previous configuration (and its buffers) in force.
+int resolve_xyz_tuning(nmc_handle* h) {
+  XyzLaunch L{};
+  NMC_TRY(resolve_xyz_launch(h->rp, h->sim, h->dF != nullptr, h->tune_req, h->lds_max, h->num_cu, -1, &L));
   double* part = nullptr;
-  if (tile_n > 0) HIP_TRY(hipMalloc(&part, (size_t)grid * tile_n * sizeof(double)));
+  if (L.tile_n > 0) HIP_TRY(hipMalloc(&part, (size_t)L.grid * L.tile_n * sizeof(double)));
   // commit: the cached graph holds the old buffers, so it goes first
   drop_graph(h);
   free_ptr(h->dTilePart);
   h->dTilePart = part;
-  h->xyz_lds = lds;
-  h->xyz_grid = grid;
-  h->tune = t;
+  h->xyz_lds = L.lds;
+  h->xyz_grid = L.grid;
+  h->tune = L.t;
   return ok();
 }
 
@@ -404,12 +535,6 @@ int resolve_sub_tuning(nmc_handle* h) {
   return ok();
 }
 
-uint32_t pick_chunk(uint64_t n, int waves, int req) {
-  if (req > 0) return (uint32_t)req;
-  uint64_t c = n / ((uint64_t)waves * 8ull);
-  return (uint32_t)std::max<uint64_t>(64, std::min<uint64_t>(1024, c));
-}
-
 uint32_t key0_of(const nmc_handle* h) { return (uint32_t)h->ext.seed; }
 uint32_t key1_of(const nmc_handle* h, uint32_t stream) { return (uint32_t)(h->ext.seed >> 32) + stream; }
 
@@ -465,62 +590,18 @@ int enqueue_run(nmc_handle* h, uint64_t n) {
     for (int j = 0; j < h->n_items; ++j) adv = std::max<uint64_t>(adv, h->prefix[j + 1] - h->prefix[j]);
     HIP_TRY(launch_state_advance(h->dState, adv, s));
   } else {
-    // With the FP32 tile each workgroup sums its share of a launch's deposits
-    // in FP32; runs are split into launches of at most tuning.split photons
-    // per workgroup (each launch flushes its tile into the FP64 array), which
-    // bounds those sums to the size the FP32 tile's accuracy was measured at,
-    // whatever the grid.  The launches take consecutive photon ids, so the
-    // photons are the same.
-    const uint64_t cap = h->tune.mode == 1 ? (uint64_t)h->tune.split * (uint64_t)h->xyz_grid : 0;
-    if (cap > 0 && n > cap) {
-      for (uint64_t done = 0; done < n; done += cap) {
-        const uint64_t part = std::min<uint64_t>(cap, n - done);
-        NMC_TRY(enqueue_run(h, part));
-      }
-      return ok();
-    }
-    h->launches++;
-    HIP_TRY(launch_state_begin(h->dState, s));
-    const RunParams& rp = h->rp;
-    const nmc_ext_config& e = h->ext;
-    XyzArgs a{};
-    a.Nx = rp.Nx;
-    a.Ny = rp.Ny;
-    a.Nz = rp.Nz;
-    a.dx = rp.dx;
-    a.dy = rp.dy;
-    a.dz = rp.dz;
-    a.hx = 0.5f * (float)rp.Nx;
-    a.hy = 0.5f * (float)rp.Ny;
-    a.bmode = rp.boundaryflag;
-    a.det_state = rp.det_state;
-    a.zsurf = rp.zsurf;
+    XyzArgs a = xyz_args(h->rp, h->ext, h->sim, key0_of(h), key1_of(h, h->ext.stream_id));
     a.aux = h->dAux;
-    a.thr = e.threshold;
-    a.chance = e.chance;
-    a.max_events = e.max_events;
-    a.key0 = key0_of(h);
-    a.key1 = key1_of(h, e.stream_id);
-    a.n = n;
-    a.chunk = pick_chunk(n, h->xyz_grid * (h->tune.block / 64), h->tune.chunk);
-    a.refill_min = (uint32_t)h->tune.refill_min;
-    a.count_visits = (e.flags & NMC_FLAG_COUNT_VISITS) != 0;
-    a.tile_x0 = h->tune.tile[0];
-    a.tile_y0 = h->tune.tile[1];
-    a.tile_z0 = h->tune.tile[2];
-    a.tile_nx = h->tune.tile[3];
-    a.tile_ny = h->tune.tile[4];
-    a.tile_nz = h->tune.tile[5];
     a.V = h->dV;
     a.tissue = h->dTis;
     a.F = h->dF;
     a.tile_part = h->dTilePart;
     a.tot = h->dTot;
     a.state = h->dState;
-    bool tt = h->sim == NMC_SIM_MCXYZ_TT;
-    HIP_TRY(launch_mcxyz(a, tt, h->tune.mode == 1, h->tune.block, h->xyz_grid, h->xyz_lds, s));
-    HIP_TRY(launch_tile_reduce(h->dTilePart, h->xyz_grid, a, s));
-    HIP_TRY(launch_state_advance(h->dState, n, s));
+    XyzLaunch L{h->tune, h->xyz_lds, h->xyz_grid, 0};
+    uint32_t launches = 0;
+    NMC_TRY(enqueue_xyz(a, h->sim == NMC_SIM_MCXYZ_TT, L, n, s, &launches));
+    h->launches += launches;
   }
   return ok();
 }
@@ -618,22 +699,12 @@ int upload_media(nmc_handle* h) {
   return ok();
 }
 
-}  // namespace
+int check_tissues(int sim, const TissueParams* tp, const bool* used);
 
-extern "C" {
-
-void nmc_ext_config_init(nmc_ext_config* cfg) {
-  if (cfg) ext_defaults(cfg);
-}
-
-const char* nmc_last_error(void) { return g_err.c_str(); }
-
-const char* nmc_version(void) {
-  return "nmc_hip 2 gfx950 -O3 -ffp-contract=off wave64 philox4x32-10";
-}
-
-int nmc_validate(int sim, const RunParams* rp, const TissueParams* tp, const uint8_t* v,
-                 const nmc_ext_config* ext) {
+// The checks of nmc_validate; scan = false skips the voxel map (the
+// allocation-free entry point checks a device-resident map separately).
+int validate_impl(int sim, const RunParams* rp, const TissueParams* tp, const uint8_t* v,
+                  const nmc_ext_config* ext, bool scan) {
   if (!rp || !tp) return fail(NMC_ERR_ARG, "RunParams and TissueParams are required");
   if (sim != NMC_SIM_MCSUB && sim != NMC_SIM_MCXYZ && sim != NMC_SIM_MCXYZ_TT)
     return fail(NMC_ERR_UNSUPPORTED, "unsupported simulation kind %d", sim);
@@ -685,10 +756,16 @@ int nmc_validate(int sim, const RunParams* rp, const TissueParams* tp, const uin
     float n2 = rp->ux0 * rp->ux0 + rp->uy0 * rp->uy0 + rp->uz0 * rp->uz0;
     if (!(std::fabs(n2 - 1.0f) < 1e-3f)) return fail(NMC_ERR_ARG, "launch direction (ux0,uy0,uz0) is not a unit vector");
   }
+  if (!scan) return ok();
   if (!v) return fail(NMC_ERR_ARG, "voxel map is required for mcxyz");
   bool used[256] = {false};
   int64_t n = nvox(rp);
   for (int64_t i = 0; i < n; ++i) used[v[i]] = true;
+  return check_tissues(sim, tp, used);
+}
+
+// The tissue rows a voxel map uses (used[id]) must exist and be valid.
+int check_tissues(int sim, const TissueParams* tp, const bool* used) {
   for (int t = 0; t < 256; ++t) {
     if (!used[t]) continue;
     if (t >= NMC_NTISS) return fail(NMC_ERR_ARG, "voxel tissue id %d >= %d", t, NMC_NTISS);
@@ -705,6 +782,25 @@ int nmc_validate(int sim, const RunParams* rp, const TissueParams* tp, const uin
     }
   }
   return ok();
+}
+
+}  // namespace
+
+extern "C" {
+
+void nmc_ext_config_init(nmc_ext_config* cfg) {
+  if (cfg) ext_defaults(cfg);
+}
+
+const char* nmc_last_error(void) { return g_err.c_str(); }
+
+const char* nmc_version(void) {
+  return "nmc_hip 2 gfx950 -O3 -ffp-contract=off wave64 philox4x32-10";
+}
+
+int nmc_validate(int sim, const RunParams* rp, const TissueParams* tp, const uint8_t* v,
+                 const nmc_ext_config* ext) {
+  return validate_impl(sim, rp, tp, v, ext, true);
 }
 
 int nmc_create(int sim_type, int device, nmc_handle** out) {
@@ -788,41 +884,10 @@ int nmc_prepare(nmc_handle* h, const RunParams* rp, const TissueParams* tp, cons
   HIP_TRY(hipMalloc(&h->dRyx, n2 * sizeof(double)));
   HIP_TRY(hipMalloc(&h->dDet, n2 * sizeof(uint32_t)));
   std::vector<XyzTissue> tis(NMC_NTISS);
-  for (int t = 0; t < NMC_NTISS; ++t) {
-    std::memset(&tis[t], 0, sizeof(XyzTissue));
-    tis[t].mua = tp->muav[t];
-    tis[t].mus = tp->musv[t];
-    tis[t].inv_mus = inv_pos(tp->musv[t]);
-    tis[t].g = tp->gv[t];
-    tis[t].hg = hg_prepare(tp->gv[t]);
-    if (h->sim == NMC_SIM_MCXYZ_TT) tt_prepare(tp->gf[t], tp->af[t], tp->gb[t], tp->ab[t], tp->CC[t], &tis[t].tt);
-  }
+  XyzAux aux{};
+  xyz_tables(*rp, *tp, h->sim, tis.data(), &aux);
   HIP_TRY(hipMalloc(&h->dTis, NMC_NTISS * sizeof(XyzTissue)));
   HIP_TRY(hipMemcpy(h->dTis, tis.data(), NMC_NTISS * sizeof(XyzTissue), hipMemcpyHostToDevice));
-  XyzAux aux{};
-  aux.mcflag = rp->mcflag;
-  aux.launchflag = rp->launchflag;
-  aux.xs = rp->xs;
-  aux.ys = rp->ys;
-  aux.zs = rp->zs;
-  aux.ux0 = rp->ux0;
-  aux.uy0 = rp->uy0;
-  aux.uz0 = rp->uz0;
-  aux.radius = rp->radius;
-  aux.waist = rp->waist;
-  aux.zfocus = rp->zfocus;
-  aux.n1 = rp->n1;
-  aux.n2 = rp->n2;
-  if (rp->xd == 0.0f) {
-    aux.rmin = 0.0f;
-    aux.rmax = rp->det_radius;
-  } else {
-    aux.rmin = rp->xd - rp->det_radius;
-    aux.rmax = aux.rmin + 2.0f * rp->det_radius;
-  }
-  aux.cos_na = rp->na >= 90.0f ? -1.0f : (float)std::cos((double)rp->na * 3.14159265358979323846 / 180.0);
-  aux.Nx = rp->Nx;
-  aux.Ny = rp->Ny;
   aux.ryx = h->dRyx;
   aux.det2d = h->dDet;
   HIP_TRY(hipMalloc(&h->dAux, sizeof(XyzAux)));
@@ -1387,3 +1452,157 @@ void nmc_destroy(nmc_handle* h) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Allocation-free mcxyz run (nmc_mcxyz_run_noalloc).  Its scratch lives in a
+// static device array of fixed size, one copy per GPU, held until the process
+// ends:
+//   [0, 16)        RunState (queue counter, first photon id)
+//   [64, 96)       bitmask of the tissue ids the voxel map uses (256 bits)
+//   [256, 1024)    XyzAux (source and detector block)
+//   [1024, 4096)   XyzTissue[NMC_NTISS]
+//   [4096, end)    FP64 block partials of the LDS tile (grid x tile voxels)
+// In host code `scratch` names a host-side stand-in, never the device memory:
+// its device address comes from hipGetSymbolAddress, once per device.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoallocHead = 4096;
+constexpr size_t kNoallocStateOff = 0, kNoallocMaskOff = 64, kNoallocAuxOff = 256,
+                 kNoallocTisOff = 1024;
+static_assert(NMC_NOALLOC_SCRATCH_BYTES > kNoallocHead, "scratch holds the head block");
+static_assert(sizeof(RunState) <= kNoallocMaskOff, "RunState fits its slot");
+static_assert(kNoallocAuxOff + sizeof(XyzAux) <= kNoallocTisOff, "XyzAux fits its slot");
+static_assert(kNoallocTisOff + NMC_NTISS * sizeof(XyzTissue) <= kNoallocHead, "tissue table fits");
+static_assert(NMC_NOALLOC_MAX_PARTIALS * sizeof(double) == NMC_NOALLOC_SCRATCH_BYTES - kNoallocHead,
+              "partials fill the rest");
+
+static __device__ __align__(256) unsigned char scratch[NMC_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::once_flag g_scratch_once[kMaxDevices];
+unsigned char* g_scratch_ptr[kMaxDevices];
+hipError_t g_scratch_err[kMaxDevices];
+
+// Device address of `scratch` on device `dev`, which must be current: looked
+// up on the first call for that device and kept.
+int scratch_on(int dev, unsigned char** out) {
+  if (dev < 0 || dev >= kMaxDevices) return fail(NMC_ERR_DEVICE, "device %d out of range", dev);
+  std::call_once(g_scratch_once[dev], [dev] {
+    void* p = nullptr;
+    g_scratch_err[dev] = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    g_scratch_ptr[dev] = g_scratch_err[dev] == hipSuccess ? (unsigned char*)p : nullptr;
+  });
+  if (g_scratch_err[dev] != hipSuccess || !g_scratch_ptr[dev])
+    return fail(NMC_ERR_DEVICE, "hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                hipGetErrorString(g_scratch_err[dev]));
+  *out = g_scratch_ptr[dev];
+  return ok();
+}
+
+__global__ void noalloc_mask_clear(uint32_t* mask) {
+  if (threadIdx.x < 8) mask[threadIdx.x] = 0u;
+}
+
+// mask |= bit v[i] for every voxel (per-workgroup LDS mask first; a lane adds
+// a bit only when its tissue id differs from the one it added last).
+__global__ void __launch_bounds__(256) noalloc_mask_scan(const uint8_t* __restrict__ v, uint64_t n,
+                                                         uint32_t* mask) {
+  __shared__ uint32_t m[8];
+  if (threadIdx.x < 8) m[threadIdx.x] = 0u;
+  __syncthreads();
+  uint32_t last = 256u;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const uint32_t t = v[i];
+    if (t != last) {
+      atomicOr(&m[t >> 5], 1u << (t & 31u));
+      last = t;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 8 && m[threadIdx.x]) atomicOr(&mask[threadIdx.x], m[threadIdx.x]);
+}
+
+struct NoallocTables {
+  XyzTissue tis[NMC_NTISS];
+  XyzAux aux;
+  RunState state;
+};
+
+// Writes the run's tables (passed by value) into the scratch array.
+__global__ void noalloc_init(NoallocTables in, XyzTissue* tis, XyzAux* aux, RunState* st) {
+  if (threadIdx.x < NMC_NTISS) tis[threadIdx.x] = in.tis[threadIdx.x];
+  if (threadIdx.x == 0) {
+    *aux = in.aux;
+    *st = in.state;
+  }
+}
+
+}  // namespace
+
+extern "C" int nmc_mcxyz_run_noalloc(int sim_type, const RunParams* rp, const TissueParams* tp,
+                                     const nmc_ext_config* ext, const nmc_tuning* tuning,
+                                     const uint8_t* d_v, uint64_t first_photon, uint64_t n_photons,
+                                     double* d_fluence, double* d_ryx, uint32_t* d_det2d,
+                                     nmc_totals* d_totals, void* stream) {
+  if (sim_type != NMC_SIM_MCXYZ && sim_type != NMC_SIM_MCXYZ_TT)
+    return fail(NMC_ERR_UNSUPPORTED, "nmc_mcxyz_run_noalloc runs NMC_SIM_MCXYZ or NMC_SIM_MCXYZ_TT");
+  NMC_TRY(validate_impl(sim_type, rp, tp, nullptr, ext, false));
+  if (!d_v || !d_ryx || !d_det2d || !d_totals)
+    return fail(NMC_ERR_ARG, "d_v, d_ryx, d_det2d and d_totals are required");
+  nmc_ext_config e;
+  ext_defaults(&e);
+  if (ext) e = *ext;
+  const bool fluence = !(e.flags & NMC_FLAG_NO_FLUENCE) && d_fluence != nullptr;
+  int dev = 0, num_cu = 0, lds = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  HIP_TRY(hipDeviceGetAttribute(&num_cu, hipDeviceAttributeMultiprocessorCount, dev));
+  HIP_TRY(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+  unsigned char* base = nullptr;
+  NMC_TRY(scratch_on(dev, &base));
+  nmc_tuning req{};
+  if (tuning) req = *tuning;
+  XyzLaunch L{};
+  NMC_TRY(resolve_xyz_launch(*rp, sim_type, fluence, req, (size_t)lds, num_cu,
+                             (int64_t)NMC_NOALLOC_MAX_PARTIALS, &L));
+  hipStream_t s = (hipStream_t)stream;
+  // the tissue ids the device-resident voxel map uses, checked on the host
+  uint32_t* dmask = (uint32_t*)(base + kNoallocMaskOff);
+  const uint64_t nv = (uint64_t)nvox(rp);
+  hipLaunchKernelGGL(noalloc_mask_clear, dim3(1), dim3(64), 0, s, dmask);
+  HIP_TRY(hipGetLastError());
+  const uint64_t scan_blocks = std::min<uint64_t>((nv + 255) / 256, 4096);
+  hipLaunchKernelGGL(noalloc_mask_scan, dim3((unsigned)scan_blocks), dim3(256), 0, s, d_v, nv, dmask);
+  HIP_TRY(hipGetLastError());
+  uint32_t mask[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  HIP_TRY(hipMemcpyAsync(mask, dmask, sizeof mask, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  bool used[256];
+  for (int t = 0; t < 256; ++t) used[t] = ((mask[t >> 5] >> (t & 31)) & 1u) != 0u;
+  NMC_TRY(check_tissues(sim_type, tp, used));
+  if (n_photons == 0) return ok();
+  NoallocTables tab;
+  std::memset(&tab, 0, sizeof tab);
+  xyz_tables(*rp, *tp, sim_type, tab.tis, &tab.aux);
+  tab.aux.ryx = d_ryx;
+  tab.aux.det2d = d_det2d;
+  tab.state.counter = 0ull;
+  tab.state.first = (unsigned long long)first_photon;
+  XyzTissue* dtis = (XyzTissue*)(base + kNoallocTisOff);
+  XyzAux* daux = (XyzAux*)(base + kNoallocAuxOff);
+  RunState* dstate = (RunState*)(base + kNoallocStateOff);
+  hipLaunchKernelGGL(noalloc_init, dim3(1), dim3(64), 0, s, tab, dtis, daux, dstate);
+  HIP_TRY(hipGetLastError());
+  XyzArgs a = xyz_args(*rp, e, sim_type, (uint32_t)e.seed, (uint32_t)(e.seed >> 32) + e.stream_id);
+  a.aux = daux;
+  a.V = d_v;
+  a.tissue = dtis;
+  a.F = fluence ? d_fluence : nullptr;
+  a.tile_part = (double*)(base + kNoallocHead);
+  a.tot = d_totals;
+  a.state = dstate;
+  uint32_t launches = 0;
+  NMC_TRY(enqueue_xyz(a, sim_type == NMC_SIM_MCXYZ_TT, L, n_photons, s, &launches));
+  return ok();
+}

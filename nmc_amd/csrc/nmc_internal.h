@@ -177,6 +177,15 @@ hipError_t launch_layered(const LayArgs& a, int grid, size_t lds_bytes, hipStrea
 hipError_t launch_layered_backward(const LayBackArgs& b, hipStream_t s);
 
 
+// A resolved mcxyz launch shape: tuning (tile box, block, ...), dynamic LDS
+// bytes, workgroups, and tile voxels (0: no tile).
+struct XyzLaunch {
+  nmc_tuning t;
+  size_t lds;
+  int grid;
+  int64_t tile_n;
+};
+
 hipError_t launch_mcxyz(const XyzArgs& a, bool two_term, bool tile32, int block, int grid,
                         size_t lds_bytes, hipStream_t s);
 hipError_t mcxyz_configure(bool two_term, bool tile32, int block, size_t lds_bytes);

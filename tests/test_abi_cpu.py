@@ -162,3 +162,21 @@ def test_workload_geometry():
     assert np.array_equal(v[:, 17, :] == 4, vessel)
     assert np.array_equal(v[:, 150, :] == 4, vessel)
     assert (v[zc >= 0.11] == 3).all()
+
+
+def test_noalloc_entry_point_rejects_bad_arguments_before_device_work():
+    """nmc_mcxyz_run_noalloc checks its arguments on the host first."""
+    w = W.tiny_voxel_case("mismatch")
+    L = lib()
+    null = C.c_void_p(0)
+    st = L.nmc_mcxyz_run_noalloc(P.SIM_MCSUB, C.byref(w.rp), C.byref(w.tp), C.byref(w.ext),
+                                 C.POINTER(P.Tuning)(), null, 0, 10, null, null, null, null, null)
+    assert st == P.ERR_UNSUPPORTED
+    st = L.nmc_mcxyz_run_noalloc(w.sim, C.byref(w.rp), C.byref(w.tp), C.byref(w.ext),
+                                 C.POINTER(P.Tuning)(), null, 0, 10, null, null, null, null, null)
+    assert st == P.ERR_ARG and b"required" in L.nmc_last_error()
+    rp = P.RunParams.from_buffer_copy(w.rp)
+    rp.Nx = 0
+    st = L.nmc_mcxyz_run_noalloc(w.sim, C.byref(rp), C.byref(w.tp), C.byref(w.ext),
+                                 C.POINTER(P.Tuning)(), null, 0, 10, null, null, null, null, null)
+    assert st == P.ERR_ARG
