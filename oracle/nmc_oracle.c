@@ -47,6 +47,18 @@
 /* ------------------------------------------------------------------------ */
 /* Philox4x32-10 (Salmon, Moraes, Dror, Shaw 2011).                          */
 /* ------------------------------------------------------------------------ */
+/* Planted mistakes for the oracle's own tests (tests/test_oracle_pins.py
+ * builds copies with -DNMCO_MUTANT=k and checks that each pin fails on them;
+ * the oracle proper is built with NMCO_MUTANT = 0):
+ *   1  flat beam radius R*xi instead of R*sqrt(xi)
+ *   2  the remaining step keeps its physical length across a change of
+ *      tissue or layer (no rescale of the optical depth by mus)
+ *   3  mcsub drops its weight at the position before the hop
+ *   4  mcsub bins counted from 1 (the first bin left empty)             */
+#ifndef NMCO_MUTANT
+#define NMCO_MUTANT 0
+#endif
+
 void nmco_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4])
 {
     uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
@@ -450,14 +462,13 @@ typedef struct { double* F; double* ryx; uint32_t* det; nmc_totals t; } xyz_acc;
 
 static inline float xface(int i, float h, float d) { return ((float)i - h) * d; }
 
-static void xyz_photon(const xyz_ctx* c, uint64_t pid, xyz_acc* a)
+/* LAUNCH (kernels.metal:366-463): start position, direction, weight and the
+ * specular loss (focused Gaussian beam entering the tissue, else 0) of photon
+ * pid.  o = {x, y, z, ux, uy, uz, W, spec}. */
+static void xyz_source(const xyz_ctx* c, uint64_t pid, float o[8])
 {
-    nmc_totals* t = &a->t;
     uint32_t b[4];
-    float x, y, z, ux, uy, uz, W = 1.0f;
-    t->photons++;
-
-    /* LAUNCH (kernels.metal:366-463) */
+    float x, y, z, ux, uy, uz, W = 1.0f, spec = 0.0f;
     if (c->launchflag == 1) {
         x = c->xs; y = c->ys; z = c->zs;
         ux = c->ux0; uy = c->uy0; uz = c->uz0;
@@ -465,7 +476,7 @@ static void xyz_photon(const xyz_ctx* c, uint64_t pid, xyz_acc* a)
         block(c->key, pid, 0u, 0u, b);
         if (c->mcflag == 0 || c->mcflag == 1) {
             float r;
-            if (c->mcflag == 0) r = c->radius * sqrtf(u01nz(b[0]));
+            if (c->mcflag == 0) r = NMCO_MUTANT == 1 ? c->radius * u01nz(b[0]) : c->radius * sqrtf(u01nz(b[0]));
             else r = c->radius * sqrtf(-nmco_logf(u01nz(b[0])));
             float sp, cp;
             nmco_sincos2pi(u01nz(b[1]), &sp, &cp);
@@ -483,7 +494,7 @@ static void xyz_photon(const xyz_ctx* c, uint64_t pid, xyz_acc* a)
             if (c->mcflag == 1) {
                 float ca2;
                 float rsp = nmco_fresnel(c->n2, c->n1, fabsf(uz), &ca2);
-                t->specular += (double)(W * rsp);
+                spec = W * rsp;
                 W = W * (1.0f - rsp);
                 snell(c->n2, c->n1, &ux, &uy, &uz);
             }
@@ -504,6 +515,19 @@ static void xyz_photon(const xyz_ctx* c, uint64_t pid, xyz_acc* a)
             ux = 0.0f; uy = 0.0f; uz = 1.0f;
         }
     }
+    o[0] = x; o[1] = y; o[2] = z; o[3] = ux; o[4] = uy; o[5] = uz; o[6] = W; o[7] = spec;
+}
+
+static void xyz_photon(const xyz_ctx* c, uint64_t pid, xyz_acc* a)
+{
+    nmc_totals* t = &a->t;
+    uint32_t b[4];
+    t->photons++;
+
+    float o[8];
+    xyz_source(c, pid, o);
+    float x = o[0], y = o[1], z = o[2], ux = o[3], uy = o[4], uz = o[5], W = o[6];
+    t->specular += (double)o[7];
 
     int ix = (int)floorf(c->hx + x / c->dx);
     int iy = (int)floorf(c->hy + y / c->dy);
@@ -621,7 +645,10 @@ static void xyz_photon(const xyz_ctx* c, uint64_t pid, xyz_acc* a)
                     else { t->lateral += (double)W; t->escapes_side++; }
                     alive = 0;
                 } else {
-                    tis = c->V[VIDX(CLAMPI(ix, c->Nx), CLAMPI(iy, c->Ny), CLAMPI(iz, c->Nz))];
+                    int nt = c->V[VIDX(CLAMPI(ix, c->Nx), CLAMPI(iy, c->Ny), CLAMPI(iz, c->Nz))];
+                    if (NMCO_MUTANT == 2 && nt != tis && c->mus[tis] > 0.0f)
+                        sleft = sleft * (c->mus[nt] / c->mus[tis]);
+                    tis = nt;
                 }
             }
         }
@@ -732,6 +759,23 @@ int nmco_mcxyz(const RunParams* rp, const TissueParams* tp, const uint8_t* v,
     return fail ? NMC_ERR_NOMEM : NMC_OK;
 }
 
+/* Launch states of photons [first, first + n) of an mcxyz configuration
+ * (test entry point for the source models): out[8 i .. 8 i + 7] = x, y, z,
+ * ux, uy, uz, W, specular loss of photon first + i. */
+int nmco_xyz_source(const RunParams* rp, const TissueParams* tp, const uint8_t* v,
+                    const nmc_ext_config* ext, uint64_t first, uint64_t n, float* out)
+{
+    nmc_ext_config e;
+    ext_defaults(ext, &e);
+    xyz_ctx* c = (xyz_ctx*)malloc(sizeof(xyz_ctx));
+    if (!c) return NMC_ERR_NOMEM;
+    int st = xyz_setup(rp, tp, v, &e, c);
+    if (st == NMC_OK)
+        for (uint64_t i = 0; i < n; ++i) xyz_source(c, first + i, out + 8 * i);
+    free(c);
+    return st;
+}
+
 /* ------------------------------------------------------------------------ */
 /* mcsub (kernels.metal:59-319).                                             */
 /* ------------------------------------------------------------------------ */
@@ -750,16 +794,16 @@ typedef struct { double* rd_ra; double* a_rz; nmc_totals t; } sub_acc;
 
 static inline int rbin(float q, int n) { return q >= (float)n ? n : (int)q; }
 
-static void sub_photon(const sub_ctx* c, uint64_t pid, sub_acc* a)
+/* LAUNCH (kernels.metal:106-166): o = {x, y, z, ux, uy, uz, rsp}. */
+static void sub_source(const sub_ctx* c, uint64_t pid, float o[7])
 {
-    nmc_totals* t = &a->t;
     uint32_t b[4];
     float x, y, z, ux, uy, uz, rsp;
-    t->photons++;
     block(c->key, pid, 0u, 0u, b);
     if (c->mcflag == 0) {
         /* uniform collimated beam at the surface (kernels.metal:106-120) */
-        x = c->radius * sqrtf(u01(b[0])); y = 0.0f; z = 0.0f;
+        x = NMCO_MUTANT == 1 ? c->radius * u01(b[0]) : c->radius * sqrtf(u01(b[0]));
+        y = 0.0f; z = 0.0f;
         ux = 0.0f; uy = 0.0f; uz = 1.0f;
         rsp = c->rsp0;
     } else if (c->mcflag == 1) {
@@ -784,6 +828,17 @@ static void sub_photon(const sub_ctx* c, uint64_t pid, sub_acc* a)
         ux = st * cp; uy = st * sp; uz = ct;
         rsp = 0.0f;
     }
+    o[0] = x; o[1] = y; o[2] = z; o[3] = ux; o[4] = uy; o[5] = uz; o[6] = rsp;
+}
+
+static void sub_photon(const sub_ctx* c, uint64_t pid, sub_acc* a)
+{
+    nmc_totals* t = &a->t;
+    uint32_t b[4];
+    t->photons++;
+    float o[7];
+    sub_source(c, pid, o);
+    float x = o[0], y = o[1], z = o[2], ux = o[3], uy = o[4], uz = o[5], rsp = o[6];
     float W = 1.0f - rsp;
     t->specular += (double)rsp;
 
@@ -810,6 +865,10 @@ static void sub_photon(const sub_ctx* c, uint64_t pid, sub_acc* a)
                 int ir = rbin(r / c->dr, c->nr);
                 int ia = 0;
                 while (ia < c->na - 1 && uz1 < c->cos_tab[ia]) ia++;
+                if (NMCO_MUTANT == 4) {
+                    ir = ir < c->nr ? ir + 1 : c->nr;
+                    ia = ia < c->na - 1 ? ia + 1 : c->na - 1;
+                }
                 if (a->rd_ra) a->rd_ra[(int64_t)ir * c->na + ia] += (double)W;
                 t->rd += (double)W;
                 t->detections++;
@@ -823,9 +882,15 @@ static void sub_photon(const sub_ctx* c, uint64_t pid, sub_acc* a)
         /* DROP (kernels.metal:224-234) */
         float absorb = W * c->one_m_albedo;
         W = W - absorb;
-        float r = sqrtf(x * x + y * y);
+        const float xa = NMCO_MUTANT == 3 ? x0 : x, ya = NMCO_MUTANT == 3 ? y0 : y;
+        const float za = NMCO_MUTANT == 3 ? z0 : z;
+        float r = sqrtf(xa * xa + ya * ya);
         int ir = rbin(r / c->dr, c->nr);
-        int iz = rbin(fabsf(z) / c->dz, c->nz);
+        int iz = rbin(fabsf(za) / c->dz, c->nz);
+        if (NMCO_MUTANT == 4) {
+            ir = ir < c->nr ? ir + 1 : c->nr;
+            iz = iz < c->nz ? iz + 1 : c->nz;
+        }
         if (a->a_rz) a->a_rz[(int64_t)ir * (c->nz + 1) + iz] += (double)absorb;
         t->absorbed += (double)absorb;
         /* SPIN (kernels.metal:243-279, with g honoured) */
@@ -842,11 +907,12 @@ static void sub_photon(const sub_ctx* c, uint64_t pid, sub_acc* a)
     }
 }
 
-int nmco_mcsub(const RunParams* rp, const TissueParams* tp, const nmc_ext_config* ext,
-               uint64_t first, uint64_t n, int n_threads, nmc_results* out)
+static int sub_setup(const RunParams* rp, const TissueParams* tp, const nmc_ext_config* ext,
+                     nmc_ext_config* e_out, sub_ctx* cp)
 {
     nmc_ext_config e;
     ext_defaults(ext, &e);
+    *e_out = e;
     if (!rp || !tp || e.nr <= 0 || e.nz <= 0 || e.na <= 0 || e.na > 64 || !(e.dr > 0) || !(e.dz > 0))
         return NMC_ERR_ARG;
     sub_ctx c;
@@ -870,6 +936,31 @@ int nmco_mcsub(const RunParams* rp, const TissueParams* tp, const nmc_ext_config
     c.key[1] = (uint32_t)(e.seed >> 32) + e.stream_id;
     for (int j = 0; j < c.na; ++j)
         c.cos_tab[j] = (float)cos((double)(j + 1) * (3.14159265358979323846 / 2.0) / (double)c.na);
+    *cp = c;
+    return NMC_OK;
+}
+
+/* Launch states of photons [first, first + n) of an mcsub configuration (test
+ * entry point for the source models): out[7 i .. 7 i + 6] = x, y, z, ux, uy,
+ * uz, specular reflectance of photon first + i. */
+int nmco_sub_source(const RunParams* rp, const TissueParams* tp, const nmc_ext_config* ext,
+                    uint64_t first, uint64_t n, float* out)
+{
+    nmc_ext_config e;
+    sub_ctx c;
+    int st = sub_setup(rp, tp, ext, &e, &c);
+    if (st == NMC_OK)
+        for (uint64_t i = 0; i < n; ++i) sub_source(&c, first + i, out + 7 * i);
+    return st;
+}
+
+int nmco_mcsub(const RunParams* rp, const TissueParams* tp, const nmc_ext_config* ext,
+               uint64_t first, uint64_t n, int n_threads, nmc_results* out)
+{
+    nmc_ext_config e;
+    sub_ctx c;
+    int st = sub_setup(rp, tp, ext, &e, &c);
+    if (st != NMC_OK) return st;
 
     int64_t nra = (int64_t)(c.nr + 1) * c.na, nrz = (int64_t)(c.nr + 1) * (c.nz + 1);
     int nt = n_threads > 0 ? n_threads : nmco_max_threads();
@@ -1031,7 +1122,9 @@ static void lay_photon(const lay_stack* S, const lay_cfg* c, uint64_t pid, lay_a
                     uz = up ? -ca2 : ca2;
                     ruz = 1.0f / uz;
                 }
-                l += up ? -1 : 1;
+                const int lnew = up ? l - 1 : l + 1;
+                if (NMCO_MUTANT == 2 && S->mus[l] > 0.0f) sleft = sleft * (S->mus[lnew] / S->mus[l]);
+                l = lnew;
             }
         }
         if (!(W > 0.0f)) return; /* all weight absorbed */

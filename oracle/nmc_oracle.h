@@ -55,6 +55,14 @@ void nmco_layered_backward(const nmc_layered_config* cfg, const nmc_stack* stack
                            const uint32_t* rec_K, const double* grad_rd, double* grad_mua,
                            double* grad_mus);
 
+/* Launch states of photons [first, first + n) (test entry points for the
+ * source models): mcxyz 8 floats per photon (x, y, z, ux, uy, uz, W,
+ * specular loss), mcsub 7 (x, y, z, ux, uy, uz, specular reflectance). */
+int nmco_xyz_source(const RunParams* rp, const TissueParams* tp, const uint8_t* v,
+                    const nmc_ext_config* ext, uint64_t first, uint64_t n, float* out);
+int nmco_sub_source(const RunParams* rp, const TissueParams* tp, const nmc_ext_config* ext,
+                    uint64_t first, uint64_t n, float* out);
+
 /* Seconds of the last nmco_mcxyz / nmco_mcsub transport loop (CLOCK_MONOTONIC). */
 double nmco_last_seconds(void);
 int nmco_max_threads(void);

@@ -26,37 +26,94 @@ def build() -> str:
     return LIB_PATH
 
 
+def _bind(L):
+    f32p = C.POINTER(C.c_float)
+    L.nmco_philox4x32_10.argtypes = [C.POINTER(C.c_uint32)] * 3
+    for name in ("nmco_logf", "nmco_expf", "nmco_expm1f"):
+        getattr(L, name).argtypes = [C.c_float]
+        getattr(L, name).restype = C.c_float
+    for name in ("nmco_logf_n", "nmco_expf_n", "nmco_expm1f_n"):
+        getattr(L, name).argtypes = [f32p, f32p, C.c_int64]
+    L.nmco_sincos2pi_n.argtypes = [f32p, f32p, f32p, C.c_int64]
+    L.nmco_fresnel.argtypes = [C.c_float, C.c_float, C.c_float, f32p]
+    L.nmco_fresnel.restype = C.c_float
+    L.nmco_hg_cos.argtypes = [C.c_float, C.c_float]
+    L.nmco_hg_cos.restype = C.c_float
+    L.nmco_tt_cos.argtypes = [C.c_float] * 6
+    L.nmco_tt_cos.restype = C.c_float
+    L.nmco_spin.argtypes = [C.c_float, C.c_float, f32p, f32p, f32p]
+    L.nmco_mcxyz.argtypes = [C.POINTER(P.RunParams), C.POINTER(P.TissueParams),
+                             C.POINTER(C.c_uint8), C.POINTER(P.ExtConfig), C.c_uint64,
+                             C.c_uint64, C.c_int, C.POINTER(P.Results)]
+    L.nmco_mcsub.argtypes = [C.POINTER(P.RunParams), C.POINTER(P.TissueParams),
+                             C.POINTER(P.ExtConfig), C.c_uint64, C.c_uint64, C.c_int,
+                             C.POINTER(P.Results)]
+    L.nmco_xyz_source.argtypes = [C.POINTER(P.RunParams), C.POINTER(P.TissueParams),
+                                  C.POINTER(C.c_uint8), C.POINTER(P.ExtConfig), C.c_uint64,
+                                  C.c_uint64, f32p]
+    L.nmco_sub_source.argtypes = [C.POINTER(P.RunParams), C.POINTER(P.TissueParams),
+                                  C.POINTER(P.ExtConfig), C.c_uint64, C.c_uint64, f32p]
+    L.nmco_layered.argtypes = [C.POINTER(P.LayeredConfig), C.POINTER(P.Stack), C.c_int,
+                               C.c_uint64, C.c_int, C.c_void_p, C.c_void_p,
+                               C.POINTER(P.Totals), C.POINTER(C.c_uint64), C.c_uint64,
+                               C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.nmco_layered_backward.argtypes = [C.POINTER(P.LayeredConfig), C.POINTER(P.Stack), C.c_int,
+                                        C.c_uint64, C.c_uint64] + [C.c_void_p] * 8
+    L.nmco_last_seconds.restype = C.c_double
+    L.nmco_max_threads.restype = C.c_int
+    return L
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        L = C.CDLL(LIB_PATH)
-        f32p = C.POINTER(C.c_float)
-        L.nmco_philox4x32_10.argtypes = [C.POINTER(C.c_uint32)] * 3
-        for name in ("nmco_logf", "nmco_expf", "nmco_expm1f"):
-            getattr(L, name).argtypes = [C.c_float]
-            getattr(L, name).restype = C.c_float
-        for name in ("nmco_logf_n", "nmco_expf_n", "nmco_expm1f_n"):
-            getattr(L, name).argtypes = [f32p, f32p, C.c_int64]
-        L.nmco_sincos2pi_n.argtypes = [f32p, f32p, f32p, C.c_int64]
-        L.nmco_fresnel.argtypes = [C.c_float, C.c_float, C.c_float, f32p]
-        L.nmco_fresnel.restype = C.c_float
-        L.nmco_hg_cos.argtypes = [C.c_float, C.c_float]
-        L.nmco_hg_cos.restype = C.c_float
-        L.nmco_tt_cos.argtypes = [C.c_float] * 6
-        L.nmco_tt_cos.restype = C.c_float
-        L.nmco_spin.argtypes = [C.c_float, C.c_float, f32p, f32p, f32p]
-        L.nmco_mcxyz.argtypes = [C.POINTER(P.RunParams), C.POINTER(P.TissueParams),
-                                 C.POINTER(C.c_uint8), C.POINTER(P.ExtConfig), C.c_uint64,
-                                 C.c_uint64, C.c_int, C.POINTER(P.Results)]
-        L.nmco_mcsub.argtypes = [C.POINTER(P.RunParams), C.POINTER(P.TissueParams),
-                                 C.POINTER(P.ExtConfig), C.c_uint64, C.c_uint64, C.c_int,
-                                 C.POINTER(P.Results)]
-        L.nmco_last_seconds.restype = C.c_double
-        L.nmco_max_threads.restype = C.c_int
-        _lib = L
+        _lib = _bind(C.CDLL(LIB_PATH))
     return _lib
+
+
+MUTANTS = {1: "flat beam radius R*xi instead of R*sqrt(xi)",
+           2: "remaining step keeps its physical length across a tissue/layer change",
+           3: "mcsub drops its weight at the pre-hop position",
+           4: "mcsub bins counted from 1"}
+_mutants: dict = {}
+
+
+def mutant(k: int, workdir: str):
+    """A copy of the oracle built with planted mistake k (MUTANTS) into
+    workdir, for tests that show a pin fails on it."""
+    if k not in _mutants:
+        out = os.path.join(workdir, f"libnmc_oracle_mutant{k}.so")
+        subprocess.run(["make", "-s", "-B", "-C", HERE, "mutant", f"K={k}", f"OUT={out}"],
+                       check=True)
+        _mutants[k] = _bind(C.CDLL(out))
+    return _mutants[k]
+
+
+def xyz_source(w: Workload, n: int, first: int = 0, L=None) -> np.ndarray:
+    """Launch states of an mcxyz workload's photons: (n, 8) x, y, z, ux, uy, uz,
+    W, specular loss."""
+    L = L or lib()
+    out = np.zeros((n, 8), np.float32)
+    v = np.ascontiguousarray(w.v, np.uint8)
+    st = L.nmco_xyz_source(C.byref(w.rp), C.byref(w.tp), v.ctypes.data_as(C.POINTER(C.c_uint8)),
+                           C.byref(w.ext), first, n, out.ctypes.data_as(C.POINTER(C.c_float)))
+    if st != 0:
+        raise RuntimeError(f"oracle returned status {st}")
+    return out
+
+
+def sub_source(w: Workload, n: int, first: int = 0, L=None) -> np.ndarray:
+    """Launch states of an mcsub workload's photons: (n, 7) x, y, z, ux, uy, uz,
+    specular reflectance."""
+    L = L or lib()
+    out = np.zeros((n, 7), np.float32)
+    st = L.nmco_sub_source(C.byref(w.rp), C.byref(w.tp), C.byref(w.ext), first, n,
+                           out.ctypes.data_as(C.POINTER(C.c_float)))
+    if st != 0:
+        raise RuntimeError(f"oracle returned status {st}")
+    return out
 
 
 def philox(ctr, key) -> list[int]:
@@ -96,11 +153,12 @@ def fresnel(n1, n2, ca1):
 
 
 def run(w: Workload, n_photons: int | None = None, first: int = 0, threads: int = 0,
-        fluence: bool = True) -> P.TallyArrays:
-    """Transport photons [first, first+n) of a workload; returns host tallies."""
+        fluence: bool = True, L=None) -> P.TallyArrays:
+    """Transport photons [first, first+n) of a workload; returns host tallies
+    (L: another build of the oracle, e.g. a mutant)."""
     n = w.n_photons if n_photons is None else n_photons
     t = P.TallyArrays(w.rp, w.ext, w.sim, fluence=fluence)
-    L = lib()
+    L = L or lib()
     if w.sim == P.SIM_MCSUB:
         st = L.nmco_mcsub(C.byref(w.rp), C.byref(w.tp), C.byref(w.ext), first, n, threads,
                           C.byref(t.res))
@@ -123,17 +181,9 @@ class LayeredOut:
 
 
 def run_layered(cfg: P.LayeredConfig, stacks, first: int = 0, threads: int = 0,
-                records: bool = True) -> LayeredOut:
+                records: bool = True, L=None) -> LayeredOut:
     """Oracle run of an array of Stack structs; records (if asked) in photon order."""
-    L = lib()
-    if not hasattr(L, "_lay_sig"):
-        L.nmco_layered.argtypes = [C.POINTER(P.LayeredConfig), C.POINTER(P.Stack), C.c_int,
-                                   C.c_uint64, C.c_int, C.c_void_p, C.c_void_p,
-                                   C.POINTER(P.Totals), C.POINTER(C.c_uint64), C.c_uint64,
-                                   C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
-        L.nmco_layered_backward.argtypes = [C.POINTER(P.LayeredConfig), C.POINTER(P.Stack), C.c_int,
-                                            C.c_uint64, C.c_uint64] + [C.c_void_p] * 8
-        L._lay_sig = True
+    L = L or lib()
     ns = len(stacks)
     arr = (P.Stack * ns)(*stacks)
     o = LayeredOut()

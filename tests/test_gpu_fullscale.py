@@ -150,3 +150,98 @@ def test_fp32_tile_long_run_is_split(gpu):
     # FP32 sums of one workgroup's share of a <= 1e8-photon launch
     np.testing.assert_allclose(f32.fluence, f64.fluence, rtol=1e-4, atol=1e-12 * f64.fluence.max())
     assert abs(f32.fluence.sum() - f64.fluence.sum()) <= 1e-6 * f64.fluence.sum()
+
+
+def _batch_sums(run_batch, K):
+    """Sum and sum of squares over K batches of (fluence, ryx) and the count of
+    batches in which each bin is non-zero."""
+    acc = None
+    for k in range(K):
+        t = run_batch(k)
+        arrs = (t.fluence, t.ryx)
+        if acc is None:
+            acc = [[np.zeros_like(a), np.zeros_like(a), np.zeros(a.shape, np.int32)] for a in arrs]
+        for (s1, s2, nz), a in zip(acc, arrs):
+            s1 += a
+            s2 += a * a
+            nz += a != 0
+    return acc
+
+
+def test_skin_vessel_1e7_fp32_tile_every_bin(gpu):
+    """The benchmark's configuration (skin-vessel 200^3, FP32 LDS tile) at
+    1e7 photons, against the oracle with sigma per bin from K = 32 batch
+    means (batches of 312500 consecutive photon ids).
+    (a) Same photons [0, 1e7): integer tallies bit-exact; totals Rd, Tt, A
+        within 0.5 % (in fact to FP64 summation order); every fluence voxel
+        and R(x,y) pixel within 3 sigma (and within the FP32 tile's 2e-5).
+    (b) Disjoint photons, GPU [1e7, 2e7) against the oracle's [0, 1e7): the
+        totals Rd, Tt, A within 4.5 sigma of each other (Tt, ~8e-5 of the
+        light, carries ~1 % noise at this N); the per-bin z of the two totals,
+        over bins non-zero in >= 30 of 32 batches on both sides, exceeds 3
+        in no more than the t-tail share plus a binomial allowance."""
+    from scipy import stats
+
+    w = W.skin_vessel()
+    K, m = 32, 312_500
+    n = K * m
+    _say(f"[fullscale] oracle, skin-vessel, {K} batches of {m}")
+    ora_b = []
+
+    def ora_batch(k):
+        t = O.run(w, m, first=k * m, threads=16)
+        ora_b.append(t.totals)
+        return t
+    ora = _batch_sums(ora_batch, K)
+
+    with MetalMC(w.sim) as mc:
+        mc.prepare_data(w.rp, w.tp, w.v, w.ext)
+        mc.set_tuning(P.Tuning.make(mode=1))
+        _say(f"[fullscale] gpu fp32 tile, photons [0, {n})")
+        mc.set_num_runs_seed(w.ext.seed, 0)
+        mc.send_compute_command(n)
+        same = mc.get_compute_results()
+        gpu_b = []
+
+        def gpu_batch(k):
+            mc.reset_tallies()
+            mc.set_num_runs_seed(w.ext.seed, n + k * m)
+            mc.send_compute_command(m)
+            t = mc.get_compute_results()
+            gpu_b.append(t.totals)
+            return t
+        _say(f"[fullscale] gpu fp32 tile, {K} batches of disjoint photons")
+        gpu = _batch_sums(gpu_batch, K)
+
+    # (a) same photons
+    ora_tot = {q: sum(b[q] for b in ora_b) for q in ora_b[0]}
+    for q in P.TOTAL_INT_FIELDS:
+        assert same.totals[q] == ora_tot[q], q
+    for q in ("rd", "tt", "absorbed"):
+        assert abs(same.totals[q] - ora_tot[q]) <= 0.005 * ora_tot[q], q
+        assert same.totals[q] == pytest.approx(ora_tot[q], rel=1e-9), q
+    for name, (s1, s2, _), a in (("fluence", ora[0], same.fluence), ("ryx", ora[1], same.ryx)):
+        sig = np.sqrt(np.maximum(s2 - s1 * s1 / K, 0.0) * K / (K - 1))   # sd of the total
+        err = np.abs(a - s1)
+        assert (err <= 3 * sig + 1e-12 * s1.max()).all(), name
+        np.testing.assert_allclose(a, s1, rtol=2e-5, atol=1e-9)
+
+    # (b) independent photons
+    for q in ("rd", "tt", "absorbed"):
+        a = np.array([b[q] for b in gpu_b])
+        b_ = np.array([b[q] for b in ora_b])
+        sig = np.sqrt(K * (a.var(ddof=1) + b_.var(ddof=1)))
+        _say(f"[fullscale] {q}: gpu {a.sum() / n:.6e} oracle {b_.sum() / n:.6e} "
+             f"(rel {a.sum() / b_.sum() - 1:+.2e}, sigma {sig / b_.sum():.2e})")
+        assert abs(a.sum() - b_.sum()) <= 4.5 * sig, q
+    p_tail = 2 * stats.t.sf(3.0, df=2 * (K - 1))
+    for name, o, g in (("fluence", ora[0], gpu[0]), ("ryx", ora[1], gpu[1])):
+        use = (o[2] >= 30) & (g[2] >= 30)
+        var = lambda s: np.maximum(s[1] - s[0] * s[0] / K, 0.0) * K / (K - 1)
+        z = (g[0] - o[0])[use] / np.sqrt(var(g)[use] + var(o)[use])
+        frac = float((np.abs(z) > 3).mean())
+        allowed = p_tail + 5 * np.sqrt(p_tail / z.size)
+        _say(f"[fullscale] {name}: {z.size} bins, |z| > 3 in {frac:.4%} (allowed {allowed:.4%}), "
+             f"mean z {z.mean():+.3f}")
+        assert frac <= allowed, (name, frac, allowed)
+        assert abs(z.mean()) < 5 / np.sqrt(z.size) + 0.05, name
