@@ -211,9 +211,18 @@ typedef struct nmc_tuning {
                                per-item tallies fit), 1 LDS, 2 global atomics;
                                mcxyz: LDS tile element, 0 FP64, 1 FP32 (twice
                                the voxels; sums within a workgroup in FP32,
-                               across workgroups in FP64; a run of more than
-                               1e8 photons becomes launches of at most 1e8,
-                               the same photons);
+                               across workgroups in FP64; runs are split into
+                               launches of tuning.split photons per
+                               workgroup, the same photons), 2 FP32 tile with
+                               binned deposit records: a voxel visit outside
+                               the tile appends an 8-byte record (voxel,
+                               weight rounded to FP32) to the workgroup's
+                               slice of the voxel's 16384-voxel bucket
+                               instead of an FP64 atomic (slice capacities
+                               from a visit-count pilot at set-up; a full
+                               slice falls back to the atomic), and a
+                               reduction sums each bucket's records in FP64
+                               into the fluence; grids up to 2048 buckets;
                                layered: Rd(r) placement, read at
                                nmc_layered_prepare, 0 or 1 LDS histogram when
                                it fits without lowering occupancy, 2 global
@@ -223,7 +232,9 @@ typedef struct nmc_tuning {
                                one 1024-lane workgroup per CU of 256 CUs); a
                                longer run becomes consecutive launches of the
                                same photons, each flushing its tile in FP64  */
-    int32_t reserved[4];
+    int32_t rec_cap_pct;    /* mode 2: record slice capacity in per cent of
+                               the pilot's estimate (0: 100)                 */
+    int32_t reserved[3];
 } nmc_tuning;
 
 /* A rejected request (NMC_ERR_ARG, or an allocation failure) leaves the

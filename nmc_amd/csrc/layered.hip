@@ -88,7 +88,7 @@ __device__ __forceinline__ void fold_layer(LPhoton& p, double* absorbed) {
 __device__ __forceinline__ void flush_records(LPhoton& p, const LayArgs& a, int lane,
                                               uint64_t& next, uint64_t& end,
                                               uint32_t& written, uint32_t rec_nl_v,
-                                              uint32_t* rec) {
+                                              gptr<uint32_t> rec) {
   uint64_t m = ballot(p.rec_pending);
   if (!m) return;
   uint64_t c = (uint64_t)__popcll(m);
@@ -103,7 +103,7 @@ __device__ __forceinline__ void flush_records(LPhoton& p, const LayArgs& a, int 
     uint64_t k = rank < rem ? next + rank : fresh + (rank - rem);
     if (k < a.cap) {
       const uint32_t nl = fresh_vgpr(rec_nl_v);
-      uint32_t* r = rec + k * (uint64_t)(3u + 2u * nl);
+      gptr<uint32_t> r = rec + k * (uint64_t)(3u + 2u * nl);
       r[0] = (uint32_t)p.item;
       r[1] = (uint32_t)p.rec_ir;
       r[2] = __float_as_uint(p.W);
@@ -142,11 +142,12 @@ __global__ __launch_bounds__(kLayBlock) void layered_kernel(LayArgs a) {
   LayStackDev* hdr = (LayStackDev*)(rd_tile + rd_len);
   uint64_t* prefix = (uint64_t*)(hdr + ni);
   LayRow* rows = (LayRow*)(prefix + ni + 1);
-  double* rd_rep = RD_LDS ? nullptr
-                          : ptr_in_vgpr(a.rd + (int64_t)(blockIdx.x % (unsigned)a.rd_reps) * ni * (a.nr + 1));
+  gptr<double> rd_rep =
+      RD_LDS ? nullptr
+             : gptr_in_vgpr(a.rd + (int64_t)(blockIdx.x % (unsigned)a.rd_reps) * ni * (a.nr + 1));
   const uint32_t rd_row = in_vgpr((uint32_t)(a.nr + 1));
   const uint32_t rec_nl_v = in_vgpr((uint32_t)a.rec_nl);
-  uint32_t* const rec_v = RECORDS ? ptr_in_vgpr(a.rec) : nullptr;
+  const gptr<uint32_t> rec_v = RECORDS ? gptr_in_vgpr(a.rec) : nullptr;
   for (int i = threadIdx.x; i < ni * (NMC_MAX_LAYERS + TF_COUNT + TI_COUNT) + rd_len; i += kLayBlock)
     smem[i] = 0.0;
   for (int i = threadIdx.x; i < ni; i += kLayBlock) hdr[i] = a.stacks[i];
@@ -289,7 +290,7 @@ __global__ __launch_bounds__(kLayBlock) void layered_kernel(LayArgs a) {
         if (RD_LDS)
           unsafeAtomicAdd(&rd_tile[(uint32_t)p.item * rd_row + (uint32_t)ir], (double)p.W);
         else
-          unsafeAtomicAdd(&rd_rep[(uint32_t)p.item * rd_row + (uint32_t)ir], (double)p.W);
+          global_add(&rd_rep[(uint32_t)p.item * rd_row + (uint32_t)ir], (double)p.W);
         addf(T, TF_RD, (double)p.W);
         addi(T, TI_DETECTIONS, 1);
         if (RECORDS) {

@@ -34,6 +34,10 @@ namespace {
 struct Photon {
   float x, y, z, ux, uy, uz, rux, ruy, ruz, W, sleft, mua, mus, inv_mus;
   int ix, iy, iz, tis;
+  // vidx(ix, iy, iz), kept up to date by the crossings: a step of one voxel
+  // adds +-1, +-Ny or +-Nx*Ny, exact in 32-bit modular arithmetic also while
+  // the photon is outside the grid, so it is right again when it re-enters
+  uint32_t lin;
   bool inside, alive;
   uint64_t pid;
   uint32_t ev, events, b1, b2, b3;
@@ -59,27 +63,56 @@ __device__ __forceinline__ void lds_addf(nmc_totals* t, int k, double v) {
 // The LDS tile box and the fluence array, in vector registers (in_vgpr).
 struct TileBox {
   uint32_t x0, y0, z0, nx, ny, nz;
-  double* F;
+  gptr<double> F;
 };
 
 // Fluence tally of a kernel variant: none (NMC_FLAG_NO_FLUENCE), the
 // deposited weight, or the number of voxel visits (the tile-placement pilot).
 // Each is a kernel of its own, so the loop carries no per-visit mode test.
-enum : int { kFluNone = 0, kFluDeposit = 1, kFluCount = 2 };
+// kFluRecords: deposits, with the visits outside the tile appended as binned
+// records (tuning.mode 2) instead of FP64 atomics.
+enum : int { kFluNone = 0, kFluDeposit = 1, kFluCount = 2, kFluRecords = 3 };
+
+// A workgroup's slice of one bucket's record region, in LDS: the first
+// record index, the capacity and the slots taken so far.
+struct RecSlice {
+  unsigned long long base;
+  uint32_t cap, cnt;
+};
+
+// The record stream of a kernel variant (kFluRecords only): the slices in
+// LDS and the record array, in vector registers.
+struct RecOut {
+  RecSlice* slices;
+  gptr<unsigned long long> rec;   // as uint2 {voxel index in the bucket, weight bits}
+};
 
 // Adds the deposits of the current voxel visit to the tile or the global grid
-// (kFluCount: adds 1 per visit instead).
+// (kFluCount: adds 1 per visit instead; kFluRecords: appends a record instead
+// of the global atomic while the bucket's slice has room).
 template <int FMODE, typename TileT>
 __device__ __forceinline__ void flush_visit(Photon& p, const XyzArgs& a, const TileBox& tb,
-                                            TileT* tile) {
+                                            TileT* tile, const RecOut& ro) {
   constexpr bool COUNT = FMODE == kFluCount;
   if (p.inside && (COUNT || p.vdep != 0.0)) {
     double v = COUNT ? 1.0 : p.vdep;
     uint32_t lx = (uint32_t)p.ix - tb.x0, ly = (uint32_t)p.iy - tb.y0, lz = (uint32_t)p.iz - tb.z0;
-    if (lx < tb.nx && ly < tb.ny && lz < tb.nz)
+    if (lx < tb.nx && ly < tb.ny && lz < tb.nz) {
       unsafeAtomicAdd(&tile[(lz * tb.nx + lx) * tb.ny + ly], (TileT)v);
-    else
-      unsafeAtomicAdd(&tb.F[vidx(a, p.ix, p.iy, p.iz)], v);
+    } else {
+      const uint32_t k = p.lin;
+      if (FMODE == kFluRecords) {
+        RecSlice* sl = &ro.slices[k >> kBucketShift];
+        const uint32_t slot = atomicAdd(&sl->cnt, 1u);
+        if (slot < sl->cap) {
+          ro.rec[sl->base + slot] = ((unsigned long long)__float_as_uint((float)v) << 32) |
+                                    (k & (kBucketVox - 1u));
+          p.vdep = 0.0;
+          return;
+        }
+      }
+      global_add(&tb.F[k], v);
+    }
   }
   p.vdep = 0.0;
 }
@@ -207,7 +240,8 @@ __device__ __forceinline__ void launch(Photon& p, const XyzArgs& a, const XyzTis
     p.alive = false;
     return;
   }
-  p.tis = a.V[vidx(a, clampi(p.ix, a.Nx), clampi(p.iy, a.Ny), clampi(p.iz, a.Nz))];
+  p.lin = vidx(a, p.ix, p.iy, p.iz);
+  p.tis = a.V[p.inside ? p.lin : vidx(a, clampi(p.ix, a.Nx), clampi(p.iy, a.Ny), clampi(p.iz, a.Nz))];
   p.mua = T[p.tis].mua;
   p.mus = T[p.tis].mus;
   p.inv_mus = T[p.tis].inv_mus;
@@ -224,13 +258,21 @@ __device__ __forceinline__ void launch(Photon& p, const XyzArgs& a, const XyzTis
 
 template <bool TWO_TERM, int BLOCK, typename TileT, int FMODE>
 __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
+  // dynamic LDS: the record slices (kFluRecords), then the tile
   extern __shared__ double tile_mem[];
-  TileT* tile = (TileT*)tile_mem;  // [tile_nz][tile_nx][tile_ny]
+  constexpr bool RECS = FMODE == kFluRecords;
+  const int nb = RECS ? a.n_buckets : 0;
+  RecSlice* slices = (RecSlice*)tile_mem;
+  TileT* tile = (TileT*)(tile_mem + 2 * nb);  // [tile_nz][tile_nx][tile_ny]
   __shared__ XyzTissue T[NMC_NTISS];
   __shared__ nmc_totals st;
   const int tile_n = a.tile_nx * a.tile_ny * a.tile_nz;
   for (int i = threadIdx.x; i < NMC_NTISS; i += BLOCK) T[i] = a.tissue[i];
   for (int i = threadIdx.x; i < tile_n; i += BLOCK) tile[i] = (TileT)0;
+  for (int b = threadIdx.x; b < nb; b += BLOCK) {
+    const uint32_t cap = a.rec_cap[b];
+    slices[b] = RecSlice{a.rec_base[b] + (unsigned long long)blockIdx.x * cap, cap, 0u};
+  }
   if (threadIdx.x < TF_COUNT) ((double*)&st)[threadIdx.x] = 0.0;
   if (threadIdx.x < TI_COUNT) (&st.photons)[threadIdx.x] = 0ull;
   __syncthreads();
@@ -238,8 +280,10 @@ __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
   const TileBox tb{in_vgpr((uint32_t)a.tile_x0), in_vgpr((uint32_t)a.tile_y0),
                    in_vgpr((uint32_t)a.tile_z0), in_vgpr((uint32_t)a.tile_nx),
                    in_vgpr((uint32_t)a.tile_ny), in_vgpr((uint32_t)a.tile_nz),
-                   FMODE == kFluNone ? nullptr : ptr_in_vgpr(a.F)};
-  const uint8_t* const vmap = ptr_in_vgpr(a.V);
+                   FMODE == kFluNone ? nullptr : gptr_in_vgpr(a.F)};
+  const gptr<const uint8_t> vmap = gptr_in_vgpr(a.V);
+  const RecOut ro{RECS ? slices : nullptr,
+                  RECS ? gptr_in_vgpr((unsigned long long*)a.rec) : nullptr};
   // grid size, boundary and detector modes, likewise in vector registers, and
   // used in forms that leave no loop-invariant lane mask for the compiler to
   // keep in scalar registers: the detector plane's side as a sign (-1: the
@@ -249,6 +293,8 @@ __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
   // through any face)
   const int gnx = (int)in_vgpr((uint32_t)a.Nx), gny = (int)in_vgpr((uint32_t)a.Ny),
             gnz = (int)in_vgpr((uint32_t)a.Nz);
+  // the linear index steps of a y-, x- and z-crossing: 1, Ny, Nx*Ny
+  const uint32_t ny_v = in_vgpr((uint32_t)a.Ny), nxy_v = in_vgpr((uint32_t)a.Nx * (uint32_t)a.Ny);
   const uint32_t esc_top = in_vgpr(a.bmode == 1 || a.bmode == 2 ? 1u : 0u);
   const uint32_t esc_any = in_vgpr(a.bmode == 1 ? 1u : 0u);
   const float dsign = __uint_as_float(
@@ -264,6 +310,7 @@ __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
   p.W = 0.0f;
   p.vdep = 0.0;
   p.ix = p.iy = p.iz = 0;
+  p.lin = 0;
   p.inside = false;
   bool finished = false;
   uint64_t pool_next = 0, pool_end = 0;  // wave-uniform
@@ -380,16 +427,27 @@ __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
         if (!(p.W > 0.0f)) alive = false;
       } else {
         // voxel face (kernels.metal:641-679)
-        if (FMODE != kFluNone) flush_visit<FMODE>(p, a, tb, tile);
-        if (axis == 0) p.ix += p.ux > 0.0f ? 1 : -1;
-        else if (axis == 1) p.iy += p.uy > 0.0f ? 1 : -1;
-        else p.iz += p.uz > 0.0f ? 1 : -1;
+        if (FMODE != kFluNone) flush_visit<FMODE>(p, a, tb, tile, ro);
+        if (axis == 0) {
+          const bool fwd = p.ux > 0.0f;
+          p.ix += fwd ? 1 : -1;
+          p.lin += fwd ? ny_v : 0u - ny_v;
+        } else if (axis == 1) {
+          const bool fwd = p.uy > 0.0f;
+          p.iy += fwd ? 1 : -1;
+          p.lin += fwd ? 1u : 0xFFFFFFFFu;
+        } else {
+          const bool fwd = p.uz > 0.0f;
+          p.iz += fwd ? 1 : -1;
+          p.lin += fwd ? nxy_v : 0u - nxy_v;
+        }
         p.inside = p.ix >= 0 && p.ix < gnx && p.iy >= 0 && p.iy < gny && p.iz >= 0 && p.iz < gnz;
         if (!p.inside && (p.iz < 0 ? esc_top : esc_any) != 0u) {
           tally_escape(&st, a, p.iz, p.W);
           alive = false;
         } else {
-          p.tis = vmap[vidx(a, clampi(p.ix, gnx), clampi(p.iy, gny), clampi(p.iz, gnz))];
+          p.tis = vmap[p.inside ? p.lin
+                                : vidx(a, clampi(p.ix, gnx), clampi(p.iy, gny), clampi(p.iz, gnz))];
           p.mua = T[p.tis].mua;
           p.mus = T[p.tis].mus;
           p.inv_mus = T[p.tis].inv_mus;
@@ -433,7 +491,7 @@ __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
       }
     }
     if (!alive) {
-      if (FMODE != kFluNone) flush_visit<FMODE>(p, a, tb, tile);
+      if (FMODE != kFluNone) flush_visit<FMODE>(p, a, tb, tile, ro);
       p.alive = false;
     }
   }
@@ -450,6 +508,8 @@ __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
     double* part = a.tile_part + (int64_t)blockIdx.x * tile_n;
     for (int i = threadIdx.x; i < tile_n; i += BLOCK) part[i] = (double)tile[i];
   }
+  for (int b = threadIdx.x; b < nb; b += BLOCK)
+    a.rec_count[(int64_t)b * gridDim.x + blockIdx.x] = slices[b].cnt;
   if (threadIdx.x < TF_COUNT) unsafeAtomicAdd(&((double*)a.tot)[threadIdx.x], ((double*)&st)[threadIdx.x]);
   if (threadIdx.x < TI_COUNT)
     atomicAdd(&(&a.tot->photons)[threadIdx.x], (unsigned long long)(&st.photons)[threadIdx.x]);
@@ -467,6 +527,41 @@ __global__ void tile_reduce_kernel(const double* __restrict__ part, int grid, Xy
     int lx = t % a.tile_nx, lz = t / a.tile_nx;
     a.F[vidx(a, a.tile_x0 + lx, a.tile_y0 + ly, a.tile_z0 + lz)] += s;
   }
+}
+
+// F[voxels of bucket b] += the records of workgroup slices [w0, w1) of
+// bucket b (one RecPart per workgroup): summed in FP64 in LDS, then added
+// to F with one FP64 atomic per non-zero voxel.
+__global__ __launch_bounds__(1024) void rec_reduce_kernel(RecReduceArgs r) {
+  extern __shared__ double acc[];   // [kBucketVox]
+  const RecPart part = r.parts[blockIdx.x];
+  for (uint32_t i = threadIdx.x; i < kBucketVox; i += 1024) acc[i] = 0.0;
+  __syncthreads();
+  const unsigned long long base = r.rec_base[part.bucket];
+  const uint32_t cap = r.rec_cap[part.bucket];
+  for (int w = part.w0; w < part.w1; ++w) {
+    const uint32_t taken = r.rec_count[(int64_t)part.bucket * r.grid + w];
+    const uint32_t n = taken < cap ? taken : cap;
+    const uint2* src = r.rec + base + (unsigned long long)w * cap;
+    // four independent loads in flight per lane, then the remainder
+    uint32_t i = threadIdx.x;
+    for (; i + 3 * 1024 < n; i += 4 * 1024) {
+      const uint2 x0 = src[i], x1 = src[i + 1024], x2 = src[i + 2048], x3 = src[i + 3072];
+      unsafeAtomicAdd(&acc[x0.x], (double)__uint_as_float(x0.y));
+      unsafeAtomicAdd(&acc[x1.x], (double)__uint_as_float(x1.y));
+      unsafeAtomicAdd(&acc[x2.x], (double)__uint_as_float(x2.y));
+      unsafeAtomicAdd(&acc[x3.x], (double)__uint_as_float(x3.y));
+    }
+    for (; i < n; i += 1024) {
+      const uint2 x = src[i];
+      unsafeAtomicAdd(&acc[x.x], (double)__uint_as_float(x.y));
+    }
+  }
+  __syncthreads();
+  const uint32_t v0 = (uint32_t)part.bucket << kBucketShift;
+  const uint32_t nv = r.nvox - v0 < kBucketVox ? r.nvox - v0 : kBucketVox;
+  for (uint32_t i = threadIdx.x; i < nv; i += 1024)
+    if (acc[i] != 0.0) unsafeAtomicAdd(&r.F[v0 + i], acc[i]);
 }
 
 template <bool TT, typename TileT, int FMODE>
@@ -492,28 +587,44 @@ const void* kernel_of(int block) {
                         : (const void*)mcxyz_kernel<TT, 1024, TileT, FMODE>;
 }
 
+// The record variant exists for the FP32 tile only (tuning.mode 2).
 template <int FMODE>
 const void* kernel_for(bool two_term, bool tile32, int block) {
-  if (two_term)
-    return tile32 ? kernel_of<true, float, FMODE>(block) : kernel_of<true, double, FMODE>(block);
-  return tile32 ? kernel_of<false, float, FMODE>(block) : kernel_of<false, double, FMODE>(block);
+  if constexpr (FMODE == kFluRecords) {
+    return two_term ? kernel_of<true, float, kFluRecords>(block)
+                    : kernel_of<false, float, kFluRecords>(block);
+  } else {
+    if (two_term)
+      return tile32 ? kernel_of<true, float, FMODE>(block) : kernel_of<true, double, FMODE>(block);
+    return tile32 ? kernel_of<false, float, FMODE>(block) : kernel_of<false, double, FMODE>(block);
+  }
 }
 
 template <int FMODE>
 hipError_t launch_variant(const XyzArgs& a, bool two_term, bool tile32, int block, int grid,
                           size_t lds_bytes, hipStream_t s) {
-  if (two_term)
-    return tile32 ? launch_block<true, float, FMODE>(a, block, grid, lds_bytes, s)
-                  : launch_block<true, double, FMODE>(a, block, grid, lds_bytes, s);
-  return tile32 ? launch_block<false, float, FMODE>(a, block, grid, lds_bytes, s)
-                : launch_block<false, double, FMODE>(a, block, grid, lds_bytes, s);
+  if constexpr (FMODE == kFluRecords) {
+    return two_term ? launch_block<true, float, kFluRecords>(a, block, grid, lds_bytes, s)
+                    : launch_block<false, float, kFluRecords>(a, block, grid, lds_bytes, s);
+  } else {
+    if (two_term)
+      return tile32 ? launch_block<true, float, FMODE>(a, block, grid, lds_bytes, s)
+                    : launch_block<true, double, FMODE>(a, block, grid, lds_bytes, s);
+    return tile32 ? launch_block<false, float, FMODE>(a, block, grid, lds_bytes, s)
+                  : launch_block<false, double, FMODE>(a, block, grid, lds_bytes, s);
+  }
 }
 
 const void* kernel_for_mode(int fmode, bool two_term, bool tile32, int block) {
-  return fmode == kFluNone    ? kernel_for<kFluNone>(two_term, tile32, block)
-         : fmode == kFluCount ? kernel_for<kFluCount>(two_term, tile32, block)
-                              : kernel_for<kFluDeposit>(two_term, tile32, block);
+  return fmode == kFluNone      ? kernel_for<kFluNone>(two_term, tile32, block)
+         : fmode == kFluCount   ? kernel_for<kFluCount>(two_term, tile32, block)
+         : fmode == kFluRecords ? kernel_for<kFluRecords>(two_term, tile32, block)
+                                : kernel_for<kFluDeposit>(two_term, tile32, block);
 }
+
+// The variants a launch configuration may run: with records (tile32 and the
+// record stream) also the record variant.
+int last_mode(bool records) { return records ? kFluRecords : kFluCount; }
 
 }  // namespace
 
@@ -522,11 +633,13 @@ hipError_t launch_mcxyz(const XyzArgs& a, bool two_term, bool tile32, int block,
   if (!a.F) return launch_variant<kFluNone>(a, two_term, tile32, block, grid, lds_bytes, s);
   if (a.count_visits)
     return launch_variant<kFluCount>(a, two_term, tile32, block, grid, lds_bytes, s);
+  if (a.n_buckets > 0 && tile32)
+    return launch_variant<kFluRecords>(a, two_term, tile32, block, grid, lds_bytes, s);
   return launch_variant<kFluDeposit>(a, two_term, tile32, block, grid, lds_bytes, s);
 }
 
 hipError_t mcxyz_configure(bool two_term, bool tile32, int block, size_t lds_bytes) {
-  for (int m = kFluNone; m <= kFluCount; ++m) {
+  for (int m = kFluNone; m <= last_mode(tile32); ++m) {
     hipError_t e = hipFuncSetAttribute(kernel_for_mode(m, two_term, tile32, block),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
     if (e != hipSuccess) return e;
@@ -537,7 +650,7 @@ hipError_t mcxyz_configure(bool two_term, bool tile32, int block, size_t lds_byt
 // Workgroups per CU that every fluence variant reaches.
 int mcxyz_blocks_per_cu(bool two_term, bool tile32, int block, size_t lds_bytes) {
   int best = 0;
-  for (int m = kFluNone; m <= kFluCount; ++m) {
+  for (int m = kFluNone; m <= last_mode(tile32); ++m) {
     int nb = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel_for_mode(m, two_term, tile32, block),
                                                      block, lds_bytes) != hipSuccess)
@@ -545,6 +658,18 @@ int mcxyz_blocks_per_cu(bool two_term, bool tile32, int block, size_t lds_bytes)
     best = m == kFluNone || nb < best ? nb : best;
   }
   return best;
+}
+
+hipError_t rec_reduce_configure() {
+  return hipFuncSetAttribute((const void*)rec_reduce_kernel,
+                             hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)(kBucketVox * sizeof(double)));
+}
+
+hipError_t launch_rec_reduce(const RecReduceArgs& r, int n_parts, hipStream_t s) {
+  if (n_parts <= 0) return hipSuccess;
+  hipLaunchKernelGGL(rec_reduce_kernel, dim3(n_parts), dim3(1024), kBucketVox * sizeof(double), s, r);
+  return hipGetLastError();
 }
 
 hipError_t launch_tile_reduce(const double* part, int grid, const XyzArgs& a, hipStream_t s) {

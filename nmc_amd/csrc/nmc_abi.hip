@@ -167,6 +167,7 @@ struct nmc_handle {
   XyzAux* dAux = nullptr;
   double* dF = nullptr;
   double* dTilePart = nullptr;
+  XyzRecords recs;             // binned deposit records (tuning.mode 2)
   double* dRyx = nullptr;
   uint32_t* dDet = nullptr;
   size_t xyz_lds = 0;
@@ -232,8 +233,18 @@ void drop_graph(nmc_handle* h) {
   h->graph_n = 0;
 }
 
+void free_records(XyzRecords& r) {
+  free_ptr(r.base);
+  free_ptr(r.cap);
+  free_ptr(r.rec);
+  free_ptr(r.count);
+  free_ptr(r.parts);
+  r = XyzRecords{};
+}
+
 void free_buffers(nmc_handle* h) {
   drop_graph(h);
+  free_records(h->recs);
   free_ptr(h->dV);
   free_ptr(h->dTis);
   free_ptr(h->dAux);
@@ -290,16 +301,33 @@ int resolve_xyz_launch(const RunParams& rp, int sim, bool has_fluence, const nmc
   t.chunk = r.chunk;
   if (r.split < 0) return fail(NMC_ERR_ARG, "tuning.split must be >= 0");
   bool tt = sim == NMC_SIM_MCXYZ_TT;
-  if (r.mode < 0 || r.mode > 1) return fail(NMC_ERR_ARG, "tuning.mode (tile element) not 0 or 1");
+  if (r.mode < 0 || r.mode > 2)
+    return fail(NMC_ERR_ARG, "tuning.mode (tile element) not 0, 1 or 2");
+  if (r.rec_cap_pct < 0) return fail(NMC_ERR_ARG, "tuning.rec_cap_pct must be >= 0");
   t.mode = r.mode;
-  const bool tile32 = t.mode == 1;
+  t.rec_cap_pct = r.rec_cap_pct;
+  const bool tile32 = t.mode >= 1;
   t.split = tile32 ? (r.split ? r.split : kFp32TileSplit) : 0;
+  int n_buckets = 0;
+  if (t.mode == 2 && has_fluence) {
+    if (part_cap >= 0)
+      return fail(NMC_ERR_ARG, "tuning.mode 2 (binned deposit records) needs its record buffer; "
+                  "not available without allocation");
+    const int64_t nv = (int64_t)rp.Nx * rp.Ny * rp.Nz;
+    const int64_t nb = (nv + kBucketVox - 1) / kBucketVox;
+    if (nb > kMaxBuckets)
+      return fail(NMC_ERR_ARG, "tuning.mode 2: %lld voxels need %lld record buckets (at most %d)",
+                  (long long)nv, (long long)nb, kMaxBuckets);
+    n_buckets = (int)nb;
+  }
   const size_t elem = tile32 ? sizeof(float) : sizeof(double);
   const size_t static_lds = sizeof(XyzTissue) * NMC_NTISS + sizeof(nmc_totals) + 256;
-  const int64_t lds_budget = lds_max > static_lds ? (int64_t)((lds_max - static_lds) / elem) : 0;
+  const size_t rec_lds = (size_t)n_buckets * 16;
+  const int64_t lds_budget =
+      lds_max > static_lds + rec_lds ? (int64_t)((lds_max - static_lds - rec_lds) / elem) : 0;
   // the kernels' dynamic-LDS limit is set to the whole budget, so no other
   // configuration can fall below what it was resolved for
-  HIP_TRY(mcxyz_configure(tt, tile32, t.block, (size_t)lds_budget * elem));
+  HIP_TRY(mcxyz_configure(tt, tile32, t.block, rec_lds + (size_t)lds_budget * elem));
   int64_t budget = lds_budget;
   if (part_cap >= 0) {
     const int occ0 = mcxyz_blocks_per_cu(tt, tile32, t.block, 0);
@@ -337,7 +365,7 @@ int resolve_xyz_launch(const RunParams& rp, int sim, bool has_fluence, const nmc
     if (tile_n > lds_budget) return fail(NMC_ERR_ARG, "tuning.tile needs %lld voxels, LDS holds %lld",
                                          (long long)tile_n, (long long)lds_budget);
   }
-  const size_t lds = (size_t)tile_n * elem;
+  const size_t lds = rec_lds + (size_t)tile_n * elem;
   int occ = mcxyz_blocks_per_cu(tt, tile32, t.block, lds);
   if (occ <= 0) return fail(NMC_ERR_ARG, "kernel does not fit on a CU with block %d, %zu B LDS", t.block, lds);
   t.blocks_per_cu = r.blocks_per_cu > 0 ? std::min(r.blocks_per_cu, occ) : occ;
@@ -349,6 +377,7 @@ int resolve_xyz_launch(const RunParams& rp, int sim, bool has_fluence, const nmc
   out->lds = lds;
   out->grid = grid;
   out->tile_n = tile_n;
+  out->n_buckets = n_buckets;
   return ok();
 }
 
@@ -392,7 +421,24 @@ XyzArgs xyz_args(const RunParams& rp, const nmc_ext_config& e, int sim, uint32_t
 // the FP32 tile's accuracy was measured at, whatever the grid.  The launches
 // take consecutive photon ids, so the photons are the same.
 int enqueue_xyz(XyzArgs a, bool two_term, const XyzLaunch& L, uint64_t n, hipStream_t s,
-                uint32_t* launches) {
+                uint32_t* launches, const XyzRecords* R) {
+  const bool recs = R && R->n_buckets > 0 && a.F && !a.count_visits;
+  a.n_buckets = recs ? R->n_buckets : 0;
+  a.rec_base = recs ? R->base : nullptr;
+  a.rec_cap = recs ? R->cap : nullptr;
+  a.rec = recs ? R->rec : nullptr;
+  a.rec_count = recs ? R->count : nullptr;
+  RecReduceArgs rr{};
+  if (recs) {
+    rr.parts = R->parts;
+    rr.rec_base = R->base;
+    rr.rec_cap = R->cap;
+    rr.rec = R->rec;
+    rr.rec_count = R->count;
+    rr.F = a.F;
+    rr.grid = L.grid;
+    rr.nvox = (uint32_t)((uint64_t)a.Nx * a.Ny * a.Nz);
+  }
   a.tile_x0 = L.t.tile[0];
   a.tile_y0 = L.t.tile[1];
   a.tile_z0 = L.t.tile[2];
@@ -400,15 +446,16 @@ int enqueue_xyz(XyzArgs a, bool two_term, const XyzLaunch& L, uint64_t n, hipStr
   a.tile_ny = L.t.tile[4];
   a.tile_nz = L.t.tile[5];
   a.refill_min = (uint32_t)L.t.refill_min;
-  const uint64_t cap = L.t.mode == 1 ? (uint64_t)L.t.split * (uint64_t)L.grid : 0;
+  const uint64_t cap = L.t.mode >= 1 ? (uint64_t)L.t.split * (uint64_t)L.grid : 0;
   const uint64_t step = cap > 0 ? cap : n;
   for (uint64_t done = 0; done < n; done += step) {
     const uint64_t part = std::min<uint64_t>(step, n - done);
     a.n = part;
     a.chunk = pick_chunk(part, L.grid * (L.t.block / 64), L.t.chunk);
     HIP_TRY(launch_state_begin(a.state, s));
-    HIP_TRY(launch_mcxyz(a, two_term, L.t.mode == 1, L.t.block, L.grid, L.lds, s));
+    HIP_TRY(launch_mcxyz(a, two_term, L.t.mode >= 1, L.t.block, L.grid, L.lds, s));
     HIP_TRY(launch_tile_reduce(a.tile_part, L.grid, a, s));
+    if (recs) HIP_TRY(launch_rec_reduce(rr, R->n_parts, s));
     HIP_TRY(launch_state_advance(a.state, part, s));
     ++*launches;
   }
@@ -457,18 +504,166 @@ void xyz_tables(const RunParams& rp, const TissueParams& tp, int sim, XyzTissue*
 
 namespace {
 
+uint32_t key0_of(const nmc_handle* h) { return (uint32_t)h->ext.seed; }
+uint32_t key1_of(const nmc_handle* h, uint32_t stream) { return (uint32_t)(h->ext.seed >> 32) + stream; }
+
+// Visits outside the tile box per photon, per record bucket: a visit-count
+// launch of kRecPilot photons (no tile, FP64 counts) on scratch buffers, so
+// the handle's tallies and photon state are untouched.
+constexpr uint64_t kRecPilot = 200000;
+
+int record_rates(nmc_handle* h, const XyzLaunch& L, std::vector<double>* rate) {
+  const RunParams& rp = h->rp;
+  const int64_t nv = nvox(&rp), n2 = (int64_t)rp.Nx * rp.Ny;
+  double* cnt = nullptr;
+  double* ryx = nullptr;
+  uint32_t* det = nullptr;
+  nmc_totals* tot = nullptr;
+  RunState* st = nullptr;
+  XyzAux* aux = nullptr;
+  auto release = [&] {
+    free_ptr(cnt);
+    free_ptr(ryx);
+    free_ptr(det);
+    free_ptr(tot);
+    free_ptr(st);
+    free_ptr(aux);
+  };
+  hipError_t e = hipMalloc(&cnt, nv * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&ryx, n2 * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&det, n2 * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMalloc(&tot, sizeof(nmc_totals));
+  if (e == hipSuccess) e = hipMalloc(&st, sizeof(RunState));
+  if (e == hipSuccess) e = hipMalloc(&aux, sizeof(XyzAux));
+  if (e == hipSuccess) e = hipMemsetAsync(cnt, 0, nv * sizeof(double), h->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(ryx, 0, n2 * sizeof(double), h->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(det, 0, n2 * sizeof(uint32_t), h->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(tot, 0, sizeof(nmc_totals), h->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(st, 0, sizeof(RunState), h->stream);
+  if (e == hipSuccess) {
+    std::vector<XyzTissue> tis(NMC_NTISS);
+    XyzAux ax{};
+    xyz_tables(rp, h->tp, h->sim, tis.data(), &ax);
+    ax.ryx = ryx;
+    ax.det2d = det;
+    e = hipMemcpyAsync(aux, &ax, sizeof ax, hipMemcpyHostToDevice, h->stream);
+  }
+  if (e != hipSuccess) {
+    release();
+    return fail(e == hipErrorOutOfMemory ? NMC_ERR_NOMEM : NMC_ERR_DEVICE, "record pilot: %s",
+                hipGetErrorString(e));
+  }
+  XyzArgs a = xyz_args(rp, h->ext, h->sim, key0_of(h), key1_of(h, h->ext.stream_id));
+  a.count_visits = 1;
+  a.aux = aux;
+  a.V = h->dV;
+  a.tissue = h->dTis;
+  a.F = cnt;
+  a.tot = tot;
+  a.state = st;
+  XyzLaunch P = L;
+  for (int i = 0; i < 6; ++i) P.t.tile[i] = 0;
+  P.t.mode = 0;
+  P.t.split = 0;
+  P.tile_n = 0;
+  P.lds = 0;
+  P.n_buckets = 0;
+  uint32_t launches = 0;
+  int rc = enqueue_xyz(a, h->sim == NMC_SIM_MCXYZ_TT, P, kRecPilot, h->stream, &launches, nullptr);
+  std::vector<double> host;
+  if (rc == NMC_OK) {
+    host.resize(nv);
+    e = hipMemcpyAsync(host.data(), cnt, nv * sizeof(double), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess)
+      rc = fail(NMC_ERR_DEVICE, "record pilot: %s", hipGetErrorString(e));
+  }
+  release();
+  if (rc != NMC_OK) return rc;
+  rate->assign(L.n_buckets, 0.0);
+  const int* b = L.t.tile;
+  for (int64_t v = 0; v < nv; ++v) {
+    if (host[v] == 0.0) continue;
+    const int64_t iy = v % rp.Ny, ix = (v / rp.Ny) % rp.Nx, iz = v / ((int64_t)rp.Ny * rp.Nx);
+    const bool in_tile = ix >= b[0] && ix < b[0] + b[3] && iy >= b[1] && iy < b[1] + b[4] &&
+                         iz >= b[2] && iz < b[2] + b[5];
+    if (!in_tile) (*rate)[v >> kBucketShift] += host[v];
+  }
+  for (double& r : *rate) r /= (double)kRecPilot;
+  return ok();
+}
+
+// Record buffers for launch shape L: each bucket's workgroup slices hold the
+// pilot's expected out-of-tile visits of tuning.split photons (plus 15 % for
+// uneven shares of the queue, 10 % and 6 sigma of Poisson slack, 32 records),
+// scaled by tuning.rec_cap_pct percent when set; the reduction takes about
+// 4e6 records per workgroup.
+int make_records(nmc_handle* h, const XyzLaunch& L, XyzRecords* R) {
+  std::vector<double> rate;
+  NMC_TRY(record_rates(h, L, &rate));
+  const int nb = L.n_buckets, grid = L.grid;
+  const double per_wg = 1.15 * (double)L.t.split;
+  const double pct = L.t.rec_cap_pct > 0 ? L.t.rec_cap_pct / 100.0 : 1.0;
+  std::vector<unsigned long long> base(nb);
+  std::vector<uint32_t> cap(nb);
+  std::vector<RecPart> parts;
+  unsigned long long total = 0;
+  for (int b = 0; b < nb; ++b) {
+    const double e = rate[b] * per_wg;
+    const double c = std::ceil((1.1 * e + 6.0 * std::sqrt(e) + 32.0) * pct);
+    if (c > 4.0e9) return fail(NMC_ERR_ARG, "tuning.split: record slices above 2^32 records");
+    cap[b] = (uint32_t)c;
+    base[b] = total;
+    total += (unsigned long long)cap[b] * grid;
+    const double expect = rate[b] * (double)L.t.split * grid;
+    int np = (int)std::min<double>(grid, std::max(1.0, std::ceil(expect / 4.0e6)));
+    for (int k = 0; k < np; ++k)
+      parts.push_back(RecPart{b, (int)((int64_t)grid * k / np), (int)((int64_t)grid * (k + 1) / np), 0});
+  }
+  R->n_buckets = nb;
+  R->n_parts = (int)parts.size();
+  R->n_rec = total;
+  HIP_TRY(rec_reduce_configure());
+  HIP_TRY(hipMalloc(&R->base, nb * sizeof(unsigned long long)));
+  HIP_TRY(hipMalloc(&R->cap, nb * sizeof(uint32_t)));
+  HIP_TRY(hipMalloc(&R->count, (size_t)nb * grid * sizeof(uint32_t)));
+  HIP_TRY(hipMalloc(&R->parts, parts.size() * sizeof(RecPart)));
+  HIP_TRY(hipMalloc(&R->rec, std::max<unsigned long long>(total, 1) * sizeof(uint2)));
+  HIP_TRY(hipMemcpy(R->base, base.data(), nb * sizeof(unsigned long long), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(R->cap, cap.data(), nb * sizeof(uint32_t), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(R->parts, parts.data(), parts.size() * sizeof(RecPart), hipMemcpyHostToDevice));
+  return ok();
+}
+
 // Resolves h->tune_req for an mcxyz handle.  Transactional: every check runs
 // and every new buffer is allocated before the handle changes, so a rejected
 // request leaves the previous configuration (and its buffers) in force.
 int resolve_xyz_tuning(nmc_handle* h) {
   XyzLaunch L{};
   NMC_TRY(resolve_xyz_launch(h->rp, h->sim, h->dF != nullptr, h->tune_req, h->lds_max, h->num_cu, -1, &L));
+  XyzRecords R;
+  if (L.n_buckets > 0) {
+    int st = make_records(h, L, &R);
+    if (st != NMC_OK) {
+      free_records(R);
+      return st;
+    }
+  }
   double* part = nullptr;
-  if (L.tile_n > 0) HIP_TRY(hipMalloc(&part, (size_t)L.grid * L.tile_n * sizeof(double)));
+  if (L.tile_n > 0) {
+    hipError_t e = hipMalloc(&part, (size_t)L.grid * L.tile_n * sizeof(double));
+    if (e != hipSuccess) {
+      free_records(R);
+      return fail(e == hipErrorOutOfMemory ? NMC_ERR_NOMEM : NMC_ERR_DEVICE, "tile partials: %s",
+                  hipGetErrorString(e));
+    }
+  }
   // commit: the cached graph holds the old buffers, so it goes first
   drop_graph(h);
   free_ptr(h->dTilePart);
+  free_records(h->recs);
   h->dTilePart = part;
+  h->recs = R;
   h->xyz_lds = L.lds;
   h->xyz_grid = L.grid;
   h->tune = L.t;
@@ -535,8 +730,6 @@ int resolve_sub_tuning(nmc_handle* h) {
   return ok();
 }
 
-uint32_t key0_of(const nmc_handle* h) { return (uint32_t)h->ext.seed; }
-uint32_t key1_of(const nmc_handle* h, uint32_t stream) { return (uint32_t)(h->ext.seed >> 32) + stream; }
 
 // Enqueues one run (no events, no capture handling).
 int enqueue_run(nmc_handle* h, uint64_t n) {
@@ -598,9 +791,10 @@ int enqueue_run(nmc_handle* h, uint64_t n) {
     a.tile_part = h->dTilePart;
     a.tot = h->dTot;
     a.state = h->dState;
-    XyzLaunch L{h->tune, h->xyz_lds, h->xyz_grid, 0};
+    XyzLaunch L{h->tune, h->xyz_lds, h->xyz_grid, 0, 0};
     uint32_t launches = 0;
-    NMC_TRY(enqueue_xyz(a, h->sim == NMC_SIM_MCXYZ_TT, L, n, s, &launches));
+    L.n_buckets = h->recs.n_buckets;
+    NMC_TRY(enqueue_xyz(a, h->sim == NMC_SIM_MCXYZ_TT, L, n, s, &launches, &h->recs));
     h->launches += launches;
   }
   return ok();
@@ -1603,6 +1797,6 @@ extern "C" int nmc_mcxyz_run_noalloc(int sim_type, const RunParams* rp, const Ti
   a.tot = d_totals;
   a.state = dstate;
   uint32_t launches = 0;
-  NMC_TRY(enqueue_xyz(a, sim_type == NMC_SIM_MCXYZ_TT, L, n_photons, s, &launches));
+  NMC_TRY(enqueue_xyz(a, sim_type == NMC_SIM_MCXYZ_TT, L, n_photons, s, &launches, nullptr));
   return ok();
 }

@@ -378,6 +378,26 @@ __device__ __forceinline__ T* ptr_in_vgpr(T* p) {
   return (T*)(((uint64_t)in_vgpr((uint32_t)(u >> 32)) << 32) | in_vgpr((uint32_t)u));
 }
 
+// A pointer into device (global) memory, typed as such: loads, stores and
+// atomics through it compile to global_* instructions.  Through a generic
+// pointer they become flat_* instructions, which count on the LDS wait
+// counter as well, so every later wait for an LDS access would also wait for
+// them (a fluence atomic or record store stalling the next tissue-table read).
+template <typename T>
+using gptr = __attribute__((address_space(1))) T*;
+
+// ptr_in_vgpr for a pointer known to address device memory.
+template <typename T>
+__device__ __forceinline__ gptr<T> gptr_in_vgpr(T* p) {
+  const uint64_t u = (uint64_t)p;
+  return (gptr<T>)(((uint64_t)in_vgpr((uint32_t)(u >> 32)) << 32) | in_vgpr((uint32_t)u));
+}
+
+// FP64 add to device memory without a return value (global_atomic_add_f64).
+__device__ __forceinline__ void global_add(gptr<double> p, double v) {
+  (void)__hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // The same value, redefined for the compiler at this point: comparisons with
 // it are evaluated where they are used instead of being hoisted out of the
 // loop as lane masks, which would occupy (or spill) scalar registers.

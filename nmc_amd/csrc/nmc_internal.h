@@ -72,6 +72,40 @@ struct XyzArgs {
   double* tile_part;                // [grid][tile voxels] block partials of the tile
   nmc_totals* tot;
   RunState* state;
+  // binned deposit records (tuning.mode 2): a visit outside the tile to a
+  // voxel of bucket b (voxels [b kBucketVox, (b + 1) kBucketVox)) is appended
+  // to the workgroup's slice rec[rec_base[b] + blockIdx.x rec_cap[b] + slot]
+  // of that bucket, slot from a counter in LDS; beyond the slice's capacity it
+  // falls back to the FP64 atomic.  n_buckets = 0: no records.
+  int n_buckets;
+  const unsigned long long* rec_base;   // [n_buckets] first record of the bucket's region
+  const uint32_t* rec_cap;              // [n_buckets] records per workgroup slice
+  uint2* rec;                           // {voxel index within the bucket, weight bits (f32)}
+  uint32_t* rec_count;                  // [n_buckets][grid] slots taken (may exceed rec_cap)
+};
+
+// Buckets of the binned deposit records: 2^14 voxels, whose FP64 sums fill
+// 128 KB of LDS in the reduction.
+constexpr int kBucketShift = 14;
+constexpr uint32_t kBucketVox = 1u << kBucketShift;
+// At most this many buckets (their slices take 16 B of the transport
+// kernel's LDS each): grids up to 2048 x 16384 = 33.5 M voxels.
+constexpr int kMaxBuckets = 2048;
+
+// One workgroup of the record reduction: bucket b, workgroup slices [w0, w1).
+struct RecPart {
+  int bucket, w0, w1, pad;
+};
+
+struct RecReduceArgs {
+  const RecPart* parts;
+  const unsigned long long* rec_base;
+  const uint32_t* rec_cap;
+  const uint2* rec;
+  const uint32_t* rec_count;
+  double* F;
+  int grid;                         // transport workgroups (slices per bucket)
+  uint32_t nvox;
 };
 
 // One medium of a (batched) mcsub launch.
@@ -184,6 +218,19 @@ struct XyzLaunch {
   size_t lds;
   int grid;
   int64_t tile_n;
+  int n_buckets;                    // > 0: binned deposit records (tuning.mode 2)
+};
+
+// Device buffers of the binned deposit records of an mcxyz handle.
+struct XyzRecords {
+  int n_buckets = 0;
+  unsigned long long* base = nullptr;   // [n_buckets]
+  uint32_t* cap = nullptr;              // [n_buckets]
+  uint2* rec = nullptr;                 // sum over b of cap[b] * grid records
+  uint32_t* count = nullptr;            // [n_buckets][grid]
+  RecPart* parts = nullptr;             // reduction workgroups
+  int n_parts = 0;
+  uint64_t n_rec = 0;                   // records the buffer holds
 };
 
 hipError_t launch_mcxyz(const XyzArgs& a, bool two_term, bool tile32, int block, int grid,
@@ -191,6 +238,8 @@ hipError_t launch_mcxyz(const XyzArgs& a, bool two_term, bool tile32, int block,
 hipError_t mcxyz_configure(bool two_term, bool tile32, int block, size_t lds_bytes);
 int mcxyz_blocks_per_cu(bool two_term, bool tile32, int block, size_t lds_bytes);
 hipError_t launch_tile_reduce(const double* part, int grid, const XyzArgs& a, hipStream_t s);
+hipError_t launch_rec_reduce(const RecReduceArgs& r, int n_parts, hipStream_t s);
+hipError_t rec_reduce_configure();
 hipError_t launch_mcsub(const SubArgs& a, int block, int grid, size_t lds_bytes, bool lds_tallies,
                         hipStream_t s);
 hipError_t mcsub_configure(int block, size_t lds_bytes, bool lds_tallies);

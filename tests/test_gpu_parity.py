@@ -126,6 +126,49 @@ def test_fp32_tile(gpu):
     assert abs(got.fluence.sum() - ref.fluence.sum()) <= 1e-6 * ref.fluence.sum()
 
 
+def _run_tuned(w, n, tuning, graph_runs=0):
+    with MetalMC(w.sim) as mc:
+        mc.prepare_data(w.rp, w.tp, w.v, w.ext)
+        mc.set_tuning(tuning)
+        eff = mc.get_tuning()
+        out = []
+        if graph_runs:
+            mc.set_graph_mode(True)
+        for _ in range(max(1, graph_runs)):
+            mc.reset_tallies()
+            mc.set_num_runs_seed(w.ext.seed, 0)
+            mc.send_compute_command(n)
+            out.append(mc.get_compute_results())
+    return eff, out
+
+
+def _assert_fp32_class(got, ref, label):
+    """Integer tallies bit-exact; fluence within the FP32 tile's bound."""
+    for k in P.TOTAL_INT_FIELDS:
+        assert got.totals[k] == ref.totals[k], (label, k)
+    assert np.array_equal(got.det2d, ref.det2d), label
+    np.testing.assert_allclose(got.fluence, ref.fluence, rtol=2e-5, atol=1e-9, err_msg=label)
+    assert abs(got.fluence.sum() - ref.fluence.sum()) <= 1e-6 * ref.fluence.sum(), label
+
+
+@pytest.mark.parametrize("case", ["skin", "overflow", "split", "two_term", "graph"])
+def test_binned_deposit_records(gpu, case):
+    """tuning.mode 2 (FP32 tile + binned deposit records): the same photons as
+    the oracle, fluence within the FP32 tile's bound; with 1 % of the pilot's
+    slice capacity most visits take the atomic fallback; split launches each
+    reduce their own records; graph replays equal the oracle every time."""
+    w = W.tiny_voxel_case("two_term") if case == "two_term" else W.skin_vessel()
+    n = 20_000 if case == "two_term" else 200_000
+    _say(f"[parity] binned deposit records: {case}, n={n}")
+    ref = O.run(w, n, threads=8)
+    t = P.Tuning.make(mode=2, rec_cap_pct=1 if case == "overflow" else 0,
+                      split=50 if case == "split" else 0)
+    eff, outs = _run_tuned(w, n, t, graph_runs=3 if case == "graph" else 0)
+    assert eff.mode == 2 and eff.tile[3] > 0
+    for i, got in enumerate(outs):
+        _assert_fp32_class(got, ref, f"{case} #{i}")
+
+
 def test_autotuner_runs(gpu):
     from nmc_amd.autotune import autotune_mcxyz
 
