@@ -8,14 +8,19 @@ recorded by the library on the stream the kernel runs on.  Prints one JSON
 line (see the keys below); the CPU baseline is the oracle run on this host's
 cores over a bounded sample of the same workload.
 
-The timed steps use the FP32 LDS fluence tile (sums within a workgroup in
-FP32, across workgroups and outside the tile in FP64; the reference adds every
-deposit into an FP32 global array) on the box the visit-count pilot finds;
-`--tile fp64` times the all-FP64 tile instead, and the `tile` leg reports the
-other element.  Every timed step (and the warm-up) launches mcxyz_kernel with
-1024-lane blocks; the other launches of this run (visit-count pilot, Rd
-parity run) use 512-lane blocks, so a kernel-trace profile of this command
-reports the timed kernel under its own name.
+The timed steps use the FP32 LDS fluence tile on the box the visit-count
+pilot finds, with the voxel visits outside it written as binned deposit
+records and summed per bucket in FP64 by a reduction kernel (tuning.mode 2;
+sums within a workgroup's tile in FP32, everything else in FP64; the
+reference adds every deposit into an FP32 global array).  `--tile fp32`
+times the FP32 tile with FP64 atomics outside it, `--tile fp64` the all-FP64
+tile; the `tile` leg reports the two modes not timed.  Every timed step (and
+the warm-up) launches mcxyz_kernel with 1024-lane blocks; the other launches
+of this run (visit-count pilots, Rd parity run) use 512-lane blocks, so a
+kernel-trace profile of this command reports the timed kernel under its own
+name.  The roofline divides by the transport kernel's own duration (HIP
+events around that kernel on its stream); `value` counts the whole run,
+reductions included.
 
     python bench.py                      # the measurement
     python bench.py --calibrate-copy 256 # a 256 MiB copy, to scale PMC byte counters
@@ -114,9 +119,20 @@ def cpu_baseline(w, target_s: float) -> dict:
             "_tallies": t, "_n": n}
 
 
+TILE_MODES = {"rec": 2, "fp32": 1, "fp64": 0}
+MODE_NAMES = {v: k for k, v in TILE_MODES.items()}
+# fluence-variant template argument of the timed kernel per tuning.mode
+MODE_FMODE = {0: 1, 1: 1, 2: 3}
+
+
+def kernel_name(mode: int, block: int = 1024) -> str:
+    elem = "float" if mode else "double"
+    return f"mcxyz_kernel<false, {block}, {elem}, {MODE_FMODE[mode]}>"
+
+
 def pilot_tuning(mc, hist, mode: int):
-    """Tuning with tile element `mode` (0 FP64, 1 FP32) and the box of that
-    element's LDS budget holding the most pilot visits."""
+    """Tuning with tile mode `mode` (0 FP64, 1 FP32, 2 FP32 + records) and the
+    box of that mode's LDS budget holding the most pilot visits."""
     from nmc_amd import params as P
     from nmc_amd.autotune import best_boxes
 
@@ -127,25 +143,25 @@ def pilot_tuning(mc, hist, mode: int):
 
 
 def tile_leg(mc, hist, n: int, mode: int, steps: int = 2) -> dict:
-    """The same workload with the other LDS tile element (`mode` 0 FP64, 1
-    FP32), box from the visit pilot."""
+    """The same workload with another tile mode (0 FP64 tile, 1 FP32 tile,
+    2 FP32 tile + records), box from the visit pilot."""
     from nmc_amd.autotune import box_coverage
 
     t = pilot_tuning(mc, hist, mode)
     mc.set_tuning(t)
     box = tuple(t.tile)
-    log(f"[bench] {'fp32' if mode else 'fp64'} tile {box}: {steps} x {n} photons")
+    log(f"[bench] {MODE_NAMES[mode]} tile {box}: {steps} x {n} photons")
     mc.reset_tallies()
     mc.set_num_runs_seed(1, 0)
-    ms = []
+    ms, kms = [], []
     for _ in range(steps):
         mc.send_compute_command(n)
         ms.append(mc.last_kernel_ms())
+        kms.append(mc.last_transport_ms())
     avg = sum(ms) / len(ms)
-    elem = "float" if mode else "double"
-    return {"kernel": f"mcxyz_kernel<false, 1024, {elem}, 1>", "tile": list(box),
-            "lds_tile_fraction": box_coverage(hist, box), "kernel_ms": avg,
-            "photons_per_s": n / (avg * 1e-3),
+    return {"kernel": kernel_name(mode), "tile": list(box),
+            "lds_tile_fraction": box_coverage(hist, box), "run_ms": avg,
+            "kernel_ms": sum(kms) / len(kms), "photons_per_s": n / (avg * 1e-3),
             "fluence_parity": ("integer tallies bit-exact; fluence within 2e-5 relative of the "
                                "oracle (tests/test_gpu_parity.py)") if mode else
                               "every tally within 1e-9 relative of the oracle"}
@@ -197,6 +213,38 @@ def sweep_leg(budget_scale: float, reps: int = 3) -> dict:
             "tallies_identical": bool(same), "batched_photons_per_s": total / (t_batch_ms * 1e-3)}
 
 
+def sweep_cpu_baseline(budget_scale: float, target_s: float) -> dict:
+    """The oracle over the 128-wavelength sweep's items on this host's cores,
+    each item's budget scaled so the whole sample takes about target_s."""
+    from oracle import oracle as O
+    from nmc_amd import workloads as W
+
+    cores = len(os.sched_getaffinity(0))
+    w, items, _ = W.spectral_sweep(budget_scale=budget_scale)
+
+    def run(scale):
+        photons, events, secs = 0, 0, 0.0
+        for it in items:
+            w.rp.n1, w.rp.n2 = it["n1"], it["n2"]
+            w.tp.muav[1], w.tp.musv[1], w.tp.gv[1] = it["mua"], it["mus"], it["g"]
+            w.ext.stream_id = it["stream_id"]
+            m = max(1, int(it["n_photons"] * scale))
+            t = O.run(w, m, threads=cores, fluence=False)
+            photons += m
+            events += t.totals["substeps_same"] + t.totals["substeps_cross"]
+            secs += t.seconds
+        return photons, events, secs
+
+    scale = 1e-3
+    p, e, sec = run(scale)
+    scale *= min(50.0, target_s / max(sec, 1e-3))
+    log(f"[bench] sweep cpu baseline: oracle on {cores} threads, item budgets x {scale:.2e}")
+    p, e, sec = run(scale)
+    return {"value": p / sec, "unit": "photons/s", "events_per_s": e / sec, "cores": cores,
+            "kind": "port", "sample": f"all 128 items at {scale:.2e} of their budgets: {p} photons, "
+                                      f"{e} events in {sec:.1f} s (oracle, OpenMP, FP64 tallies)"}
+
+
 def gradient_leg(n_photons: int) -> dict:
     """5-layer differentiable case: (forward with path records + backward)
     against the plain forward, kernel times on the library stream."""
@@ -223,7 +271,46 @@ def gradient_leg(n_photons: int) -> dict:
         nrec = mc.results()[3]
     return {"photons": n_photons, "forward_ms": fwd, "forward_with_records_ms": fwd_rec,
             "backward_ms": bwd, "records": nrec, "grad_over_forward": (fwd_rec + bwd) / fwd,
-            "forward_photons_per_s": n_photons / fwd * 1e3}
+            "forward_photons_per_s": n_photons / fwd * 1e3,
+            "grad_photons_per_s": n_photons / (fwd_rec + bwd) * 1e3}
+
+
+def gradient_cpu_baseline(target_s: float) -> dict:
+    """The oracle's forward with path records plus its pMC backward on the
+    5-layer stack, on this host's cores, over a sample of about target_s."""
+    from oracle import oracle as O
+    from nmc_amd import workloads as W
+
+    cores = len(os.sched_getaffinity(0))
+
+    def run(n):
+        cfg, stacks = W.multilayer_5(n_photons=n)
+        t0 = time.perf_counter()
+        o = O.run_layered(cfg, stacks, threads=cores)
+        O.layered_backward(o, np.ones((1, cfg.nr + 1)))
+        return time.perf_counter() - t0
+
+    n = 20_000
+    sec = run(n)
+    n = int(min(5e7, n * target_s / max(sec, 1e-3)))
+    log(f"[bench] gradient cpu baseline: oracle on {cores} threads, {n} photons")
+    sec = run(n)
+    return {"value": n / sec, "unit": "photons/s", "cores": cores, "kind": "port",
+            "sample": f"photons [0,{n}) of the 5-layer stack, forward with records and pMC "
+                      f"backward in {sec:.1f} s (oracle, OpenMP)"}
+
+
+def fit_leg(steps: int) -> dict:
+    """Batched inverse fit: 256 three-layer instances, `steps` Adam steps of
+    one forward (with path records) and one backward launch each, against the
+    same step done as 256 per-instance launches; per-layer median relative
+    errors at the start and the end, and the Cramer-Rao bound of each
+    parameter from the target's noise (what any fit of this data can reach)."""
+    from nmc_amd.inverse_fit import fit
+
+    log(f"[bench] inverse fit: 256 instances x {steps} Adam steps")
+    r = fit(256, steps, plain_compare=True)
+    return r
 
 
 def main() -> int:
@@ -234,11 +321,13 @@ def main() -> int:
     ap.add_argument("--photons", type=float, default=1e8, help="photons per step")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--tile", choices=("fp32", "fp64"), default="fp32",
-                    help="LDS fluence tile element of the timed steps")
-    ap.add_argument("--legs", default="tile,sweep,grad",
-                    help="extra measurements: tile (the other LDS tile element), sweep "
-                         "(batched vs plain), grad (pMC cost)")
+    ap.add_argument("--tile", choices=tuple(TILE_MODES), default="rec",
+                    help="deposit scheme of the timed steps: rec (FP32 LDS tile + binned "
+                         "records), fp32 (FP32 tile + FP64 atomics), fp64 (FP64 tile + atomics)")
+    ap.add_argument("--legs", default="tile,sweep,grad,fit",
+                    help="extra measurements: tile (the other deposit schemes), sweep "
+                         "(batched vs plain), grad (pMC cost), fit (batched inverse fit)")
+    ap.add_argument("--fit-steps", type=int, default=50)
     ap.add_argument("--sweep-scale", type=float, default=1.0, help="scale of the sweep budgets")
     ap.add_argument("--grad-photons", type=float, default=1e7)
     ap.add_argument("--autotune", action="store_true",
@@ -271,7 +360,7 @@ def main() -> int:
     ceiling = atomic_ceiling(w.rp.Nx * w.rp.Ny * w.rp.Nz)
     log("[bench] visit-count pilot (512-lane blocks)")
     hist = visit_histogram(w)
-    mode = 1 if args.tile == "fp32" else 0
+    mode = TILE_MODES[args.tile]
     mc = MetalMC(w.sim, 0)
     mc.prepare_data(w.rp, w.tp, w.v, w.ext)
     if tuning is None:
@@ -285,13 +374,14 @@ def main() -> int:
         mc.send_compute_command(n)
     mc.reset_tallies()
     mc.set_num_runs_seed(1, 0)
-    kern_ms = []
+    run_ms, kern_ms = [], []
     log(f"[bench] timed {args.steps} x {n} photons")
     mc.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         mc.send_compute_command_async(n)
-        kern_ms.append(mc.last_kernel_ms())   # waits for this step's end event
+        run_ms.append(mc.last_kernel_ms())      # waits for this step's end event
+        kern_ms.append(mc.last_transport_ms())  # the transport kernel alone
     mc.synchronize()
     t1 = time.perf_counter()
     wall = t1 - t0
@@ -300,6 +390,7 @@ def main() -> int:
     value = n * args.steps / wall
     ksec = sum(kern_ms) * 1e-3
     avg_ms = sum(kern_ms) / len(kern_ms)
+    avg_run_ms = sum(run_ms) / len(run_ms)
 
     ops = OPS_SAME * tot["substeps_same"] + OPS_CROSS * tot["substeps_cross"] \
         + OPS_INT * tot["interactions"]
@@ -307,16 +398,19 @@ def main() -> int:
     # voxel-visit flushes: one per voxel left plus one per photon ending in the grid
     visits = tot["substeps_cross"] - tot["surface_hits"] + tot["photons"] \
         - tot["escapes_bottom"] - tot["escapes_side"] - tot["escapes_top"]
-    global_atomics = visits * (1.0 - coverage) + tot["detections"]
+    out_of_tile = visits * (1.0 - coverage)
+    records = out_of_tile if eff.mode == 2 else 0.0
+    global_atomics = (out_of_tile - records) + tot["detections"]
     rd = tot["rd"] / tot["photons"]
     per_step = tot["photons"] / args.steps
 
     roof = {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_T,
             "unit": "Tlane-op/s", "frac": achieved / VALU_PEAK_T, "traffic": None,
             # template arguments: two-term phase function, block, tile element,
-            # fluence mode (1 = deposits), as in a kernel trace
-            "kernel": f"mcxyz_kernel<false, {eff.block}, {'float' if eff.mode else 'double'}, 1>",
-            "kernel_ms": avg_ms,
+            # fluence variant (1 deposits, 3 deposits with records), as in a
+            # kernel trace
+            "kernel": kernel_name(eff.mode, eff.block),
+            "kernel_ms": avg_ms, "run_ms": avg_run_ms,
             "ops_per_photon": ops / tot["photons"],
             "events_per_photon": {"same": tot["substeps_same"] / tot["photons"],
                                   "cross": tot["substeps_cross"] / tot["photons"],
@@ -324,11 +418,21 @@ def main() -> int:
             "atomic": {"bound": "fp64 atomic adds, scattered", "ceiling_per_s": ceiling["per_s"],
                        "visits_per_photon": visits / tot["photons"],
                        "lds_tile_fraction": coverage,
+                       "global_atomics_per_photon": global_atomics / tot["photons"],
                        "global_atomics_per_s": global_atomics / ksec,
-                       "frac": global_atomics / ksec / ceiling["per_s"],
-                       "photons_per_s_at_ceiling": ceiling["per_s"] * tot["photons"]
-                       / global_atomics},
+                       "frac": global_atomics / ksec / ceiling["per_s"]},
             "tuning": eff.as_dict()}
+    if eff.mode == 2:
+        # the out-of-tile visits leave as 8-byte records, written by the
+        # transport kernel and read once by the bucket reduction (pilot-sized
+        # slices: the atomic fallback takes none here)
+        roof["records"] = {"per_photon": records / tot["photons"],
+                           "bytes_per_photon": 8.0 * records / tot["photons"],
+                           "reduction_ms": avg_run_ms - avg_ms,
+                           "reduction_GB_per_s": 8.0 * records / args.steps
+                           / ((avg_run_ms - avg_ms) * 1e-3) / 1e9}
+    else:
+        roof["atomic"]["photons_per_s_at_ceiling"] = ceiling["per_s"] * tot["photons"] / global_atomics
     if TRAFFIC_BYTES_PER_PHOTON.get(eff.mode) is not None:
         roof["traffic"] = TRAFFIC_BYTES_PER_PHOTON[eff.mode] * per_step
         roof["hbm_GB_per_s"] = roof["traffic"] / (avg_ms * 1e-3) / 1e9
@@ -341,9 +445,14 @@ def main() -> int:
         "config": {"workload": "mcxyz skin-vessel 200^3 uint8 (0.005 cm voxels; air/epidermis/"
                    "dermis/blood cylinder), flat collimated beam r=0.06 cm, n=1, 1e8 photons/step",
                    "photons_per_step": n, "autotuned": args.autotune,
-                   "lds_tile": ("FP32 sums within a workgroup, FP64 across workgroups and "
-                                "outside the tile (the reference adds every deposit into an "
-                                "FP32 global array)" if eff.mode else "FP64"),
+                   "lds_tile": {2: "FP32 sums within a workgroup, FP64 across workgroups; "
+                                   "visits outside the tile as binned records (FP32 weight per "
+                                   "visit) summed per bucket in FP64 (the reference adds every "
+                                   "deposit into an FP32 global array)",
+                                1: "FP32 sums within a workgroup, FP64 across workgroups and "
+                                   "outside the tile (the reference adds every deposit into an "
+                                   "FP32 global array)",
+                                0: "FP64"}[eff.mode],
                    "tile_box": list(eff.tile)},
         "roofline": roof,
         "rd": rd,
@@ -369,12 +478,21 @@ def main() -> int:
                             "int_tallies_equal": all(g[k] == r[k] for k in P.TOTAL_INT_FIELDS)}
     legs = [s for s in args.legs.split(",") if s]
     if "tile" in legs:
-        out["fp32_tile" if eff.mode == 0 else "fp64_tile"] = tile_leg(mc, hist, n, 1 - eff.mode)
+        for m in sorted(TILE_MODES.values(), reverse=True):
+            if m != eff.mode:
+                out[f"{MODE_NAMES[m]}_tile"] = tile_leg(mc, hist, n, m)
     mc.close()
     if "sweep" in legs:
         out["batched_sweep"] = sweep_leg(args.sweep_scale)
+        if not args.no_cpu_baseline:
+            out["batched_sweep"]["cpu_baseline"] = sweep_cpu_baseline(args.sweep_scale,
+                                                                      args.cpu_seconds / 2)
     if "grad" in legs:
         out["gradient"] = gradient_leg(int(args.grad_photons))
+        if not args.no_cpu_baseline:
+            out["gradient"]["cpu_baseline"] = gradient_cpu_baseline(args.cpu_seconds / 2)
+    if "fit" in legs:
+        out["inverse_fit"] = fit_leg(args.fit_steps)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -242,6 +242,10 @@ typedef struct nmc_tuning {
 int nmc_set_tuning(nmc_handle* h, const nmc_tuning* t);
 /* Effective values after defaults and clamping (tile resolved to a box). */
 int nmc_get_tuning(nmc_handle* h, nmc_tuning* t);
+/* mcxyz: milliseconds of the last transport-kernel launch of the last eager
+ * run (HIP events on the handle's stream around that kernel alone, without
+ * the tile and record reductions that nmc_last_kernel_ms includes). */
+int nmc_last_transport_ms(nmc_handle* h, float* ms);
 /* Transport-kernel launches of the last run (no reference counterpart; more
  * than one when an FP32-tile mcxyz run is split by tuning.split). */
 int nmc_last_launch_count(nmc_handle* h, uint32_t* n);

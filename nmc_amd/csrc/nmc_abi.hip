@@ -151,6 +151,8 @@ struct nmc_handle {
   size_t lds_max = 0;
   hipStream_t stream = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipEvent_t k0 = nullptr, k1 = nullptr;  // around the last transport kernel (eager runs)
+  bool k_timed = false;
   bool prepared = false, timed = false, batch = false;
   RunParams rp{};
   TissueParams tp{};
@@ -421,7 +423,7 @@ XyzArgs xyz_args(const RunParams& rp, const nmc_ext_config& e, int sim, uint32_t
 // the FP32 tile's accuracy was measured at, whatever the grid.  The launches
 // take consecutive photon ids, so the photons are the same.
 int enqueue_xyz(XyzArgs a, bool two_term, const XyzLaunch& L, uint64_t n, hipStream_t s,
-                uint32_t* launches, const XyzRecords* R) {
+                uint32_t* launches, const XyzRecords* R, hipEvent_t k0, hipEvent_t k1) {
   const bool recs = R && R->n_buckets > 0 && a.F && !a.count_visits;
   a.n_buckets = recs ? R->n_buckets : 0;
   a.rec_base = recs ? R->base : nullptr;
@@ -453,7 +455,9 @@ int enqueue_xyz(XyzArgs a, bool two_term, const XyzLaunch& L, uint64_t n, hipStr
     a.n = part;
     a.chunk = pick_chunk(part, L.grid * (L.t.block / 64), L.t.chunk);
     HIP_TRY(launch_state_begin(a.state, s));
+    if (k0) HIP_TRY(hipEventRecord(k0, s));
     HIP_TRY(launch_mcxyz(a, two_term, L.t.mode >= 1, L.t.block, L.grid, L.lds, s));
+    if (k1) HIP_TRY(hipEventRecord(k1, s));
     HIP_TRY(launch_tile_reduce(a.tile_part, L.grid, a, s));
     if (recs) HIP_TRY(launch_rec_reduce(rr, R->n_parts, s));
     HIP_TRY(launch_state_advance(a.state, part, s));
@@ -569,7 +573,8 @@ int record_rates(nmc_handle* h, const XyzLaunch& L, std::vector<double>* rate) {
   P.lds = 0;
   P.n_buckets = 0;
   uint32_t launches = 0;
-  int rc = enqueue_xyz(a, h->sim == NMC_SIM_MCXYZ_TT, P, kRecPilot, h->stream, &launches, nullptr);
+  int rc = enqueue_xyz(a, h->sim == NMC_SIM_MCXYZ_TT, P, kRecPilot, h->stream, &launches, nullptr,
+                       nullptr, nullptr);
   std::vector<double> host;
   if (rc == NMC_OK) {
     host.resize(nv);
@@ -794,7 +799,10 @@ int enqueue_run(nmc_handle* h, uint64_t n) {
     XyzLaunch L{h->tune, h->xyz_lds, h->xyz_grid, 0, 0};
     uint32_t launches = 0;
     L.n_buckets = h->recs.n_buckets;
-    NMC_TRY(enqueue_xyz(a, h->sim == NMC_SIM_MCXYZ_TT, L, n, s, &launches, &h->recs));
+    // transport-kernel events only outside graph capture
+    hipEvent_t k0 = h->graph_mode ? nullptr : h->k0, k1 = h->graph_mode ? nullptr : h->k1;
+    NMC_TRY(enqueue_xyz(a, h->sim == NMC_SIM_MCXYZ_TT, L, n, s, &launches, &h->recs, k0, k1));
+    h->k_timed = k0 != nullptr;
     h->launches += launches;
   }
   return ok();
@@ -1017,6 +1025,8 @@ int nmc_create(int sim_type, int device, nmc_handle** out) {
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreate(&h->e0);
   if (e == hipSuccess) e = hipEventCreate(&h->e1);
+  if (e == hipSuccess) e = hipEventCreate(&h->k0);
+  if (e == hipSuccess) e = hipEventCreate(&h->k1);
   if (e == hipSuccess) e = hipMalloc(&h->dTot, sizeof(nmc_totals));
   if (e == hipSuccess) e = hipMalloc(&h->dState, sizeof(RunState));
   if (e != hipSuccess) {
@@ -1168,6 +1178,18 @@ int nmc_set_tuning(nmc_handle* h, const nmc_tuning* t) {
   int st = h->sim == NMC_SIM_MCSUB ? resolve_sub_tuning(h) : resolve_xyz_tuning(h);
   if (st != NMC_OK) h->tune_req = prev;
   return st;
+}
+
+int nmc_last_transport_ms(nmc_handle* h, float* ms) {
+  if (!h || !ms) return fail(NMC_ERR_ARG, "null argument");
+  if (h->sim != NMC_SIM_MCXYZ && h->sim != NMC_SIM_MCXYZ_TT)
+    return fail(NMC_ERR_UNSUPPORTED, "transport-kernel timing is kept for mcxyz handles");
+  if (!h->timed || !h->k_timed)
+    return fail(NMC_ERR_STATE, "no eager mcxyz run has been timed (graph replays keep no kernel events)");
+  HIP_TRY(hipSetDevice(h->device));
+  HIP_TRY(hipEventSynchronize(h->k1));
+  HIP_TRY(hipEventElapsedTime(ms, h->k0, h->k1));
+  return ok();
 }
 
 int nmc_last_launch_count(nmc_handle* h, uint32_t* n) {
@@ -1641,6 +1663,8 @@ void nmc_destroy(nmc_handle* h) {
   free_ptr(h->dState);
   if (h->e0) (void)hipEventDestroy(h->e0);
   if (h->e1) (void)hipEventDestroy(h->e1);
+  if (h->k0) (void)hipEventDestroy(h->k0);
+  if (h->k1) (void)hipEventDestroy(h->k1);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
 }
@@ -1797,6 +1821,7 @@ extern "C" int nmc_mcxyz_run_noalloc(int sim_type, const RunParams* rp, const Ti
   a.tot = d_totals;
   a.state = dstate;
   uint32_t launches = 0;
-  NMC_TRY(enqueue_xyz(a, sim_type == NMC_SIM_MCXYZ_TT, L, n_photons, s, &launches, nullptr));
+  NMC_TRY(enqueue_xyz(a, sim_type == NMC_SIM_MCXYZ_TT, L, n_photons, s, &launches, nullptr,
+                      nullptr, nullptr));
   return ok();
 }

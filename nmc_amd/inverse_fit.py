@@ -56,13 +56,18 @@ def _fit(runner, cfg, hidden, nl, true_mua, true_mus, steps, lr, seed, init_spre
     from .autograd import layered_rd
 
     target = make_target(runner, seed)
+    # identifiability at the hidden parameters, from the target run's path
+    # records, with the target's own noise: 1 % of each bin's noise-free value
+    # (bins no light reaches carry no information)
+    rd0 = runner.rd_per_photon()
+    crb = cramer_rao(runner, np.where(rd0 > 0, 0.01 * rd0, np.inf), nl)
     rng = np.random.default_rng(seed + 2000)
     start_mua = true_mua * np.exp(rng.uniform(-init_spread, init_spread, true_mua.shape))
     start_mus = true_mus * np.exp(rng.uniform(-init_spread, init_spread, true_mus.shape))
     la = torch.tensor(np.log(start_mua), requires_grad=True)
     ls = torch.tensor(np.log(start_mus), requires_grad=True)
     tgt = torch.from_numpy(target)
-    sigma = 0.01 * tgt.abs() + 1e-3 * tgt.abs().amax(dim=1, keepdim=True) + 1e-12
+    sigma = torch.from_numpy(fit_sigma(target))
     opt = torch.optim.Adam([la, ls], lr=lr)
     history = []
     t0 = time.perf_counter()
@@ -92,12 +97,54 @@ def _fit(runner, cfg, hidden, nl, true_mua, true_mus, steps, lr, seed, init_spre
         "median_rel_err_mus_per_layer": np.median(err_s, axis=0).tolist(),
         "median_start_err_mua_per_layer": np.median(start_err_a, axis=0).tolist(),
         "median_start_err_mus_per_layer": np.median(start_err_s, axis=0).tolist(),
+        "median_crb_rel_mua_per_layer": crb["median_crb_rel_mua_per_layer"],
+        "median_crb_rel_mus_per_layer": crb["median_crb_rel_mus_per_layer"],
+        # share of instances whose error is within 3 times their own bound
+        "share_within_3crb_mua_per_layer": np.mean(err_a <= 3 * crb["crb_rel"][:, :nl], axis=0).tolist(),
+        "share_within_3crb_mus_per_layer": np.mean(err_s <= 3 * crb["crb_rel"][:, nl:], axis=0).tolist(),
         "wall_s": wall, "wall_s_per_step": wall / steps, "kernel_ms_per_step": kernel_ms / steps,
     }
     if plain_compare:
         out.update(plain_step_time(cfg, hidden, device))
         out["batched_speedup_per_step"] = out["plain_wall_s_per_step"] / out["wall_s_per_step"]
     return out
+
+
+def fit_sigma(target: np.ndarray, noise: float = 0.01) -> np.ndarray:
+    """Per-bin standard deviation the fit's loss assumes: the target's relative
+    noise plus a floor of 1e-3 of each profile's largest bin."""
+    return noise * np.abs(target) + 1e-3 * np.abs(target).max(axis=1, keepdims=True) + 1e-12
+
+
+def cramer_rao(runner: LayeredMC, sigma: np.ndarray, nl: int) -> dict:
+    """Identifiability of each layer's mua and mus from Rd(r) with noise sigma
+    [n_stacks, nr+1] at the runner's current properties (its last run's path
+    records): the Fisher information F = J^T diag(sigma^-2) J of the
+    log-parameters, with the Jacobian J = dRd(r) / dlog(theta) assembled from
+    one perturbation-MC backward call per Rd bin.  sqrt(diag(F^-1)) bounds
+    from below the standard deviation of any unbiased estimate of
+    log(theta): the relative error no fit of this data can beat."""
+    ns, nb = sigma.shape
+    mua = np.array([[s.layer[l].mua for l in range(nl)] for s in runner.stacks()])
+    mus = np.array([[s.layer[l].mus for l in range(nl)] for s in runner.stacks()])
+    J = np.zeros((ns, nb, 2 * nl))
+    for k in range(nb):
+        g = np.zeros((ns, nb))
+        g[:, k] = 1.0
+        gmua, gmus = runner.backward(g)
+        J[:, k, :nl] = gmua[:, :nl] * mua
+        J[:, k, nl:] = gmus[:, :nl] * mus
+    Jw = J / sigma[:, :, None]
+    F = np.einsum("skp,skq->spq", Jw, Jw)
+    crb = np.full((ns, 2 * nl), np.inf)
+    for j in range(ns):
+        try:
+            crb[j] = np.sqrt(np.clip(np.diag(np.linalg.inv(F[j])), 0.0, None))
+        except np.linalg.LinAlgError:
+            pass
+    return {"median_crb_rel_mua_per_layer": np.median(crb[:, :nl], axis=0).tolist(),
+            "median_crb_rel_mus_per_layer": np.median(crb[:, nl:], axis=0).tolist(),
+            "crb_rel": crb}
 
 
 def plain_step_time(cfg: P.LayeredConfig, stacks, device: int = 0) -> dict:

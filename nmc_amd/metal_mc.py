@@ -61,6 +61,12 @@ class MetalMC:
         check(lib().nmc_last_kernel_ms(self._h, C.byref(ms)))
         return ms.value
 
+    def last_transport_ms(self) -> float:
+        """The last transport-kernel launch alone (mcxyz, eager runs)."""
+        ms = C.c_float()
+        check(lib().nmc_last_transport_ms(self._h, C.byref(ms)))
+        return ms.value
+
     def last_launch_count(self) -> int:
         """Transport-kernel launches of the last run."""
         n = C.c_uint32()
