@@ -321,7 +321,10 @@ typedef struct nmc_layered_config {
     uint32_t max_events;
     int32_t records;              /* 1: keep per-detected-photon path records */
     uint64_t record_capacity;     /* 0: sum of the budgets                    */
-    int32_t reserved[8];
+    int32_t absorbed_grad;        /* 1: the forward run also accumulates the
+                                     absorbed-weight sensitivities
+                                     (nmc_layered_absorbed_grad)              */
+    int32_t reserved[7];
 } nmc_layered_config;
 
 /* Allocates tallies (and the record buffer) for n_stacks stacks and uploads
@@ -383,6 +386,17 @@ int nmc_mcxyz_run_noalloc(int sim_type, const RunParams* rp, const TissueParams*
                           const nmc_ext_config* ext, const nmc_tuning* tuning, const uint8_t* d_v,
                           uint64_t first_photon, uint64_t n_photons, double* d_fluence,
                           double* d_ryx, uint32_t* d_det2d, nmc_totals* d_totals, void* stream);
+
+/* Per-layer absorbed-weight sensitivities of the last nmc_layered_run (the
+ * handle prepared with absorbed_grad = 1): d_mua[j][l][t] and d_mus[j][l][t]
+ * (each n_stacks x NMC_MAX_LAYERS x NMC_MAX_LAYERS) are the derivatives of
+ * the raw weight absorbed in layer l of stack j (nmc_layered_results'
+ * `absorbed`) with respect to mua and mus of layer t: the perturbation-MC
+ * estimate over every deposit, each weighted by the photon's per-layer path
+ * lengths and scattering counts before it (no reference counterpart; the
+ * reference's absorbed tallies are F3D / A(r,z), kernels.metal:530,550,
+ * 224-234). */
+int nmc_layered_absorbed_grad(nmc_handle* h, double* d_mua, double* d_mus);
 
 /* Message of the last failing call on this host thread ("" if none). */
 const char* nmc_last_error(void);

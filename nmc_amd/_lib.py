@@ -65,6 +65,7 @@ def lib():
         "nmc_layered_backward": (C.c_int, [H, C.c_void_p, C.c_void_p, C.c_void_p]),
         "nmc_layered_records": (C.c_int, [H, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64),
                                           C.POINTER(C.c_int)]),
+        "nmc_layered_absorbed_grad": (C.c_int, [H, C.c_void_p, C.c_void_p]),
         "nmc_mcxyz_run_noalloc": (C.c_int, [C.c_int, rp, tp, ext, C.POINTER(P.Tuning), C.c_void_p,
                                             C.c_uint64, C.c_uint64, C.c_void_p, C.c_void_p,
                                             C.c_void_p, C.c_void_p, C.c_void_p]),
@@ -86,7 +87,7 @@ EXPORTED = ("nmc_ext_config_init", "nmc_validate", "nmc_create", "nmc_prepare", 
             "nmc_run_batch", "nmc_run_batch_async", "nmc_get_batch_results",
             "nmc_set_graph_mode", "nmc_layered_prepare", "nmc_layered_update", "nmc_layered_run",
             "nmc_layered_results", "nmc_layered_backward", "nmc_layered_records",
-            "nmc_mcxyz_run_noalloc")
+            "nmc_layered_absorbed_grad", "nmc_mcxyz_run_noalloc")
 
 
 def check(status: int) -> None:

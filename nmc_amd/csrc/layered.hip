@@ -11,6 +11,20 @@
 // which the backward kernel forms dLoss/dmua_l = -sum g w L_l and
 // dLoss/dmus_l = sum g w (K_l/mus_l - L_l).
 //
+// Absorbed-weight sensitivities (kernel variant SENS): the expected weight
+// absorbed in layer l is the integral over the path of mua_l W, and the
+// density of the path up to a point has the score sum_t (K_t/mus_t - L_t)
+// with respect to mus_t (scatterings K_t and length L_t before that point;
+// absorption is carried by the weight).  So for a segment of length s in
+// layer l, entered with weight W and depositing D = W (1 - e^{-x}), x = mua_l s,
+// with L_t, K_t the photon's totals before the segment:
+//   dD/dmua_t = -L_t D + [t = l] W s e^{-x}
+//   dD/dmus_t = (K_t/mus_t - L_t) D - [t = l] W s h(x),
+//   h(x) = (1 - e^{-x} (1 + x)) / x  (the score's growth along the segment).
+// Within one visit to layer l the totals of the other layers stay fixed, so
+// the photon keeps only the visit's sums of D L_l, D K_l, W s e^{-x} and
+// W s h(x), and adds 2 nl terms per visit to its stack's LDS sums.
+//
 // Launch model as mcxyz.hip: persistent wave64 workgroups, one photon per
 // lane, batched lane refills from a device queue spanning every stack's
 // budget (queue index -> stack by binary search over budget prefix sums).
@@ -45,6 +59,10 @@ struct LPhoton {
   uint32_t ev, events, hits, b1, b2, b3;
   uint32_t n_same, n_cross, n_int;
   double acur;
+  // SENS: the layer's L and K at the start of the visit, and the visit's
+  // sums of D L_l, D K_l, W s e^{-x} and W s h(x)
+  float Lbase, sp, sq, sza, szs;
+  uint32_t Kbase;
   bool alive;
   bool rec_pending;  // left through the top; record written at the next wave-wide point
   int rec_ir;
@@ -53,6 +71,7 @@ struct LPhoton {
   uint32_t K[NMC_MAX_LAYERS];
 };
 
+template <bool SENS>
 __device__ __forceinline__ void load_layer(LPhoton& p, const LayRow* rows) {
   const LayRow& r = rows[p.row0 + p.l];
   p.zt = r.zt;
@@ -60,11 +79,58 @@ __device__ __forceinline__ void load_layer(LPhoton& p, const LayRow* rows) {
   p.mua = r.mua;
   p.mus = r.mus;
   p.inv_mus = r.inv_mus;
+  if (SENS) {
+#pragma unroll
+    for (int i = 0; i < NMC_MAX_LAYERS; ++i) {
+      if (i == p.l) {
+        p.Lbase = p.L[i];
+        p.Kbase = p.K[i];
+      }
+    }
+  }
+}
+
+// h(x) = (1 - e^{-x} (1 + x)) / x for x = mua s >= 0, with em = expm1(-x):
+// a series below x = 0.05 (where the direct form cancels), else direct.
+__device__ __forceinline__ float score_growth(float x, float em) {
+  if (x < 0.05f) return x * (0.5f - x * (0.33333334f - x * (0.125f - x * 0.033333335f)));
+  return (-em - x * (1.0f + em)) / x;
+}
+
+// The sensitivity terms of one segment (SENS): D = dep deposited over length
+// s from weight w_pre, x = mua s, em = expm1(-x), before Lcur and Kcur grow.
+__device__ __forceinline__ void sens_segment(LPhoton& p, float dep, float w_pre, float s, float x,
+                                             float em) {
+  p.sp = p.sp + dep * (p.Lbase + p.Lcur);
+  p.sq = p.sq + dep * (float)(p.Kbase + p.Kcur);
+  const float ws = w_pre * s;
+  p.sza = p.sza + ws * (1.0f + em);
+  p.szs = p.szs + ws * score_growth(x, em);
 }
 
 // Folds the current layer's path length, scatter count and absorption into
-// the per-layer records / LDS absorption tally.
-__device__ __forceinline__ void fold_layer(LPhoton& p, double* absorbed) {
+// the per-layer records / LDS absorption tally; SENS: adds the visit's terms
+// to the stack's sensitivity sums sens[item][0 mua | 1 mus][l][t].
+template <bool SENS>
+__device__ __forceinline__ void fold_layer(LPhoton& p, double* absorbed, double* sens,
+                                           const LayRow* rows) {
+  if (SENS && (p.acur != 0.0 || p.sza != 0.0f)) {
+    double* S = sens + (int64_t)p.item * kSensPerStack + p.l * NMC_MAX_LAYERS;
+    const double D = p.acur;
+#pragma unroll
+    for (int t = 0; t < NMC_MAX_LAYERS; ++t) {
+      if (t < p.nl && t != p.l && D != 0.0) {
+        const double Lt = (double)p.L[t];
+        unsafeAtomicAdd(&S[t], -Lt * D);
+        unsafeAtomicAdd(&S[NMC_MAX_LAYERS * NMC_MAX_LAYERS + t],
+                        ((double)p.K[t] * (double)rows[p.row0 + t].inv_mus - Lt) * D);
+      }
+    }
+    unsafeAtomicAdd(&S[p.l], -(double)p.sp + (double)p.sza);
+    unsafeAtomicAdd(&S[NMC_MAX_LAYERS * NMC_MAX_LAYERS + p.l],
+                    (double)p.sq * (double)p.inv_mus - (double)p.sp - (double)p.szs);
+    p.sp = p.sq = p.sza = p.szs = 0.0f;
+  }
 #pragma unroll
   for (int i = 0; i < NMC_MAX_LAYERS; ++i) {
     if (i == p.l) {
@@ -127,13 +193,14 @@ __device__ __forceinline__ void flush_records(LPhoton& p, const LayArgs& a, int 
 }
 
 // RD_LDS: Rd(r) in an LDS histogram (else the block's global replica);
-// RECORDS: photons leaving through the top write path records.  Both are
-// kernel variants, so the loop holds no mask for them.
-template <bool RD_LDS, bool RECORDS>
+// RECORDS: photons leaving through the top write path records; SENS: the
+// absorbed-weight sensitivities.  Each is a kernel variant, so the loop holds
+// no mask for them.
+template <bool RD_LDS, bool RECORDS, bool SENS>
 __global__ __launch_bounds__(kLayBlock) void layered_kernel(LayArgs a) {
   extern __shared__ double smem[];
   // [items] absorbed[MAXL] | [items] totals | [items][nr+1] Rd (RD_LDS only) |
-  // [items] headers | [items+1] prefix | [rows]
+  // [items] headers | [items+1] prefix | [rows] | [items] sensitivities (SENS)
   const int ni = a.n_items;
   const int rd_len = RD_LDS ? ni * (a.nr + 1) : 0;
   double* absorbed = smem;
@@ -142,6 +209,8 @@ __global__ __launch_bounds__(kLayBlock) void layered_kernel(LayArgs a) {
   LayStackDev* hdr = (LayStackDev*)(rd_tile + rd_len);
   uint64_t* prefix = (uint64_t*)(hdr + ni);
   LayRow* rows = (LayRow*)(prefix + ni + 1);
+  // after the rows, rounded up to 8 bytes (layered_lds_bytes counts the pad)
+  double* sens = SENS ? (double*)(((size_t)(rows + a.n_rows) + 7u) & ~(size_t)7u) : nullptr;
   gptr<double> rd_rep =
       RD_LDS ? nullptr
              : gptr_in_vgpr(a.rd + (int64_t)(blockIdx.x % (unsigned)a.rd_reps) * ni * (a.nr + 1));
@@ -153,6 +222,8 @@ __global__ __launch_bounds__(kLayBlock) void layered_kernel(LayArgs a) {
   for (int i = threadIdx.x; i < ni; i += kLayBlock) hdr[i] = a.stacks[i];
   for (int i = threadIdx.x; i <= ni; i += kLayBlock) prefix[i] = a.prefix[i];
   for (int i = threadIdx.x; i < a.n_rows; i += kLayBlock) rows[i] = a.rows[i];
+  if (SENS)
+    for (int i = threadIdx.x; i < ni * kSensPerStack; i += kLayBlock) sens[i] = 0.0;
   __syncthreads();
   const uint64_t first = a.state->first;
   const uint64_t total = prefix[ni];
@@ -164,6 +235,8 @@ __global__ __launch_bounds__(kLayBlock) void layered_kernel(LayArgs a) {
   p.item = 0;
   p.l = 0;
   p.acur = 0.0;
+  p.Lbase = p.sp = p.sq = p.sza = p.szs = 0.0f;
+  p.Kbase = 0u;
   p.rec_pending = false;
   p.rec_ir = 0;
   bool finished = false;
@@ -213,16 +286,17 @@ __global__ __launch_bounds__(kLayBlock) void layered_kernel(LayArgs a) {
           p.uz = 1.0f;
           p.ruz = rcp_rn(p.uz);
           p.l = 0;
-          load_layer(p, rows);
-          p.hits = 0;
-          p.n_same = p.n_cross = p.n_int = 0;
-          p.ev = 1;
-          p.events = 0;
 #pragma unroll
           for (int i = 0; i < NMC_MAX_LAYERS; ++i) {
             p.L[i] = 0.0f;
             p.K[i] = 0u;
           }
+          load_layer<SENS>(p, rows);
+          p.hits = 0;
+          p.n_same = p.n_cross = p.n_int = 0;
+          p.ev = 1;
+          p.events = 0;
+          p.sp = p.sq = p.sza = p.szs = 0.0f;
           p.Lcur = 0.0f;
           p.Kcur = 0u;
           p.acur = 0.0;
@@ -252,8 +326,10 @@ __global__ __launch_bounds__(kLayBlock) void layered_kernel(LayArgs a) {
     p.x = fmaf(s, p.ux, p.x);
     p.y = fmaf(s, p.uy, p.y);
     p.z = fmaf(s, p.uz, p.z);
-    float em = expm1_f(-(p.mua * s));
+    const float x = p.mua * s;
+    float em = expm1_f(-x);
     float dep = -(p.W * em);
+    if (SENS) sens_segment(p, dep, p.W, s, x, em);
     p.W = p.W - dep;
     p.acur += (double)dep;
     p.Lcur = p.Lcur + s;
@@ -281,7 +357,7 @@ __global__ __launch_bounds__(kLayBlock) void layered_kernel(LayArgs a) {
         p.ruz = -p.ruz;
       } else if (up && p.l == 0) {
         // leaves through the top: Rd(r) and the path record
-        fold_layer(p, absorbed);
+        fold_layer<SENS>(p, absorbed, sens, rows);
         float re = sqrtf(p.x * p.x + p.y * p.y);
         int ir = rbin(re / a.dr, a.nr);
         // the bins near r = 0 take a large share of all exits: an LDS
@@ -299,12 +375,12 @@ __global__ __launch_bounds__(kLayBlock) void layered_kernel(LayArgs a) {
         }
         alive = false;
       } else if (!up && p.l == p.nl - 1) {
-        fold_layer(p, absorbed);
+        fold_layer<SENS>(p, absorbed, sens, rows);
         addf(T, TF_TT, (double)p.W);
         addi(T, TI_ESC_BOTTOM, 1);
         alive = false;
       } else {
-        fold_layer(p, absorbed);
+        fold_layer<SENS>(p, absorbed, sens, rows);
         if (ni_ != nt) {
           p.ux = p.ux * ni_ / nt;
           p.uy = p.uy * ni_ / nt;
@@ -312,7 +388,7 @@ __global__ __launch_bounds__(kLayBlock) void layered_kernel(LayArgs a) {
           p.ruz = rcp_rn(p.uz);
         }
         p.l += up ? -1 : 1;
-        load_layer(p, rows);
+        load_layer<SENS>(p, rows);
       }
     }
     if (alive && !(p.W > 0.0f)) alive = false;  // all weight absorbed
@@ -350,7 +426,7 @@ __global__ __launch_bounds__(kLayBlock) void layered_kernel(LayArgs a) {
       }
     }
     if (!alive) {
-      if (p.acur != 0.0) fold_layer(p, absorbed);
+      if (p.acur != 0.0 || (SENS && p.sza != 0.0f)) fold_layer<SENS>(p, absorbed, sens, rows);
       addi(T, TI_SUB_SAME, p.n_same);
       addi(T, TI_SUB_CROSS, p.n_cross);
       addi(T, TI_INTERACTIONS, p.n_int);
@@ -384,6 +460,10 @@ __global__ __launch_bounds__(kLayBlock) void layered_kernel(LayArgs a) {
   if (RD_LDS) {
     double* rp = a.rd + (int64_t)blockIdx.x * rd_len;
     for (int i = threadIdx.x; i < rd_len; i += kLayBlock) rp[i] = rd_tile[i];
+  }
+  if (SENS) {
+    double* sp = a.sens_part + (int64_t)blockIdx.x * ni * kSensPerStack;
+    for (int i = threadIdx.x; i < ni * kSensPerStack; i += kLayBlock) sp[i] = sens[i];
   }
 }
 
@@ -448,25 +528,30 @@ __global__ __launch_bounds__(256) void layered_backward_kernel(LayBackArgs b) {
 
 }  // namespace
 
-size_t layered_lds_bytes(int n_items, int n_rows, int rd_len) {
+size_t layered_lds_bytes(int n_items, int n_rows, int rd_len, bool sens) {
   return ((size_t)n_items * (NMC_MAX_LAYERS + TF_COUNT + TI_COUNT) + (size_t)rd_len) * sizeof(double) +
          (size_t)n_items * sizeof(LayStackDev) + (size_t)(n_items + 1) * sizeof(uint64_t) +
-         (size_t)n_rows * sizeof(LayRow);
+         (size_t)n_rows * sizeof(LayRow) +
+         (sens ? 8 + (size_t)n_items * kSensPerStack * sizeof(double) : 0);
 }
 
 namespace {
 const void* layered_variant(int v) {
   switch (v) {
-    case 0: return (const void*)layered_kernel<false, false>;
-    case 1: return (const void*)layered_kernel<false, true>;
-    case 2: return (const void*)layered_kernel<true, false>;
-    default: return (const void*)layered_kernel<true, true>;
+    case 0: return (const void*)layered_kernel<false, false, false>;
+    case 1: return (const void*)layered_kernel<false, true, false>;
+    case 2: return (const void*)layered_kernel<true, false, false>;
+    case 3: return (const void*)layered_kernel<true, true, false>;
+    case 4: return (const void*)layered_kernel<false, false, true>;
+    case 5: return (const void*)layered_kernel<false, true, true>;
+    case 6: return (const void*)layered_kernel<true, false, true>;
+    default: return (const void*)layered_kernel<true, true, true>;
   }
 }
 }  // namespace
 
 hipError_t layered_configure(size_t lds_bytes) {
-  for (int v = 0; v < 4; ++v) {
+  for (int v = 0; v < 8; ++v) {
     hipError_t e = hipFuncSetAttribute(layered_variant(v), hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)lds_bytes);
     if (e != hipSuccess) return e;
@@ -477,7 +562,7 @@ hipError_t layered_configure(size_t lds_bytes) {
 // Workgroups per CU that every variant reaches.
 int layered_blocks_per_cu(size_t lds_bytes) {
   int best = 0;
-  for (int v = 0; v < 4; ++v) {
+  for (int v = 0; v < 8; ++v) {
     int nb = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, layered_variant(v), kLayBlock, lds_bytes) !=
         hipSuccess)
@@ -488,7 +573,7 @@ int layered_blocks_per_cu(size_t lds_bytes) {
 }
 
 hipError_t launch_layered(const LayArgs& a, int grid, size_t lds_bytes, hipStream_t s) {
-  const int v = (a.rd_lds ? 2 : 0) + (a.records ? 1 : 0);
+  const int v = (a.sens ? 4 : 0) + (a.rd_lds ? 2 : 0) + (a.records ? 1 : 0);
   LayArgs arg = a;
   void* args[] = {&arg};
   return hipLaunchKernel(layered_variant(v), dim3(grid), dim3(kLayBlock), args, lds_bytes, s);

@@ -200,6 +200,9 @@ struct nmc_handle {
   double* dLOut = nullptr;     // [n][MAXL] absorbed | [n] totals
   double* dLPart = nullptr;
   double* dLRdPart = nullptr;  // Rd partials [lay_rd_count][n][nr+1]
+  double* dLSens = nullptr;    // absorbed-weight sensitivities [n][2][MAXL][MAXL]
+  double* dLSensPart = nullptr;  // their block partials [grid][n][2][MAXL][MAXL]
+  bool sens_valid = false;
   int lay_rd_lds = 0, lay_rd_count = 0;
   unsigned long long* dRecCount = nullptr;
   uint32_t* dRec = nullptr;
@@ -267,6 +270,8 @@ void free_buffers(nmc_handle* h) {
   free_ptr(h->dLOut);
   free_ptr(h->dLPart);
   free_ptr(h->dLRdPart);
+  free_ptr(h->dLSens);
+  free_ptr(h->dLSensPart);
   free_ptr(h->dRecCount);
   free_ptr(h->dRec);
   free_ptr(h->dGrad);
@@ -1453,7 +1458,9 @@ int nmc_layered_prepare(nmc_handle* h, const nmc_layered_config* cfg, const nmc_
   HIP_TRY(hipMalloc(&h->dRecCount, 2 * sizeof(unsigned long long)));
   h->rec_cap = 0;
   h->rec_valid = false;
-  h->lay_lds = layered_lds_bytes(n, h->lay_rows, 0);
+  const bool sens = cfg->absorbed_grad != 0;
+  h->sens_valid = false;
+  h->lay_lds = layered_lds_bytes(n, h->lay_rows, 0, sens);
   if (h->lay_lds + 1024 > h->lds_max)
     return fail(NMC_ERR_ARG, "%d stacks / %d layers need %zu B of LDS (%zu available)", n, h->lay_rows,
                 h->lay_lds, h->lds_max);
@@ -1462,7 +1469,7 @@ int nmc_layered_prepare(nmc_handle* h, const nmc_layered_config* cfg, const nmc_
   if (h->tune_req.mode < 0 || h->tune_req.mode > 2)
     return fail(NMC_ERR_ARG, "tuning.mode (Rd placement) not in 0..2");
   const int rd_len = n * (cfg->nr + 1);
-  const size_t lds_rd = layered_lds_bytes(n, h->lay_rows, rd_len);
+  const size_t lds_rd = layered_lds_bytes(n, h->lay_rows, rd_len, sens);
   const bool rd_fits = lds_rd + 1024 <= h->lds_max;
   HIP_TRY(layered_configure(rd_fits ? lds_rd : h->lay_lds));
   int occ = layered_blocks_per_cu(h->lay_lds);
@@ -1480,6 +1487,10 @@ int nmc_layered_prepare(nmc_handle* h, const nmc_layered_config* cfg, const nmc_
   HIP_TRY(hipMalloc(&h->dLPart, (size_t)h->lay_grid * n * (NMC_MAX_LAYERS + TF_COUNT + TI_COUNT) * sizeof(double)));
   h->lay_rd_count = h->lay_rd_lds ? h->lay_grid : replica_count(rd_len, h->lay_grid);
   HIP_TRY(hipMalloc(&h->dLRdPart, (size_t)h->lay_rd_count * rd_len * sizeof(double)));
+  if (sens) {
+    HIP_TRY(hipMalloc(&h->dLSens, (size_t)n * kSensPerStack * sizeof(double)));
+    HIP_TRY(hipMalloc(&h->dLSensPart, (size_t)h->lay_grid * n * kSensPerStack * sizeof(double)));
+  }
   HIP_TRY(hipMemsetAsync(h->dLRdPart, 0, (size_t)h->lay_rd_count * rd_len * sizeof(double), h->stream));
   if (cfg->records) {
     // every photon leaves at most one record; each wave may leave the unused
@@ -1511,6 +1522,7 @@ int nmc_layered_update(nmc_handle* h, const nmc_stack* stacks, int n_stacks) {
   HIP_TRY(hipStreamSynchronize(h->stream));
   h->stacks.assign(stacks, stacks + n_stacks);
   h->rec_valid = false;
+  h->sens_valid = false;
   return upload_stacks(h);
 }
 
@@ -1525,6 +1537,8 @@ int nmc_layered_run(nmc_handle* h) {
   HIP_TRY(hipMemsetAsync(h->dLRd, 0, (size_t)rd_len * sizeof(double), s));
   HIP_TRY(hipMemsetAsync(h->dLOut, 0, (size_t)n * (NMC_MAX_LAYERS + TF_COUNT + TI_COUNT) * sizeof(double), s));
   HIP_TRY(hipMemsetAsync(h->dRecCount, 0, 2 * sizeof(unsigned long long), s));
+  h->sens_valid = false;
+  if (h->dLSens) HIP_TRY(hipMemsetAsync(h->dLSens, 0, (size_t)n * kSensPerStack * sizeof(double), s));
   HIP_TRY(hipEventRecord(h->e0, s));
   HIP_TRY(launch_state_begin(h->dState, s));
   LayArgs a{};
@@ -1552,16 +1566,39 @@ int nmc_layered_run(nmc_handle* h) {
   a.rec_count = h->dRecCount;
   a.rec = h->dRec;
   a.state = h->dState;
+  a.sens = h->dLSens != nullptr;
+  a.sens_part = h->dLSensPart;
   HIP_TRY(launch_layered(a, h->lay_grid, h->lay_lds, s));
   h->launches = 1;
   HIP_TRY(launch_reduce_partials(h->dLPart, h->lay_grid, (int64_t)n * (NMC_MAX_LAYERS + TF_COUNT + TI_COUNT),
                                  h->dLOut, n, NMC_MAX_LAYERS, s));
   HIP_TRY(launch_reduce_partials(h->dLRdPart, h->lay_rd_count, rd_len, h->dLRd, n, h->lcfg.nr + 1, s,
                                  !h->lay_rd_lds));
+  if (h->dLSens)
+    HIP_TRY(launch_reduce_partials(h->dLSensPart, h->lay_grid, (int64_t)n * kSensPerStack, h->dLSens, n,
+                                   kSensPerStack, s));
   HIP_TRY(hipEventRecord(h->e1, s));
   h->timed = true;
   HIP_TRY(hipStreamSynchronize(s));
   h->rec_valid = true;
+  h->sens_valid = h->dLSens != nullptr;
+  return ok();
+}
+
+int nmc_layered_absorbed_grad(nmc_handle* h, double* d_mua, double* d_mus) {
+  if (!h) return fail(NMC_ERR_ARG, "null handle");
+  if (h->sim != NMC_SIM_LAYERED || !h->prepared) return fail(NMC_ERR_STATE, "nmc_layered_prepare has not succeeded");
+  if (!h->dLSens) return fail(NMC_ERR_STATE, "absorbed_grad was not enabled at prepare");
+  if (!h->sens_valid) return fail(NMC_ERR_STATE, "no nmc_layered_run since the last change of properties");
+  HIP_TRY(hipSetDevice(h->device));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  const size_t n = h->stacks.size(), per = (size_t)NMC_MAX_LAYERS * NMC_MAX_LAYERS;
+  std::vector<double> host(n * kSensPerStack);
+  HIP_TRY(hipMemcpy(host.data(), h->dLSens, host.size() * sizeof(double), hipMemcpyDeviceToHost));
+  for (size_t j = 0; j < n; ++j) {
+    if (d_mua) std::copy(&host[j * kSensPerStack], &host[j * kSensPerStack] + per, d_mua + j * per);
+    if (d_mus) std::copy(&host[j * kSensPerStack + per], &host[j * kSensPerStack] + 2 * per, d_mus + j * per);
+  }
   return ok();
 }
 

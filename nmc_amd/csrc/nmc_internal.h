@@ -173,7 +173,14 @@ struct LayArgs {
   uint32_t* rec;                    // [cap][3 + 2 rec_nl]: stack, bin, w, L[rec_nl], K[rec_nl];
                                     // stack 0xFFFFFFFF marks an unused slot
   RunState* state;
+  // absorbed-weight sensitivities (sens = 1): block partials
+  // [grid][n_items][2][MAXL][MAXL] of d(absorbed in layer l)/d(mua_t | mus_t)
+  int sens;
+  double* sens_part;
 };
+
+// Doubles of the absorbed-weight sensitivities per stack: [2][MAXL][MAXL].
+constexpr int kSensPerStack = 2 * NMC_MAX_LAYERS * NMC_MAX_LAYERS;
 
 // Record slots a wave reserves at once (one global atomic per kRecChunk records).
 constexpr int kRecChunk = 256;
@@ -193,7 +200,7 @@ struct LayBackArgs {
 };
 
 constexpr int kLayBlock = 1024;
-size_t layered_lds_bytes(int n_items, int n_rows, int rd_len);
+size_t layered_lds_bytes(int n_items, int n_rows, int rd_len, bool sens = false);
 // Global tally replicas (Rd(r) of the layered kernel, Rd/A of batched mcsub)
 // when the tallies do not fit in LDS: as many as kRepBytes allows, at least
 // kMinReps (one per XCD), at most one per block.

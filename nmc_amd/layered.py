@@ -86,6 +86,16 @@ class LayeredMC:
         return {"stack": buf[:, 0], "bin": buf[:, 1], "w": buf[:, 2].view(np.float32),
                 "L": buf[:, 3:3 + k].view(np.float32), "K": buf[:, 3 + k:3 + 2 * k]}
 
+    def absorbed_grad(self):
+        """Sensitivities of the raw weight absorbed per layer (config
+        absorbed_grad = 1): (d/dmua, d/dmus) [n_stacks, MAX_LAYERS (absorbing
+        layer l), MAX_LAYERS (parameter layer t)]."""
+        n = self.n_stacks
+        da = np.zeros((n, P.MAX_LAYERS, P.MAX_LAYERS))
+        ds = np.zeros((n, P.MAX_LAYERS, P.MAX_LAYERS))
+        check(lib().nmc_layered_absorbed_grad(self._h, da.ctypes.data, ds.ctypes.data))
+        return da, ds
+
     def get_tuning(self) -> P.Tuning:
         """Effective launch values; `mode` 1 = Rd(r) in LDS, 2 = global replicas."""
         t = P.Tuning()

@@ -181,7 +181,8 @@ class LayeredConfig(C.Structure):
     _fields_ = [("struct_size", C.c_uint32), ("version", C.c_uint32), ("seed", C.c_uint64),
                 ("nr", C.c_int32), ("dr", C.c_float), ("threshold", C.c_float),
                 ("chance", C.c_float), ("max_events", C.c_uint32), ("records", C.c_int32),
-                ("record_capacity", C.c_uint64), ("reserved", C.c_int32 * 8)]
+                ("record_capacity", C.c_uint64), ("absorbed_grad", C.c_int32),
+                ("reserved", C.c_int32 * 7)]
 
     @classmethod
     def default(cls, **kw) -> "LayeredConfig":

@@ -54,7 +54,9 @@
  *   2  the remaining step keeps its physical length across a change of
  *      tissue or layer (no rescale of the optical depth by mus)
  *   3  mcsub drops its weight at the position before the hop
- *   4  mcsub bins counted from 1 (the first bin left empty)             */
+ *   4  mcsub bins counted from 1 (the first bin left empty)
+ *   5  the absorbed-weight sensitivity to mus drops the score's growth
+ *      within a segment (the W s h(x) term)                             */
 #ifndef NMCO_MUTANT
 #define NMCO_MUTANT 0
 #endif
@@ -1030,7 +1032,48 @@ typedef struct {
     float* rec_L;        /* [NMC_MAX_LAYERS][cap] */
     uint32_t* rec_K;     /* [NMC_MAX_LAYERS][cap] */
     uint32_t item;
+    double* sens;        /* [2][NMC_MAX_LAYERS][NMC_MAX_LAYERS] or NULL */
 } lay_acc;
+
+/* Absorbed-weight sensitivities of one layer visit (no reference
+ * counterpart).  The weight absorbed in layer l is the path integral of
+ * mua_l W; the density of the path up to a point has the score
+ * sum_t (K_t/mus_t - L_t) for mus_t, with K_t, L_t the scatterings and length
+ * before that point.  A segment of length s entered with weight W deposits
+ * D = W (1 - e^{-x}), x = mua_l s, and contributes
+ *   dD/dmua_t = -L_t D + [t = l] W s e^{-x},
+ *   dD/dmus_t = (K_t/mus_t - L_t) D - [t = l] W s h(x),
+ * h(x) = (1 - e^{-x} (1 + x)) / x.  During a visit to l the other layers'
+ * totals are fixed, so a visit keeps the sums of D L_l, D K_l, W s e^{-x}
+ * and W s h(x) (pinned by central finite differences with common random
+ * numbers and by the closed form of a non-scattering slab,
+ * tests/test_oracle_layered.py). */
+typedef struct { double D; float sp, sq, sza, szs; } lay_visit;
+
+static float score_growth(float x, float em)
+{
+    if (x < 0.05f) return x * (0.5f - x * (0.33333334f - x * (0.125f - x * 0.033333335f)));
+    return (-em - x * (1.0f + em)) / x;
+}
+
+static void sens_fold(const lay_stack* S, int l, const float* L, const uint32_t* K,
+                      lay_visit* v, double* sens)
+{
+    if (sens && (v->D != 0.0 || v->sza != 0.0f)) {
+        double* S0 = sens + l * NMC_MAX_LAYERS;
+        double* S1 = sens + NMC_MAX_LAYERS * NMC_MAX_LAYERS + l * NMC_MAX_LAYERS;
+        for (int t = 0; t < S->nl; ++t) {
+            if (t == l || v->D == 0.0) continue;
+            S0[t] += -(double)L[t] * v->D;
+            S1[t] += ((double)K[t] * (double)S->inv_mus[t] - (double)L[t]) * v->D;
+        }
+        S0[l] += -(double)v->sp + (double)v->sza;
+        S1[l] += (double)v->sq * (double)S->inv_mus[l] - (double)v->sp -
+                 (NMCO_MUTANT == 5 ? 0.0 : (double)v->szs);
+    }
+    v->D = 0.0;
+    v->sp = v->sq = v->sza = v->szs = 0.0f;
+}
 
 void nmco_layered_prepare_stack(const nmc_stack* s, uint64_t seed, lay_stack* o);
 
@@ -1051,6 +1094,8 @@ static void lay_photon(const lay_stack* S, const lay_cfg* c, uint64_t pid, lay_a
      * the photon leaves the layer */
     float L[NMC_MAX_LAYERS] = {0}, Lcur = 0.0f;
     uint32_t K[NMC_MAX_LAYERS] = {0};
+    lay_visit v = {0.0, 0.0f, 0.0f, 0.0f, 0.0f};
+#define SENS_FOLD() sens_fold(S, l, L, K, &v, a->sens)
     block(S->key, pid, ev, 0u, b);
     float sleft = -nmco_logf(u01nz(b[0]));
     for (;;) {
@@ -1064,11 +1109,20 @@ static void lay_photon(const lay_stack* S, const lay_cfg* c, uint64_t pid, lay_a
         x = fmaf(s, ux, x);
         y = fmaf(s, uy, y);
         z = fmaf(s, uz, z);
-        float em = nmco_expm1f(-(mua * s));
+        const float xa = mua * s;
+        float em = nmco_expm1f(-xa);
         float dep = -(W * em);
+        if (a->sens) {
+            v.sp = v.sp + dep * (L[l] + Lcur);
+            v.sq = v.sq + dep * (float)K[l];
+            const float ws = W * s;
+            v.sza = v.sza + ws * (1.0f + em);
+            v.szs = v.szs + ws * score_growth(xa, em);
+        }
         W = W - dep;
         a->absorbed[l] += (double)dep;
         t->absorbed += (double)dep;
+        v.D += (double)dep;
         Lcur = Lcur + s;
         events++;
         if (inter) {
@@ -1090,6 +1144,7 @@ static void lay_photon(const lay_stack* S, const lay_cfg* c, uint64_t pid, lay_a
                 uz = -uz;
                 ruz = -ruz;
             } else if (up && l == 0) {
+                SENS_FOLD();
                 L[l] = L[l] + Lcur;
                 float re = sqrtf(x * x + y * y);
                 int ir = rbin(re / c->dr, c->nr);
@@ -1110,10 +1165,12 @@ static void lay_photon(const lay_stack* S, const lay_cfg* c, uint64_t pid, lay_a
                 }
                 return;
             } else if (!up && l == S->nl - 1) {
+                SENS_FOLD();
                 t->tt += (double)W;
                 t->escapes_bottom++;
                 return;
             } else {
+                SENS_FOLD();
                 L[l] = L[l] + Lcur;
                 Lcur = 0.0f;
                 if (ni != nt) {
@@ -1127,8 +1184,9 @@ static void lay_photon(const lay_stack* S, const lay_cfg* c, uint64_t pid, lay_a
                 l = lnew;
             }
         }
-        if (!(W > 0.0f)) return; /* all weight absorbed */
+        if (!(W > 0.0f)) { SENS_FOLD(); return; } /* all weight absorbed */
         if (events >= c->max_events) {
+            SENS_FOLD();
             t->capped += (double)W;
             t->capped_photons++;
             return;
@@ -1139,11 +1197,12 @@ static void lay_photon(const lay_stack* S, const lay_cfg* c, uint64_t pid, lay_a
         ruz = 1.0f / uz;
         K[l]++;
         t->interactions++;
-        if (!roulette(&W, u01(b[3]), c->thr, c->chance, t)) return;
+        if (!roulette(&W, u01(b[3]), c->thr, c->chance, t)) { SENS_FOLD(); return; }
         ev++;
         block(S->key, pid, ev, 0u, b);
         sleft = -nmco_logf(u01nz(b[0]));
     }
+#undef SENS_FOLD
 }
 
 /* Interface depths summed in double, rounded once to float. */
@@ -1178,9 +1237,10 @@ void nmco_layered_prepare_stack(const nmc_stack* s, uint64_t seed, lay_stack* o)
 int nmco_layered(const nmc_layered_config* cfg, const nmc_stack* stacks, int n_stacks,
                  uint64_t first, int n_threads, double* rd, double* absorbed, nmc_totals* totals,
                  uint64_t* n_rec, uint64_t cap, uint32_t* rec_item, uint32_t* rec_bin,
-                 float* rec_w, float* rec_L, uint32_t* rec_K)
+                 float* rec_w, float* rec_L, uint32_t* rec_K, double* sens)
 {
     if (!cfg || !stacks || n_stacks <= 0 || cfg->nr <= 0 || !(cfg->dr > 0.0f)) return NMC_ERR_ARG;
+    const int per_sens = 2 * NMC_MAX_LAYERS * NMC_MAX_LAYERS;
     lay_cfg c = {cfg->nr, cfg->dr, cfg->threshold, cfg->chance, cfg->max_events};
     int nt = n_threads > 0 ? n_threads : nmco_max_threads();
     double t0 = now_s();
@@ -1194,6 +1254,7 @@ int nmco_layered(const nmc_layered_config* cfg, const nmc_stack* stacks, int n_s
         for (int i = 0; i < ntj; ++i) {
             acc[i].rd = (double*)calloc((size_t)(cfg->nr + 1), sizeof(double));
             acc[i].absorbed = (double*)calloc(NMC_MAX_LAYERS, sizeof(double));
+            if (sens) acc[i].sens = (double*)calloc((size_t)per_sens, sizeof(double));
         }
         if (rec_w) {
             /* records are written in photon order: run the threads' ranges
@@ -1220,6 +1281,10 @@ int nmco_layered(const nmc_layered_config* cfg, const nmc_stack* stacks, int n_s
         for (int i = 0; i < ntj; ++i) {
             for (int b = 0; b <= cfg->nr; ++b) rd[(int64_t)j * (cfg->nr + 1) + b] += acc[i].rd[b];
             for (int b = 0; b < NMC_MAX_LAYERS; ++b) absorbed[(int64_t)j * NMC_MAX_LAYERS + b] += acc[i].absorbed[b];
+            if (sens) {
+                for (int b = 0; b < per_sens; ++b) sens[(int64_t)j * per_sens + b] += acc[i].sens[b];
+                free(acc[i].sens);
+            }
             totals_add(&totals[j], &acc[i].t);
             if (rec_w) {
                 uint64_t m = *acc[i].n_rec;

@@ -45,10 +45,12 @@ int nmco_mcxyz(const RunParams* rp, const TissueParams* tp, const uint8_t* v,
 int nmco_mcsub(const RunParams* rp, const TissueParams* tp, const nmc_ext_config* ext,
                uint64_t first, uint64_t n, int n_threads, nmc_results* out);
 
+/* sens (may be NULL): [n_stacks][2][MAXL][MAXL] += d(absorbed in layer l) /
+ * d(mua_t | mus_t), the perturbation-MC sums over every deposit. */
 int nmco_layered(const nmc_layered_config* cfg, const nmc_stack* stacks, int n_stacks,
                  uint64_t first, int n_threads, double* rd, double* absorbed, nmc_totals* totals,
                  uint64_t* n_rec, uint64_t cap, uint32_t* rec_item, uint32_t* rec_bin,
-                 float* rec_w, float* rec_L, uint32_t* rec_K);
+                 float* rec_w, float* rec_L, uint32_t* rec_K, double* sens);
 void nmco_layered_backward(const nmc_layered_config* cfg, const nmc_stack* stacks, int n_stacks,
                            uint64_t n_rec, uint64_t cap, const uint32_t* rec_item,
                            const uint32_t* rec_bin, const float* rec_w, const float* rec_L,

@@ -56,7 +56,8 @@ def _bind(L):
     L.nmco_layered.argtypes = [C.POINTER(P.LayeredConfig), C.POINTER(P.Stack), C.c_int,
                                C.c_uint64, C.c_int, C.c_void_p, C.c_void_p,
                                C.POINTER(P.Totals), C.POINTER(C.c_uint64), C.c_uint64,
-                               C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+                               C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                               C.c_void_p]
     L.nmco_layered_backward.argtypes = [C.POINTER(P.LayeredConfig), C.POINTER(P.Stack), C.c_int,
                                         C.c_uint64, C.c_uint64] + [C.c_void_p] * 8
     L.nmco_last_seconds.restype = C.c_double
@@ -76,7 +77,8 @@ def lib():
 MUTANTS = {1: "flat beam radius R*xi instead of R*sqrt(xi)",
            2: "remaining step keeps its physical length across a tissue/layer change",
            3: "mcsub drops its weight at the pre-hop position",
-           4: "mcsub bins counted from 1"}
+           4: "mcsub bins counted from 1",
+           5: "absorbed sensitivity to mus without the within-segment score term"}
 _mutants: dict = {}
 
 
@@ -181,8 +183,10 @@ class LayeredOut:
 
 
 def run_layered(cfg: P.LayeredConfig, stacks, first: int = 0, threads: int = 0,
-                records: bool = True, L=None) -> LayeredOut:
-    """Oracle run of an array of Stack structs; records (if asked) in photon order."""
+                records: bool = True, L=None, sens: bool = False) -> LayeredOut:
+    """Oracle run of an array of Stack structs; records (if asked) in photon
+    order; sens: the absorbed-weight sensitivities o.sens_mua, o.sens_mus
+    [n_stacks, MAXL (absorbing layer), MAXL (parameter layer)]."""
     L = L or lib()
     ns = len(stacks)
     arr = (P.Stack * ns)(*stacks)
@@ -205,8 +209,12 @@ def run_layered(cfg: P.LayeredConfig, stacks, first: int = 0, threads: int = 0,
                 o.rec_L.ctypes.data, o.rec_K.ctypes.data]
     else:
         ptrs = [None] * 5
+    sens_arr = np.zeros((ns, 2, P.MAX_LAYERS, P.MAX_LAYERS)) if sens else None
     st = L.nmco_layered(C.byref(cfg), arr, ns, first, threads, o.rd.ctypes.data,
-                        o.absorbed.ctypes.data, tots, C.byref(nrec), cap, *ptrs)
+                        o.absorbed.ctypes.data, tots, C.byref(nrec), cap, *ptrs,
+                        sens_arr.ctypes.data if sens else None)
+    if sens:
+        o.sens_mua, o.sens_mus = sens_arr[:, 0], sens_arr[:, 1]
     if st != 0:
         raise RuntimeError(f"oracle returned status {st}")
     o.totals = [t.as_dict() for t in tots]

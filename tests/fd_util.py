@@ -61,6 +61,22 @@ def fd_groups(rd_pert: np.ndarray, stack, nl: int):
     return fa, fs
 
 
+def fd_values(vals: np.ndarray, stack, nl: int):
+    """FD of per-group values (e.g. the absorbed weight of each layer) from
+    the perturbed runs' values vals [PER_LAYER nl, G]: (fd_mua [G, nl],
+    fd_mus [G, nl])."""
+    G = vals.shape[1]
+    fa = np.zeros((G, nl))
+    fs = np.zeros((G, nl))
+    for l in range(nl):
+        r = [vals[PER_LAYER * l + i] for i in range(PER_LAYER)]
+        ha = H_MUA * stack.layer[l].mua
+        hs = H_MUS * stack.layer[l].mus
+        fa[:, l] = (r[0] - r[1]) / (2 * ha)
+        fs[:, l] = (8 * (r[2] - r[3]) - (r[4] - r[5])) / (12 * hs)
+    return fa, fs
+
+
 def grad_rd_groups(nr: int) -> np.ndarray:
     """One grad_rd row per group: the indicator of its bins."""
     g = np.zeros((len(GROUPS), nr + 1))

@@ -265,3 +265,84 @@ def test_pmc_central_difference_per_layer_c4_stack(gpu):
     ra, rs = _gpu_fd_check(cfg, stacks[0], 16, 625_000, "C4")
     assert ra["counts"]["zero"] >= 5 and ra["counts"]["1%"] >= 3
     assert rs["counts"]["zero"] >= 5
+
+
+@pytest.mark.parametrize("case", ["c4", "thin", "batched"])
+def test_absorbed_sensitivities_match_oracle(gpu, case):
+    """The absorbed-weight sensitivities (absorbed_grad = 1) equal the
+    oracle's on the same photons (FP64 summation order aside), and turning
+    them on changes none of the other tallies."""
+    from test_oracle_layered import thin_stack
+    if case == "c4":
+        cfg, stacks = W.multilayer_5(n_photons=20_000)
+    elif case == "thin":
+        cfg, stacks = P.LayeredConfig.default(nr=200, dr=0.005), [thin_stack(20_000)]
+    else:
+        cfg, stacks = W.inverse_fit_instances(n_inst=6, budget_scale=0.005)
+    cfg.absorbed_grad = 1
+    _say(f"[layered] absorbed sensitivities {case}")
+    ref = O.run_layered(cfg, stacks, threads=8, sens=True)
+    with LayeredMC(cfg, stacks) as mc:
+        mc.run()
+        res = mc.results()
+        da, ds = mc.absorbed_grad()
+    _compare(res, ref, len(stacks))
+    for got, exp in ((da, ref.sens_mua), (ds, ref.sens_mus)):
+        scale = np.abs(exp).max()
+        np.testing.assert_allclose(got, exp, rtol=1e-9, atol=1e-12 * scale)
+        assert np.array_equal(got == 0, exp == 0)
+    assert np.abs(da).max() > 0
+
+
+def test_absorbed_autograd_jacobian(gpu):
+    """layered_absorbed's backward is the sensitivities contracted with the
+    incoming gradient."""
+    import torch
+    from nmc_amd.autograd import layered_absorbed
+    from test_oracle_layered import thin_stack
+    cfg = P.LayeredConfig.default(nr=10, dr=1.0, records=0)
+    cfg.absorbed_grad = 1
+    st = thin_stack(50_000)
+    with LayeredMC(cfg, [st]) as mc:
+        mua = torch.tensor([[st.layer[l].mua for l in range(5)]], dtype=torch.float64, requires_grad=True)
+        mus = torch.tensor([[st.layer[l].mus for l in range(5)]], dtype=torch.float64, requires_grad=True)
+        a = layered_absorbed(mua, mus, mc)
+        w = torch.arange(1.0, 6.0, dtype=torch.float64)[None, :]
+        (a * w).sum().backward()
+        da, ds = mc.absorbed_grad()
+    n = st.n_photons
+    np.testing.assert_allclose(mua.grad.numpy()[0], (w.numpy()[0][:, None] * da[0, :5, :5] / n).sum(0),
+                               rtol=1e-12)
+    np.testing.assert_allclose(mus.grad.numpy()[0], (w.numpy()[0][:, None] * ds[0, :5, :5] / n).sum(0),
+                               rtol=1e-12)
+
+
+def test_absorbed_sensitivities_central_difference_thin_stack(gpu):
+    """d(absorbed in layer l)/d(mua_t, mus_t), all 25 layer pairs of the thin
+    stack, N = 16 x 1e7, against CRN central differences (fd_util's bar)."""
+    from test_oracle_layered import thin_stack
+    cfg = P.LayeredConfig.default(nr=10, dr=1.0, records=0)
+    cfg.absorbed_grad = 1
+    cfg_fd = P.LayeredConfig.from_buffer_copy(cfg)
+    cfg_fd.absorbed_grad = 0
+    K, m = 16, 10_000_000
+    base = thin_stack(m)
+    nl = 5
+    pert = fd_util.perturbed(base, nl)
+    _say(f"[layered] absorbed sensitivities vs CRN central FD, {K} x {m} photons")
+    pa, ps, fa, fs = [], [], [], []
+    with LayeredMC(cfg, [base]) as mb, LayeredMC(cfg_fd, pert) as mp:
+        for k in range(K):
+            mb.set_seed(cfg.seed, k * m)
+            mb.run()
+            da, ds = mb.absorbed_grad()
+            pa.append(da[0, :nl, :nl] / m), ps.append(ds[0, :nl, :nl] / m)
+            mp.set_seed(cfg.seed, k * m)
+            mp.run()
+            f1, f2 = fd_util.fd_values(mp.results()[1][:, :nl] / m, base, nl)
+            fa.append(f1), fs.append(f2)
+    ra = fd_util.judge(np.array(pa), np.array(fa), "absorbed mua")
+    rs = fd_util.judge(np.array(ps), np.array(fs), "absorbed mus")
+    _say(fd_util.table(f"thin dA_l/dmua_t, N = {K} x {m}", ra))
+    _say(fd_util.table(f"thin dA_l/dmus_t, N = {K} x {m}", rs))
+    assert ra["counts"]["1%"] == 25

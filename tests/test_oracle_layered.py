@@ -145,3 +145,74 @@ def test_pmc_matches_central_difference_per_layer_c4_stack():
     cfg, stacks = W.multilayer_5(n_photons=1)
     ra, rs = _oracle_fd_check(cfg, stacks[0], 8, 12_500, "C4")
     assert ra["counts"]["zero"] >= 15 and ra["counts"]["1%"] >= 3
+
+
+# ------------------------------------------- absorbed-weight sensitivities ---
+def test_absorbed_sensitivity_to_mua_non_scattering_slab():
+    """A non-scattering slab with matched indices: every photon crosses it
+    straight, absorbing A = 1 - e^{-mua d}, so dA/dmua = d e^{-mua d} exactly,
+    photon by photon."""
+    d, mua, n = 0.05, 7.0, 20_000
+    cfg = P.LayeredConfig.default(nr=10, dr=1.0, records=0)
+    st = W.make_stack([d, 0.0], [mua, 1.0], [0.0, 100.0], [1.0, 1.0], n, g=0.9)
+    st.n_layers = 1
+    st.layer[0].thickness = d
+    o = O.run_layered(cfg, [st], threads=4, records=False, sens=True)
+    assert o.absorbed[0, 0] / n == pytest.approx(1 - math.exp(-mua * d), rel=1e-6)
+    assert o.sens_mua[0, 0, 0] / n == pytest.approx(d * math.exp(-mua * d), rel=1e-6)
+
+
+def test_absorbed_sensitivity_to_mus_vanishes_without_deflection(tmp_path_factory):
+    """g = 1: scattering never changes a path, so the absorbed weight cannot
+    depend on mus: the estimator's mean is 0 within its noise.  Without the
+    score's growth inside each segment (planted mistake 5) it is not."""
+    n, K = 800_000, 16
+    cfg = P.LayeredConfig.default(nr=10, dr=1.0, records=0)
+
+    def per_batch(L=None):
+        out = []
+        for k in range(K):
+            st = W.make_stack([0.02, 0.03], [3.0, 5.0], [80.0, 150.0], [1.0, 1.0], n // K, g=1.0)
+            o = O.run_layered(cfg, [st], first=k * (n // K), threads=8, records=False, sens=True, L=L)
+            out.append(o.sens_mus[0, :2, :2].ravel() / (n // K))
+        return np.array(out)
+
+    ok = per_batch()
+    # light reaches layer 1 after layer 0, so d(absorbed in 0)/dmus_1 is 0
+    assert not ok[:, 1].any()
+    use = [0, 2, 3]
+    z = ok.mean(0)[use] / (ok.std(0, ddof=1)[use] / math.sqrt(K))
+    assert np.abs(z).max() < 4.5, z
+    bad = per_batch(O.mutant(5, str(tmp_path_factory.mktemp("m5"))))
+    zb = bad.mean(0)[use] / (bad.std(0, ddof=1)[use] / math.sqrt(K))
+    assert np.abs(zb).max() > 10, "the g = 1 pin misses the within-segment term"
+
+
+def _oracle_absorbed_fd_check(stack, K, m, label):
+    """Absorbed weight per layer: perturbation-MC sensitivities against
+    central FD with common random numbers, per batch (fd_util's bar with the
+    absorbing layer as the group)."""
+    cfg = P.LayeredConfig.default(nr=10, dr=1.0, records=0)
+    nl = stack.n_layers
+    pa, ps, fa, fs = [], [], [], []
+    for k in range(K):
+        base = P.Stack.from_buffer_copy(stack)
+        base.n_photons = m
+        o = O.run_layered(cfg, [base], first=k * m, threads=8, records=False, sens=True)
+        pa.append(o.sens_mua[0, :nl, :nl] / m)
+        ps.append(o.sens_mus[0, :nl, :nl] / m)
+        op = O.run_layered(cfg, fd_util.perturbed(base, nl), first=k * m, threads=8, records=False)
+        f1, f2 = fd_util.fd_values(op.absorbed[:, :nl] / m, base, nl)
+        fa.append(f1), fs.append(f2)
+    ra = fd_util.judge(np.array(pa), np.array(fa), label + " dA/dmua")
+    rs = fd_util.judge(np.array(ps), np.array(fs), label + " dA/dmus")
+    print(fd_util.table(label + " dA_l/dmua_t (group = l)", ra))
+    print(fd_util.table(label + " dA_l/dmus_t (group = l)", rs))
+    return ra, rs
+
+
+def test_absorbed_sensitivities_match_central_difference_thin_stack():
+    """d(absorbed in layer l)/d(mua_t, mus_t) for all 25 layer pairs of the
+    thin stack, N = 8 x 25000, against CRN central differences."""
+    ra, rs = _oracle_absorbed_fd_check(thin_stack(1), 8, 25_000, "thin")
+    assert ra["counts"]["1%"] >= 20
