@@ -47,13 +47,15 @@ OPS_SAME, OPS_CROSS, OPS_INT = 44, 97, 145
 # gfx950: 256 CUs x 64 FP32 lanes x 2.4 GHz, non-packed lane-ops/s (T).
 VALU_PEAK_T = 256 * 64 * 2.4e9 / 1e12
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per photon of the timed kernel on this workload, by tile element
-# (tuning.mode 0 FP64, 1 FP32), from the rocprofv3 FETCH_SIZE and WRITE_SIZE
-# passes over one 1e8-photon launch, scaled by the copy calibration (FETCH
-# x2.0, WRITE x1.0): FP64 (18 955 068 KiB x 2 + 98 872 849 KiB) x 1024 B / 1e8,
-# FP32 (18 945 480 KiB x 2 + 92 328 830 KiB) x 1024 B / 1e8, each on the
-# visit-pilot box (profiles/mcxyz_skinvessel_200_pmc_traffic.md).
-TRAFFIC_BYTES_PER_PHOTON = {0: 1400.7, 1: 1333.5}
+# HBM bytes per photon of the timed kernel on this workload, by tuning.mode
+# (0 FP64 tile, 1 FP32 tile, 2 FP32 tile + binned records), from the
+# rocprofv3 FETCH_SIZE and WRITE_SIZE passes over one 1e8-photon launch,
+# scaled by the copy calibration (FETCH x2.0, WRITE x1.0): FP64
+# (18 955 068 KiB x 2 + 98 872 849 KiB) x 1024 B / 1e8, FP32 (18 945 480 KiB
+# x 2 + 92 328 830 KiB) x 1024 B / 1e8, records (38 779 902 KiB x 2 +
+# 63 476 753 KiB) x 1024 B / 1e8, each on the visit-pilot box
+# (profiles/mcxyz_skinvessel_200_pmc_traffic.md).
+TRAFFIC_BYTES_PER_PHOTON = {0: 1400.7, 1: 1333.5, 2: 1444.2}
 
 
 def log(msg: str) -> None:
