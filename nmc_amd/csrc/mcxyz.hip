@@ -89,11 +89,13 @@ struct RecOut {
 
 // Adds the deposits of the current voxel visit to the tile or the global grid
 // (kFluCount: adds 1 per visit instead; kFluRecords: appends a record instead
-// of the global atomic while the bucket's slice has room).
+// of the global atomic while the bucket's slice has room), and to the lane's
+// absorbed total.
 template <int FMODE, typename TileT>
 __device__ __forceinline__ void flush_visit(Photon& p, const XyzArgs& a, const TileBox& tb,
-                                            TileT* tile, const RecOut& ro) {
+                                            TileT* tile, const RecOut& ro, double& absorbed) {
   constexpr bool COUNT = FMODE == kFluCount;
+  absorbed += p.vdep;
   if (p.inside && (COUNT || p.vdep != 0.0)) {
     double v = COUNT ? 1.0 : p.vdep;
     uint32_t lx = (uint32_t)p.ix - tb.x0, ly = (uint32_t)p.iy - tb.y0, lz = (uint32_t)p.iz - tb.z0;
@@ -117,17 +119,34 @@ __device__ __forceinline__ void flush_visit(Photon& p, const XyzArgs& a, const T
   p.vdep = 0.0;
 }
 
+// Event tallies of the transport loop into the workgroup totals in LDS, at
+// an address the compiler cannot prove wave-uniform (zv: a zero in a vector
+// register).  For a uniform address it would sum the active lanes first, an
+// FP64 add per active lane in a loop that the whole wave pays each time any
+// lane takes the branch; this way each lane issues its own LDS atomic, one
+// instruction for the wave.
+struct LoopTotals {
+  nmc_totals* st;
+  uint32_t zv;
+  __device__ __forceinline__ void addf(int k, double v) const {
+    unsafeAtomicAdd(&((double*)st)[(uint32_t)k + zv], v);
+  }
+  __device__ __forceinline__ void addi(int k) const {
+    atomicAdd(&((unsigned long long*)&st->photons)[(uint32_t)k + zv], 1ull);
+  }
+};
+
 // Escape through a grid face under boundaryflag 1 (or the top under 2).
-__device__ __forceinline__ void tally_escape(nmc_totals* st, const XyzArgs& a, int iz, float W) {
+__device__ __forceinline__ void tally_escape(const LoopTotals& t, const XyzArgs& a, int iz, float W) {
   if (iz >= a.Nz) {
-    lds_addf(st, TF_TT, (double)W);
-    lds_addi(st, TI_ESC_BOTTOM, 1);
+    t.addf(TF_TT, (double)W);
+    t.addi(TI_ESC_BOTTOM);
   } else if (iz < 0) {
-    lds_addf(st, TF_TOP, (double)W);
-    lds_addi(st, TI_ESC_TOP, 1);
+    t.addf(TF_TOP, (double)W);
+    t.addi(TI_ESC_TOP);
   } else {
-    lds_addf(st, TF_LATERAL, (double)W);
-    lds_addi(st, TI_ESC_SIDE, 1);
+    t.addf(TF_LATERAL, (double)W);
+    t.addi(TI_ESC_SIDE);
   }
 }
 
@@ -225,9 +244,9 @@ __device__ __noinline__ DetectorHit detector_plane(float x, float y, float uz, f
 // LAUNCH (kernels.metal:366-486).  Leaves p.alive false if the photon starts
 // outside a grid whose faces absorb it.
 __device__ __forceinline__ void launch(Photon& p, const XyzArgs& a, const XyzTissue* T,
-                                      nmc_totals* st) {
+                                      const LoopTotals& lt) {
   const SourceState o = source_photon(p.pid, a.key0, a.key1, a.aux);
-  if (o.spec != 0.0f) lds_addf(st, TF_SPECULAR, (double)o.spec);
+  if (o.spec != 0.0f) lt.addf(TF_SPECULAR, (double)o.spec);
   const float x = o.x, y = o.y, z = o.z, ux = o.ux, uy = o.uy, uz = o.uz, W = o.W;
   p.x = x; p.y = y; p.z = z; p.ux = ux; p.uy = uy; p.uz = uz; p.W = W;
   p.ix = (int)floorf(a.hx + x / a.dx);
@@ -236,7 +255,7 @@ __device__ __forceinline__ void launch(Photon& p, const XyzArgs& a, const XyzTis
   p.inside = p.ix >= 0 && p.ix < a.Nx && p.iy >= 0 && p.iy < a.Ny && p.iz >= 0 && p.iz < a.Nz;
   p.vdep = 0.0;
   if (!p.inside && (a.bmode == 1 || (a.bmode == 2 && p.iz < 0))) {
-    tally_escape(st, a, p.iz, W);
+    tally_escape(lt, a, p.iz, W);
     p.alive = false;
     return;
   }
@@ -316,6 +335,7 @@ __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
   uint64_t pool_next = 0, pool_end = 0;  // wave-uniform
   double absorbed = 0.0, absorbed_out = 0.0;
   uint32_t c_same = 0, c_cross = 0, c_int = 0, c_phot = 0;
+  const LoopTotals lt{&st, in_vgpr(0u)};
 
   for (;;) {
     // ---- refill idle lanes, in batches of at least refill_min ----
@@ -340,7 +360,7 @@ __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
         if (((idle >> lane) & 1ull) && rank < take) {
           p.pid = first + pool_next + rank;
           c_phot++;
-          launch(p, a, T, &st);
+          launch(p, a, T, lt);
         }
         pool_next += take;
         idle = ballot(!p.alive && !finished);
@@ -351,16 +371,16 @@ __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
 
     // ---- HOP: one piece of the current step (kernels.metal:505-685) ----
     float s_int = p.sleft * p.inv_mus;
-    float tx = kInf, ty = kInf, tz = kInf;
-    if (p.ux > 0.0f) tx = (((float)(p.ix + 1) - a.hx) * a.dx - p.x) * p.rux;
-    else if (p.ux < 0.0f) tx = (((float)p.ix - a.hx) * a.dx - p.x) * p.rux;
-    if (p.uy > 0.0f) ty = (((float)(p.iy + 1) - a.hy) * a.dy - p.y) * p.ruy;
-    else if (p.uy < 0.0f) ty = (((float)p.iy - a.hy) * a.dy - p.y) * p.ruy;
-    if (p.uz > 0.0f) tz = ((float)(p.iz + 1) * a.dz - p.z) * p.ruz;
-    else if (p.uz < 0.0f) tz = ((float)p.iz * a.dz - p.z) * p.ruz;
-    tx = fmaxf(tx, 0.0f);
-    ty = fmaxf(ty, 0.0f);
-    tz = fmaxf(tz, 0.0f);
+    // distance to the face ahead on each axis (index + 1 going forward, index
+    // going backward; infinite for a zero direction component), computed on
+    // every lane and selected, rather than in branches both of which a wave
+    // with lanes going either way would issue
+    float tx = (((float)(p.ix + (p.ux > 0.0f ? 1 : 0)) - a.hx) * a.dx - p.x) * p.rux;
+    float ty = (((float)(p.iy + (p.uy > 0.0f ? 1 : 0)) - a.hy) * a.dy - p.y) * p.ruy;
+    float tz = ((float)(p.iz + (p.uz > 0.0f ? 1 : 0)) * a.dz - p.z) * p.ruz;
+    tx = p.ux != 0.0f ? fmaxf(tx, 0.0f) : kInf;
+    ty = p.uy != 0.0f ? fmaxf(ty, 0.0f) : kInf;
+    tz = p.uz != 0.0f ? fmaxf(tz, 0.0f) : kInf;
     int axis;
     float s_face;
     if (tx <= ty && tx <= tz) {
@@ -392,12 +412,14 @@ __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
     p.y = fmaf(s, p.uy, p.y);
     p.z = fmaf(s, p.uz, p.z);
     // DROP (kernels.metal:523-533)
-    float em = expm1_f(-(p.mua * s));
+    float em = expm1_neg(-(p.mua * s));
     float dep = -(p.W * em);
     p.W = p.W - dep;
+    // inside the grid the deposit joins the visit's sum, which the voxel's
+    // flush also adds to the lane's absorbed total (no fluence: at once)
     if (p.inside) {
-      p.vdep += (double)dep;
-      absorbed += (double)dep;
+      if (FMODE == kFluNone) absorbed += (double)dep;
+      else p.vdep += (double)dep;
     } else {
       absorbed_out += (double)dep;
     }
@@ -413,13 +435,13 @@ __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
       if (kind == 1) {
         // detector plane (kernels.metal:556-630)
         p.z = zsurf;
-        lds_addi(&st, TI_SURFACE_HITS, 1);
+        lt.addi(TI_SURFACE_HITS);
         const DetectorHit d = detector_plane(p.x, p.y, p.uz, p.W, p.ix, p.iy, a.aux);
         if (d.where == 1) {
-          lds_addf(&st, TF_RD, (double)d.wt);
-          lds_addi(&st, TI_DETECTIONS, 1);
+          lt.addf(TF_RD, (double)d.wt);
+          lt.addi(TI_DETECTIONS);
         } else if (d.where == 2) {
-          lds_addf(&st, TF_RD_OUTSIDE, (double)d.wt);
+          lt.addf(TF_RD_OUTSIDE, (double)d.wt);
         }
         p.uz = -p.uz;
         p.ruz = -p.ruz;
@@ -427,7 +449,7 @@ __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
         if (!(p.W > 0.0f)) alive = false;
       } else {
         // voxel face (kernels.metal:641-679)
-        if (FMODE != kFluNone) flush_visit<FMODE>(p, a, tb, tile, ro);
+        if (FMODE != kFluNone) flush_visit<FMODE>(p, a, tb, tile, ro, absorbed);
         if (axis == 0) {
           const bool fwd = p.ux > 0.0f;
           p.ix += fwd ? 1 : -1;
@@ -443,7 +465,7 @@ __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
         }
         p.inside = p.ix >= 0 && p.ix < gnx && p.iy >= 0 && p.iy < gny && p.iz >= 0 && p.iz < gnz;
         if (!p.inside && (p.iz < 0 ? esc_top : esc_any) != 0u) {
-          tally_escape(&st, a, p.iz, p.W);
+          tally_escape(lt, a, p.iz, p.W);
           alive = false;
         } else {
           p.tis = vmap[p.inside ? p.lin
@@ -456,8 +478,8 @@ __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
     }
     if (alive && !(p.W > 0.0f)) alive = false;  // all weight absorbed
     if (alive && p.events >= a.max_events) {
-      lds_addf(&st, TF_CAPPED, (double)p.W);
-      lds_addi(&st, TI_CAPPED, 1);
+      lt.addf(TF_CAPPED, (double)p.W);
+      lt.addi(TI_CAPPED);
       alive = false;
     }
     if (alive && !(p.sleft > 0.0f)) {
@@ -472,12 +494,12 @@ __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
       if (p.W < a.thr) {
         if (u01(p.b3) <= a.chance) {
           float nw = p.W / a.chance;
-          lds_addf(&st, TF_ROULETTE_GAIN, (double)nw - (double)p.W);
-          lds_addi(&st, TI_ROULETTE_SURVIVALS, 1);
+          lt.addf(TF_ROULETTE_GAIN, (double)nw - (double)p.W);
+          lt.addi(TI_ROULETTE_SURVIVALS);
           p.W = nw;
         } else {
-          lds_addf(&st, TF_ROULETTE_LOSS, (double)p.W);
-          lds_addi(&st, TI_ROULETTE_KILLS, 1);
+          lt.addf(TF_ROULETTE_LOSS, (double)p.W);
+          lt.addi(TI_ROULETTE_KILLS);
           alive = false;
         }
       }
@@ -491,7 +513,7 @@ __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
       }
     }
     if (!alive) {
-      if (FMODE != kFluNone) flush_visit<FMODE>(p, a, tb, tile, ro);
+      if (FMODE != kFluNone) flush_visit<FMODE>(p, a, tb, tile, ro, absorbed);
       p.alive = false;
     }
   }

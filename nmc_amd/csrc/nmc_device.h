@@ -197,6 +197,20 @@ __device__ __forceinline__ float expm1_f(float x) {
   return exp_f(x) - 1.0f;
 }
 
+// expm1_f(x) for x <= 0, bit for bit, as one path for every lane: |x| <=
+// ln2/2 takes k = 0, so r = x exactly (fma(-0, c, x) = x) and the result is
+// em1_core(x); otherwise (exp_f(x) - 1) with k in [-126, -1], whose scale
+// 2^k is a normal float and needs no overflow step.
+__device__ __forceinline__ float expm1_neg(float x) {
+  const bool small = fabsf(x) <= 0x1.62e430p-2f;
+  const float k = small ? 0.0f : rintf(x * 0x1.715476p+0f);
+  float r = fmaf(-k, 0x1.62e4p-1f, x);
+  r = fmaf(-k, 0x1.7f7d1cp-20f, r);
+  const float p = em1_core(r);
+  const float big = (1.0f + p) * __uint_as_float((uint32_t)((int)k + 127) << 23) - 1.0f;
+  return small ? p : (x < -87.0f ? -1.0f : big);
+}
+
 // sin(2 pi u), cos(2 pi u), u in [0,1]: exact quarter-turn reduction, Taylor
 // polynomials of sin (to f^9) and cos (to f^10) of 2 pi f, |f| <= 1/8.
 __device__ __forceinline__ void sincos2pi(float u, float* s, float* c) {

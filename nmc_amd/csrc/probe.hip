@@ -10,7 +10,9 @@
 //   nmcp_div_check    their short division against IEEE division on the
 //                     operand domains of its call sites and on general pairs;
 //   nmcp_sqrt_check   their short square root against sqrtf over every input
-//                     of its domain.
+//                     of its domain;
+//   nmcp_expm1_check  the one-path expm1 of the voxel walk against the
+//                     branched one over every input <= 0.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -154,6 +156,25 @@ __global__ void __launch_bounds__(256) sqrt_check_kernel(unsigned long long* bad
   if (cnt) atomicAdd(bad, cnt);
 }
 
+// Counts the inputs x in [-inf, -0] (every such bit pattern) whose
+// expm1_neg(x) differs in bits from expm1_f(x); keeps the smallest pattern.
+__global__ void __launch_bounds__(256) expm1_check_kernel(unsigned long long* bad,
+                                                          unsigned int* first_bad) {
+  const uint32_t lo = 0x80000000u, hi = 0xFF800000u;   // -0 .. -inf
+  const uint64_t n = (uint64_t)(hi - lo) + 1u;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  unsigned long long cnt = 0;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += stride) {
+    const uint32_t bits = lo + (uint32_t)i;
+    const float x = __uint_as_float(bits);
+    if (__float_as_uint(nmc::expm1_neg(x)) != __float_as_uint(nmc::expm1_f(x))) {
+      ++cnt;
+      atomicMin(first_bad, bits);
+    }
+  }
+  if (cnt) atomicAdd(bad, cnt);
+}
+
 }  // namespace
 
 // Returns 0 on success; *ms = the timed launch, *atomics = the atomics it made.
@@ -283,6 +304,33 @@ extern "C" int nmcp_sqrt_check(unsigned long long* mismatches, unsigned int* fir
   if (hipMemset(d_bad, 0, sizeof(*d_bad)) == hipSuccess &&
       hipMemcpy(d_first, &none, sizeof(none), hipMemcpyHostToDevice) == hipSuccess) {
     hipLaunchKernelGGL(sqrt_check_kernel, dim3(kGrid), dim3(kBlock), 0, 0, d_bad, d_first);
+    if (hipGetLastError() == hipSuccess && hipDeviceSynchronize() == hipSuccess &&
+        hipMemcpy(mismatches, d_bad, sizeof(*d_bad), hipMemcpyDeviceToHost) == hipSuccess &&
+        hipMemcpy(first, d_first, sizeof(*first), hipMemcpyDeviceToHost) == hipSuccess)
+      rc = 0;
+  }
+  hipFree(d_bad);
+  hipFree(d_first);
+  return rc;
+}
+
+// expm1_neg against expm1_f on every input <= 0; returns 0 and fills
+// *mismatches and *first (the smallest differing bit pattern, 0xFFFFFFFF if
+// none).
+extern "C" int nmcp_expm1_check(unsigned long long* mismatches, unsigned int* first) {
+  if (!mismatches || !first) return -1;
+  unsigned long long* d_bad = nullptr;
+  unsigned int* d_first = nullptr;
+  if (hipMalloc(&d_bad, sizeof(*d_bad)) != hipSuccess) return -4;
+  if (hipMalloc(&d_first, sizeof(*d_first)) != hipSuccess) {
+    hipFree(d_bad);
+    return -4;
+  }
+  int rc = -3;
+  const unsigned int none = 0xFFFFFFFFu;
+  if (hipMemset(d_bad, 0, sizeof(*d_bad)) == hipSuccess &&
+      hipMemcpy(d_first, &none, sizeof(none), hipMemcpyHostToDevice) == hipSuccess) {
+    hipLaunchKernelGGL(expm1_check_kernel, dim3(kGrid), dim3(kBlock), 0, 0, d_bad, d_first);
     if (hipGetLastError() == hipSuccess && hipDeviceSynchronize() == hipSuccess &&
         hipMemcpy(mismatches, d_bad, sizeof(*d_bad), hipMemcpyDeviceToHost) == hipSuccess &&
         hipMemcpy(first, d_first, sizeof(*first), hipMemcpyDeviceToHost) == hipSuccess)

@@ -6,7 +6,9 @@ to the oracle's (which divides and takes square roots the IEEE way).
 - division: every operand pair of log_pos, every Henyey-Greenstein divisor of
   256 anisotropies, 2^32 pairs shaped like the rotation's, 2^32 general pairs
   with exponents in [-60, 60];
-- square root: every input in [2^-100, 2^126], and +0."""
+- square root: every input in [2^-100, 2^126], and +0;
+- the voxel walk's one-path expm1 against the branched one: every input
+  in [-inf, -0]."""
 from __future__ import annotations
 
 import ctypes as C
@@ -44,4 +46,12 @@ def test_sqrt_rn_equals_sqrtf_on_its_domain(gpu):
     L.nmcp_sqrt_check.argtypes = [C.POINTER(C.c_ulonglong), C.POINTER(C.c_uint)]
     bad, first = C.c_ulonglong(), C.c_uint()
     assert L.nmcp_sqrt_check(C.byref(bad), C.byref(first)) == 0
+    assert bad.value == 0, f"{bad.value} inputs differ, first 0x{first.value:08x}"
+
+
+def test_expm1_neg_equals_expm1_for_all_nonpositive_inputs(gpu):
+    L = C.CDLL(PROBE_LIB)
+    L.nmcp_expm1_check.argtypes = [C.POINTER(C.c_ulonglong), C.POINTER(C.c_uint)]
+    bad, first = C.c_ulonglong(), C.c_uint()
+    assert L.nmcp_expm1_check(C.byref(bad), C.byref(first)) == 0
     assert bad.value == 0, f"{bad.value} inputs differ, first 0x{first.value:08x}"
