@@ -306,8 +306,11 @@ def fit_leg(steps: int) -> dict:
     """Batched inverse fit: 256 three-layer instances, `steps` Adam steps of
     one forward (with path records) and one backward launch each, against the
     same step done as 256 per-instance launches; per-layer median relative
-    errors at the start and the end, and the Cramer-Rao bound of each
-    parameter from the target's noise (what any fit of this data can reach)."""
+    errors at the start, after the Adam steps and after 8 Levenberg-Marquardt
+    steps on the same posterior, and the Cramer-Rao bound of each parameter
+    for the noise the loss weights by (the target's 1 % noise and the Monte
+    Carlo variance of target and model runs, measured from 16 target batches)
+    and the start-value prior: what an efficient fit of this data reaches."""
     from nmc_amd.inverse_fit import fit
 
     log(f"[bench] inverse fit: 256 instances x {steps} Adam steps")
