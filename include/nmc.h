@@ -251,6 +251,29 @@ int nmc_last_transport_ms(nmc_handle* h, float* ms);
 int nmc_last_launch_count(nmc_handle* h, uint32_t* n);
 
 /*
+ * Per-tissue fluence sensitivities of the voxel walk (no reference
+ * counterpart; the reference's absorbed-weight tally is F3D,
+ * kernels.metal:530,550, normalised to fluence in main.m:481-490).
+ * nmc_mcxyz_sens selects up to NMC_SENS_MAX distinct tissue ids (n_tissues
+ * = 0: none) of a prepared mcxyz handle with the fluence tally; later runs
+ * then also accumulate, for every voxel v and selected tissue t, the
+ * perturbation-MC derivatives of the raw absorbed weight F[v] (the
+ * `fluence` array of nmc_get_results) with respect to mua_t and mus_t: each
+ * deposit weighted by the photon's path length and scatterings in every
+ * tissue before it.  Those runs use their own kernel variant: every deposit
+ * goes to the FP64 array directly (no LDS tile, no records; tuning.mode and
+ * tile are ignored), the photons and F are the same.  The sensitivities are
+ * zeroed here and by nmc_reset_tallies.  For the fluence phi = F / (N vol
+ * mua_k) of a voxel of tissue k: dphi/dmua_t = dF/dmua_t / (N vol mua_k)
+ * - [t = k] phi / mua_k, dphi/dmus_t = dF/dmus_t / (N vol mua_k).
+ */
+#define NMC_SENS_MAX 8
+int nmc_mcxyz_sens(nmc_handle* h, const int32_t* tissue_ids, int n_tissues);
+/* d_mua, d_mus: n_tissues x Nz*Nx*Ny doubles each (voxel order of the
+ * fluence array), the sums of all runs since the selection or reset. */
+int nmc_mcxyz_sens_results(nmc_handle* h, double* d_mua, double* d_mus);
+
+/*
  * Batched mcsub: many media (e.g. wavelengths), each with its own optical
  * properties, Philox stream and photon budget, transported by ONE persistent
  * launch from a device photon queue.  No counterpart in the reference, whose

@@ -52,6 +52,8 @@ def lib():
         "nmc_get_tuning": (C.c_int, [H, C.POINTER(P.Tuning)]),
         "nmc_last_launch_count": (C.c_int, [H, C.POINTER(C.c_uint32)]),
         "nmc_last_transport_ms": (C.c_int, [H, C.POINTER(C.c_float)]),
+        "nmc_mcxyz_sens": (C.c_int, [H, C.POINTER(C.c_int32), C.c_int]),
+        "nmc_mcxyz_sens_results": (C.c_int, [H, C.c_void_p, C.c_void_p]),
         "nmc_prepare_batch": (C.c_int, [H, rp, ext, C.POINTER(P.WorkItem), C.c_int]),
         "nmc_run_batch": (C.c_int, [H]),
         "nmc_run_batch_async": (C.c_int, [H]),
@@ -82,7 +84,7 @@ EXPORTED = ("nmc_ext_config_init", "nmc_validate", "nmc_create", "nmc_prepare", 
             "nmc_run", "nmc_run_async", "nmc_synchronize", "nmc_last_kernel_ms",
             "nmc_get_results", "nmc_reset_tallies", "nmc_destroy", "nmc_last_error",
             "nmc_version", "nmc_set_tuning", "nmc_get_tuning", "nmc_last_launch_count",
-            "nmc_last_transport_ms",
+            "nmc_last_transport_ms", "nmc_mcxyz_sens", "nmc_mcxyz_sens_results",
             "nmc_prepare_batch",
             "nmc_run_batch", "nmc_run_batch_async", "nmc_get_batch_results",
             "nmc_set_graph_mode", "nmc_layered_prepare", "nmc_layered_update", "nmc_layered_run",

@@ -90,13 +90,6 @@ __device__ __forceinline__ void load_layer(LPhoton& p, const LayRow* rows) {
   }
 }
 
-// h(x) = (1 - e^{-x} (1 + x)) / x for x = mua s >= 0, with em = expm1(-x):
-// a series below x = 0.05 (where the direct form cancels), else direct.
-__device__ __forceinline__ float score_growth(float x, float em) {
-  if (x < 0.05f) return x * (0.5f - x * (0.33333334f - x * (0.125f - x * 0.033333335f)));
-  return (-em - x * (1.0f + em)) / x;
-}
-
 // The sensitivity terms of one segment (SENS): D = dep deposited over length
 // s from weight w_pre, x = mua s, em = expm1(-x), before Lcur and Kcur grow.
 __device__ __forceinline__ void sens_segment(LPhoton& p, float dep, float w_pre, float s, float x,

@@ -42,6 +42,10 @@ struct Photon {
   uint64_t pid;
   uint32_t ev, events, b1, b2, b3;
   double vdep;
+  // sensitivity variant: the voxel visit's sums (sp = sum D L_k, sq = sum D
+  // K_k, sza = sum W s e^{-x}, szs = sum W s h(x)) and the tissue's slot
+  float sp, sq, sza, szs;
+  int slot;
 };
 
 __device__ __forceinline__ int clampi(int v, int n) { return v < 0 ? 0 : (v >= n ? n - 1 : v); }
@@ -71,7 +75,16 @@ struct TileBox {
 // Each is a kernel of its own, so the loop carries no per-visit mode test.
 // kFluRecords: deposits, with the visits outside the tile appended as binned
 // records (tuning.mode 2) instead of FP64 atomics.
-enum : int { kFluNone = 0, kFluDeposit = 1, kFluCount = 2, kFluRecords = 3 };
+enum : int { kFluNone = 0, kFluDeposit = 1, kFluCount = 2, kFluRecords = 3, kFluSens = 4 };
+
+// The sensitivity variant's per-lane path length L and scatterings K per
+// tracked tissue slot, in LDS rows [slot][lane]; these point at this lane's
+// column (entry j at [j * stride]).
+struct SensRows {
+  float* L;
+  uint32_t* K;
+  uint32_t stride;
+};
 
 // A workgroup's slice of one bucket's record region, in LDS: the first
 // record index, the capacity and the slots taken so far.
@@ -91,10 +104,40 @@ struct RecOut {
 // (kFluCount: adds 1 per visit instead; kFluRecords: appends a record instead
 // of the global atomic while the bucket's slice has room), and to the lane's
 // absorbed total.
+//
+// kFluSens also folds the visit into the per-voxel sensitivities (no
+// reference counterpart): with D the visit's deposit in tissue k, L_t and K_t
+// the photon's path length and scatterings in tracked tissue t, it adds
+// d/dmua_t = -L_t D and d/dmus_t = (K_t/mus_t - L_t) D for t != k, and the
+// visit's sums -sp + sza and sq/mus_k - sp - szs for t = k (the layered
+// walk's per-segment terms, summed over the segments in the voxel).
 template <int FMODE, typename TileT>
 __device__ __forceinline__ void flush_visit(Photon& p, const XyzArgs& a, const TileBox& tb,
-                                            TileT* tile, const RecOut& ro, double& absorbed) {
+                                            TileT* tile, const RecOut& ro, double& absorbed,
+                                            const SensRows& sr) {
   constexpr bool COUNT = FMODE == kFluCount;
+  if constexpr (FMODE == kFluSens) {
+    if (p.inside && (p.vdep != 0.0 || p.sza != 0.0f)) {
+      const size_t nvox = (size_t)a.Nx * (size_t)a.Ny * (size_t)a.Nz;
+      const gptr<double> out = (gptr<double>)a.sens;
+      for (int j = 0; j < a.sens_n; ++j) {
+        double ga, gs;
+        if (j == p.slot) {
+          ga = -(double)p.sp + (double)p.sza;
+          gs = (double)p.sq * (double)p.inv_mus - (double)p.sp - (double)p.szs;
+        } else {
+          const float Lj = sr.L[j * sr.stride];
+          const uint32_t Kj = sr.K[j * sr.stride];
+          if (p.vdep == 0.0 || (Lj == 0.0f && Kj == 0u)) continue;
+          ga = -(double)Lj * p.vdep;
+          gs = ((double)Kj * (double)a.sens_inv_mus[j] - (double)Lj) * p.vdep;
+        }
+        global_add(out + ((size_t)j * nvox + p.lin), ga);
+        global_add(out + ((size_t)(a.sens_n + j) * nvox + p.lin), gs);
+      }
+    }
+    p.sp = p.sq = p.sza = p.szs = 0.0f;
+  }
   absorbed += p.vdep;
   if (p.inside && (COUNT || p.vdep != 0.0)) {
     double v = COUNT ? 1.0 : p.vdep;
@@ -277,9 +320,14 @@ __device__ __forceinline__ void launch(Photon& p, const XyzArgs& a, const XyzTis
 
 template <bool TWO_TERM, int BLOCK, typename TileT, int FMODE>
 __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
-  // dynamic LDS: the record slices (kFluRecords), then the tile
+  // dynamic LDS: the record slices (kFluRecords), then the tile; kFluSens:
+  // the per-lane sensitivity rows instead (no tile)
   extern __shared__ double tile_mem[];
   constexpr bool RECS = FMODE == kFluRecords;
+  constexpr bool SENS = FMODE == kFluSens;
+  const int n_sens = SENS ? a.sens_n : 0;
+  const SensRows sr{(float*)tile_mem + threadIdx.x, (uint32_t*)tile_mem + n_sens * BLOCK + threadIdx.x,
+                    (uint32_t)BLOCK};
   const int nb = RECS ? a.n_buckets : 0;
   RecSlice* slices = (RecSlice*)tile_mem;
   TileT* tile = (TileT*)(tile_mem + 2 * nb);  // [tile_nz][tile_nx][tile_ny]
@@ -329,6 +377,8 @@ __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
   p.W = 0.0f;
   p.vdep = 0.0;
   p.ix = p.iy = p.iz = 0;
+  p.tis = 0;
+  p.slot = -1;
   p.lin = 0;
   p.inside = false;
   bool finished = false;
@@ -361,6 +411,14 @@ __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
           p.pid = first + pool_next + rank;
           c_phot++;
           launch(p, a, T, lt);
+          if constexpr (SENS) {
+            for (int j = 0; j < n_sens; ++j) {
+              sr.L[j * BLOCK] = 0.0f;
+              sr.K[j * BLOCK] = 0u;
+            }
+            p.sp = p.sq = p.sza = p.szs = 0.0f;
+            p.slot = p.alive ? T[p.tis].slot : -1;
+          }
         }
         pool_next += take;
         idle = ballot(!p.alive && !finished);
@@ -412,8 +470,22 @@ __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
     p.y = fmaf(s, p.uy, p.y);
     p.z = fmaf(s, p.uz, p.z);
     // DROP (kernels.metal:523-533)
-    float em = expm1_neg(-(p.mua * s));
+    const float xa = p.mua * s;
+    float em = expm1_neg(-xa);
     float dep = -(p.W * em);
+    if constexpr (SENS) {
+      // the segment's sensitivity terms, before L of its tissue grows by s
+      if (p.slot >= 0) {
+        float* Lr = &sr.L[p.slot * BLOCK];
+        const float Lv = *Lr;
+        p.sp = p.sp + dep * Lv;
+        p.sq = p.sq + dep * (float)sr.K[p.slot * BLOCK];
+        const float ws = p.W * s;
+        p.sza = p.sza + ws * (1.0f + em);
+        p.szs = p.szs + ws * score_growth(xa, em);
+        *Lr = Lv + s;
+      }
+    }
     p.W = p.W - dep;
     // inside the grid the deposit joins the visit's sum, which the voxel's
     // flush also adds to the lane's absorbed total (no fluence: at once)
@@ -449,7 +521,7 @@ __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
         if (!(p.W > 0.0f)) alive = false;
       } else {
         // voxel face (kernels.metal:641-679)
-        if (FMODE != kFluNone) flush_visit<FMODE>(p, a, tb, tile, ro, absorbed);
+        if (FMODE != kFluNone) flush_visit<FMODE>(p, a, tb, tile, ro, absorbed, sr);
         if (axis == 0) {
           const bool fwd = p.ux > 0.0f;
           p.ix += fwd ? 1 : -1;
@@ -473,6 +545,7 @@ __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
           p.mua = T[p.tis].mua;
           p.mus = T[p.tis].mus;
           p.inv_mus = T[p.tis].inv_mus;
+          if constexpr (SENS) p.slot = T[p.tis].slot;
         }
       }
     }
@@ -490,6 +563,9 @@ __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
       p.ruy = rcp_rn(p.uy);
       p.ruz = rcp_rn(p.uz);
       c_int++;
+      if constexpr (SENS) {
+        if (p.slot >= 0) sr.K[p.slot * BLOCK]++;
+      }
       // ROULETTE (kernels.metal:740-744)
       if (p.W < a.thr) {
         if (u01(p.b3) <= a.chance) {
@@ -513,7 +589,7 @@ __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
       }
     }
     if (!alive) {
-      if (FMODE != kFluNone) flush_visit<FMODE>(p, a, tb, tile, ro, absorbed);
+      if (FMODE != kFluNone) flush_visit<FMODE>(p, a, tb, tile, ro, absorbed, sr);
       p.alive = false;
     }
   }
@@ -680,6 +756,33 @@ int mcxyz_blocks_per_cu(bool two_term, bool tile32, int block, size_t lds_bytes)
     best = m == kFluNone || nb < best ? nb : best;
   }
   return best;
+}
+
+hipError_t launch_mcxyz_sens(const XyzArgs& a, bool two_term, int grid, size_t lds, hipStream_t s) {
+  if (two_term)
+    hipLaunchKernelGGL((mcxyz_kernel<true, kSensBlock, double, kFluSens>), dim3(grid),
+                       dim3(kSensBlock), lds, s, a);
+  else
+    hipLaunchKernelGGL((mcxyz_kernel<false, kSensBlock, double, kFluSens>), dim3(grid),
+                       dim3(kSensBlock), lds, s, a);
+  return hipGetLastError();
+}
+
+hipError_t mcxyz_sens_configure(bool two_term, size_t lds) {
+  return hipFuncSetAttribute(two_term ? (const void*)mcxyz_kernel<true, kSensBlock, double, kFluSens>
+                                      : (const void*)mcxyz_kernel<false, kSensBlock, double, kFluSens>,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+}
+
+int mcxyz_sens_blocks_per_cu(bool two_term, size_t lds) {
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          &nb,
+          two_term ? (const void*)mcxyz_kernel<true, kSensBlock, double, kFluSens>
+                   : (const void*)mcxyz_kernel<false, kSensBlock, double, kFluSens>,
+          kSensBlock, lds) != hipSuccess)
+    return 0;
+  return nb;
 }
 
 hipError_t rec_reduce_configure() {

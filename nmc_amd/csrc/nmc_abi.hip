@@ -170,6 +170,12 @@ struct nmc_handle {
   double* dF = nullptr;
   double* dTilePart = nullptr;
   XyzRecords recs;             // binned deposit records (tuning.mode 2)
+  // per-tissue sensitivities (nmc_mcxyz_sens): [2][sens_n][nvox]
+  double* dSens = nullptr;
+  int sens_n = 0;
+  int32_t sens_ids[NMC_SENS_MAX] = {};
+  size_t sens_lds = 0;
+  int sens_grid = 0;
   double* dRyx = nullptr;
   uint32_t* dDet = nullptr;
   size_t xyz_lds = 0;
@@ -255,6 +261,8 @@ void free_buffers(nmc_handle* h) {
   free_ptr(h->dAux);
   free_ptr(h->dF);
   free_ptr(h->dTilePart);
+  free_ptr(h->dSens);
+  h->sens_n = 0;
   free_ptr(h->dRyx);
   free_ptr(h->dDet);
   free_ptr(h->dItems);
@@ -482,6 +490,7 @@ void xyz_tables(const RunParams& rp, const TissueParams& tp, int sim, XyzTissue*
     tis[t].g = tp.gv[t];
     tis[t].hg = hg_prepare(tp.gv[t]);
     if (sim == NMC_SIM_MCXYZ_TT) tt_prepare(tp.gf[t], tp.af[t], tp.gb[t], tp.ab[t], tp.CC[t], &tis[t].tt);
+    tis[t].slot = -1;
   }
   std::memset(aux, 0, sizeof(*aux));
   aux->mcflag = rp.mcflag;
@@ -742,6 +751,8 @@ int resolve_sub_tuning(nmc_handle* h) {
 
 
 // Enqueues one run (no events, no capture handling).
+int enqueue_xyz_sens(nmc_handle* h, XyzArgs a, uint64_t n, hipStream_t s, hipEvent_t k0, hipEvent_t k1);
+
 int enqueue_run(nmc_handle* h, uint64_t n) {
   hipStream_t s = h->stream;
   if (h->sim == NMC_SIM_MCSUB) {
@@ -806,10 +817,35 @@ int enqueue_run(nmc_handle* h, uint64_t n) {
     L.n_buckets = h->recs.n_buckets;
     // transport-kernel events only outside graph capture
     hipEvent_t k0 = h->graph_mode ? nullptr : h->k0, k1 = h->graph_mode ? nullptr : h->k1;
-    NMC_TRY(enqueue_xyz(a, h->sim == NMC_SIM_MCXYZ_TT, L, n, s, &launches, &h->recs, k0, k1));
+    if (h->sens_n > 0) {
+      NMC_TRY(enqueue_xyz_sens(h, a, n, s, k0, k1));
+      launches = 1;
+    } else {
+      NMC_TRY(enqueue_xyz(a, h->sim == NMC_SIM_MCXYZ_TT, L, n, s, &launches, &h->recs, k0, k1));
+    }
     h->k_timed = k0 != nullptr;
     h->launches += launches;
   }
+  return ok();
+}
+
+// A run of the sensitivity variant: one launch of n photons, every deposit
+// to the FP64 arrays (no tile, no records).
+int enqueue_xyz_sens(nmc_handle* h, XyzArgs a, uint64_t n, hipStream_t s, hipEvent_t k0, hipEvent_t k1) {
+  a.tile_x0 = a.tile_y0 = a.tile_z0 = 0;
+  a.tile_nx = a.tile_ny = a.tile_nz = 0;
+  a.n_buckets = 0;
+  a.refill_min = (uint32_t)h->tune.refill_min;
+  a.sens = h->dSens;
+  a.sens_n = h->sens_n;
+  for (int j = 0; j < h->sens_n; ++j) a.sens_inv_mus[j] = inv_pos(h->tp.musv[h->sens_ids[j]]);
+  a.n = n;
+  a.chunk = pick_chunk(n, h->sens_grid * (kSensBlock / 64), h->tune.chunk);
+  HIP_TRY(launch_state_begin(a.state, s));
+  if (k0) HIP_TRY(hipEventRecord(k0, s));
+  HIP_TRY(launch_mcxyz_sens(a, h->sim == NMC_SIM_MCXYZ_TT, h->sens_grid, h->sens_lds, s));
+  if (k1) HIP_TRY(hipEventRecord(k1, s));
+  HIP_TRY(launch_state_advance(a.state, n, s));
   return ok();
 }
 
@@ -1145,6 +1181,8 @@ int nmc_reset_tallies(nmc_handle* h) {
   } else {
     int64_t n = nvox(&h->rp), n2 = (int64_t)h->rp.Nx * h->rp.Ny;
     if (h->dF) HIP_TRY(hipMemsetAsync(h->dF, 0, n * sizeof(double), h->stream));
+    if (h->dSens)
+      HIP_TRY(hipMemsetAsync(h->dSens, 0, 2 * (size_t)h->sens_n * n * sizeof(double), h->stream));
     HIP_TRY(hipMemsetAsync(h->dRyx, 0, n2 * sizeof(double), h->stream));
     HIP_TRY(hipMemsetAsync(h->dDet, 0, n2 * sizeof(uint32_t), h->stream));
   }
@@ -1194,6 +1232,75 @@ int nmc_last_transport_ms(nmc_handle* h, float* ms) {
   HIP_TRY(hipSetDevice(h->device));
   HIP_TRY(hipEventSynchronize(h->k1));
   HIP_TRY(hipEventElapsedTime(ms, h->k0, h->k1));
+  return ok();
+}
+
+int nmc_mcxyz_sens(nmc_handle* h, const int32_t* tissue_ids, int n_tissues) {
+  if (!h) return fail(NMC_ERR_ARG, "null handle");
+  if (h->sim != NMC_SIM_MCXYZ && h->sim != NMC_SIM_MCXYZ_TT)
+    return fail(NMC_ERR_UNSUPPORTED, "sensitivities are computed for mcxyz handles");
+  if (!h->prepared) return fail(NMC_ERR_STATE, "nmc_prepare has not succeeded");
+  if (!h->dF) return fail(NMC_ERR_STATE, "sensitivities need the fluence tally (no NMC_FLAG_NO_FLUENCE)");
+  if (n_tissues < 0 || n_tissues > NMC_SENS_MAX)
+    return fail(NMC_ERR_ARG, "n_tissues %d not in 0..%d", n_tissues, NMC_SENS_MAX);
+  if (n_tissues > 0 && !tissue_ids) return fail(NMC_ERR_ARG, "null tissue_ids");
+  for (int j = 0; j < n_tissues; ++j) {
+    if (tissue_ids[j] < 0 || tissue_ids[j] >= NMC_NTISS)
+      return fail(NMC_ERR_ARG, "tissue id %d not in 0..%d", tissue_ids[j], NMC_NTISS - 1);
+    for (int i = 0; i < j; ++i)
+      if (tissue_ids[i] == tissue_ids[j]) return fail(NMC_ERR_ARG, "tissue id %d given twice", tissue_ids[j]);
+  }
+  HIP_TRY(hipSetDevice(h->device));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  const bool tt = h->sim == NMC_SIM_MCXYZ_TT;
+  const size_t lds = (size_t)2 * n_tissues * kSensBlock * sizeof(uint32_t);
+  int grid = 0;
+  double* buf = nullptr;
+  if (n_tissues > 0) {
+    HIP_TRY(mcxyz_sens_configure(tt, lds));
+    const int occ = mcxyz_sens_blocks_per_cu(tt, lds);
+    if (occ <= 0) return fail(NMC_ERR_DEVICE, "the sensitivity kernel does not fit on a CU");
+    grid = h->num_cu * occ;
+    const size_t bytes = 2 * (size_t)n_tissues * (size_t)nvox(&h->rp) * sizeof(double);
+    hipError_t e = hipMalloc(&buf, bytes);
+    if (e != hipSuccess)
+      return fail(e == hipErrorOutOfMemory ? NMC_ERR_NOMEM : NMC_ERR_DEVICE, "sensitivity arrays: %s",
+                  hipGetErrorString(e));
+    e = hipMemset(buf, 0, bytes);
+    if (e != hipSuccess) {
+      hipFree(buf);
+      return fail(NMC_ERR_DEVICE, "sensitivity arrays: %s", hipGetErrorString(e));
+    }
+  }
+  // the tissue table's slots
+  std::vector<XyzTissue> tis(NMC_NTISS);
+  XyzAux aux{};
+  xyz_tables(h->rp, h->tp, h->sim, tis.data(), &aux);
+  for (int j = 0; j < n_tissues; ++j) tis[tissue_ids[j]].slot = j;
+  hipError_t e = hipMemcpy(h->dTis, tis.data(), NMC_NTISS * sizeof(XyzTissue), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    if (buf) hipFree(buf);
+    return fail(NMC_ERR_DEVICE, "tissue table: %s", hipGetErrorString(e));
+  }
+  // commit: the cached graph runs the previous variant
+  drop_graph(h);
+  free_ptr(h->dSens);
+  h->dSens = buf;
+  h->sens_n = n_tissues;
+  for (int j = 0; j < n_tissues; ++j) h->sens_ids[j] = tissue_ids[j];
+  h->sens_lds = lds;
+  h->sens_grid = grid;
+  return ok();
+}
+
+int nmc_mcxyz_sens_results(nmc_handle* h, double* d_mua, double* d_mus) {
+  if (!h) return fail(NMC_ERR_ARG, "null handle");
+  if (!h->prepared || !h->dSens) return fail(NMC_ERR_STATE, "no tissues selected by nmc_mcxyz_sens");
+  HIP_TRY(hipSetDevice(h->device));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  const size_t per = (size_t)h->sens_n * (size_t)nvox(&h->rp);
+  if (d_mua) HIP_TRY(hipMemcpy(d_mua, h->dSens, per * sizeof(double), hipMemcpyDeviceToHost));
+  if (d_mus) HIP_TRY(hipMemcpy(d_mus, h->dSens + per, per * sizeof(double), hipMemcpyDeviceToHost));
   return ok();
 }
 

@@ -211,6 +211,14 @@ __device__ __forceinline__ float expm1_neg(float x) {
   return small ? p : (x < -87.0f ? -1.0f : big);
 }
 
+// h(x) = (1 - e^{-x} (1 + x)) / x for x = mua s >= 0, with em = expm1(-x):
+// a series below x = 0.05 (where the direct form cancels), else direct.  The
+// growth of a segment's mus score in the perturbation-MC sensitivities.
+__device__ __forceinline__ float score_growth(float x, float em) {
+  if (x < 0.05f) return x * (0.5f - x * (0.33333334f - x * (0.125f - x * 0.033333335f)));
+  return (-em - x * (1.0f + em)) / x;
+}
+
 // sin(2 pi u), cos(2 pi u), u in [0,1]: exact quarter-turn reduction, Taylor
 // polynomials of sin (to f^9) and cos (to f^10) of 2 pi f, |f| <= 1/8.
 __device__ __forceinline__ void sincos2pi(float u, float* s, float* c) {

@@ -27,6 +27,7 @@ struct XyzTissue {
   float mua, mus, inv_mus, g;       // inv_mus = 1/mus (FLT_MAX for mus = 0)
   HGCoef hg;
   TTCoef tt;
+  int32_t slot;                     // sensitivity slot (nmc_mcxyz_sens), -1: none
 };
 
 // Photon-queue state read by the kernels from device memory, so that a
@@ -82,6 +83,12 @@ struct XyzArgs {
   const uint32_t* rec_cap;              // [n_buckets] records per workgroup slice
   uint2* rec;                           // {voxel index within the bucket, weight bits (f32)}
   uint32_t* rec_count;                  // [n_buckets][grid] slots taken (may exceed rec_cap)
+  // per-tissue sensitivities (nmc_mcxyz_sens): sens[0 or 1][slot][voxel]
+  // (d/dmua, d/dmus) for sens_n tracked tissues, whose 1/mus are given; null:
+  // none.  Runs with them use their own kernel variant (no tile, no records).
+  double* sens;
+  int sens_n;
+  float sens_inv_mus[NMC_SENS_MAX];
 };
 
 // Buckets of the binned deposit records: 2^14 voxels, whose FP64 sums fill
@@ -240,6 +247,11 @@ struct XyzRecords {
   uint64_t n_rec = 0;                   // records the buffer holds
 };
 
+// The sensitivity variant: 1024-lane workgroups, lds = its per-lane rows.
+hipError_t launch_mcxyz_sens(const XyzArgs& a, bool two_term, int grid, size_t lds, hipStream_t s);
+hipError_t mcxyz_sens_configure(bool two_term, size_t lds);
+int mcxyz_sens_blocks_per_cu(bool two_term, size_t lds);
+constexpr int kSensBlock = 1024;
 hipError_t launch_mcxyz(const XyzArgs& a, bool two_term, bool tile32, int block, int grid,
                         size_t lds_bytes, hipStream_t s);
 hipError_t mcxyz_configure(bool two_term, bool tile32, int block, size_t lds_bytes);

@@ -29,12 +29,14 @@ class MetalMC:
         check(lib().nmc_create(int(kernel_type), int(device), C.byref(self._h)))
         self.kernel_type = int(kernel_type)
         self.rp = self.tp = self.ext = None
+        self._sens_ids = np.zeros(0, np.int32)
 
     # -- configuration -------------------------------------------------
     def prepare_data(self, run_params: P.RunParams, tiss_params: P.TissueParams,
                      v: np.ndarray | None = None, ext: P.ExtConfig | None = None) -> None:
         self.rp, self.tp = run_params, tiss_params
         self.ext = ext if ext is not None else P.ExtConfig.default()
+        self._sens_ids = np.zeros(0, np.int32)   # a new preparation selects none
         vp = C.POINTER(C.c_uint8)()
         if v is not None:
             self._v = np.ascontiguousarray(v, np.uint8)
@@ -84,6 +86,24 @@ class MetalMC:
 
     def reset_tallies(self) -> None:
         check(lib().nmc_reset_tallies(self._h))
+
+    # -- per-tissue fluence sensitivities (mcxyz) ----------------------------
+    def set_sensitivity(self, tissue_ids) -> None:
+        """Later runs also accumulate d(absorbed weight per voxel)/d(mua, mus)
+        of these tissue ids (at most NMC_SENS_MAX; empty: off)."""
+        ids = np.ascontiguousarray(list(tissue_ids), np.int32)
+        self._sens_ids = ids
+        check(lib().nmc_mcxyz_sens(self._h, ids.ctypes.data_as(C.POINTER(C.c_int32)), len(ids)))
+
+    def sensitivity_results(self):
+        """(d_mua, d_mus), each [n_tissues, Nz, Nx, Ny]: derivatives of the raw
+        fluence tally (get_compute_results().fluence) since the selection or
+        the last reset_tallies."""
+        shape = (len(self._sens_ids), self.rp.Nz, self.rp.Nx, self.rp.Ny)
+        da = np.zeros(shape)
+        ds = np.zeros(shape)
+        check(lib().nmc_mcxyz_sens_results(self._h, da.ctypes.data, ds.ctypes.data))
+        return da, ds
 
     # -- launch tuning / graphs ---------------------------------------------
     def set_tuning(self, tuning: P.Tuning) -> None:

@@ -56,7 +56,9 @@
  *   3  mcsub drops its weight at the position before the hop
  *   4  mcsub bins counted from 1 (the first bin left empty)
  *   5  the absorbed-weight sensitivity to mus drops the score's growth
- *      within a segment (the W s h(x) term)                             */
+ *      within a segment (the W s h(x) term)
+ *   6  the voxel sensitivity to mua drops the own-tissue term W s e^{-x}
+ *   7  the voxel sensitivity to mus drops the own-tissue W s h(x) term  */
 #ifndef NMCO_MUTANT
 #define NMCO_MUTANT 0
 #endif
@@ -458,9 +460,48 @@ typedef struct {
     float mua[NMC_NTISS], mus[NMC_NTISS], inv_mus[NMC_NTISS];
     hg_coef hg[NMC_NTISS];
     tt_coef tt[NMC_NTISS];
+    /* sensitivities: tracked tissues ids[0..n_sens), slot[id] (-1: untracked) */
+    int n_sens, sens_ids[NMC_SENS_MAX], slot[NMC_NTISS];
 } xyz_ctx;
 
-typedef struct { double* F; double* ryx; uint32_t* det; nmc_totals t; } xyz_acc;
+typedef struct { double* F; double* ryx; uint32_t* det; nmc_totals t; double* sens; } xyz_acc;
+
+static float score_growth(float x, float em);
+
+/* Per-voxel absorbed-weight sensitivities (no reference counterpart; the
+ * reference's absorbed tally is F3D, kernels.metal:530,550).  The weight a
+ * voxel visit in tissue k deposits is the path integral of mua_k W along its
+ * segments; as for the layered walk (lay_visit / sens_fold below), with L_t,
+ * K_t the photon's path length and scatterings in tissue t before a point, a
+ * segment of length s entered with weight W adds
+ *   dD/dmua_t = -L_t D + [t = k] W s e^{-x},
+ *   dD/dmus_t = (K_t/mus_t - L_t) D - [t = k] W s h(x),  x = mua_k s,
+ * and during the visit only L_k and K_k change, so the visit keeps the sums
+ * sp = sum D L_k, sq = sum D K_k, sza = sum W s e^{-x}, szs = sum W s h(x)
+ * and the deposit total D.  sens[0][j][v], sens[1][j][v]: d/dmua and d/dmus
+ * of tissue ids[j] for voxel v. */
+typedef struct { double D; float sp, sq, sza, szs; } xyz_visit;
+
+static void xyz_sens_fold(const xyz_ctx* c, int k, const float* Ls, const uint32_t* Ks,
+                          const xyz_visit* v, int64_t vox, int64_t nv, double* sens)
+{
+    if (v->D == 0.0 && v->sza == 0.0f) return;
+    const int sk = c->slot[k];
+    for (int j = 0; j < c->n_sens; ++j) {
+        double ga, gs;
+        if (j == sk) {
+            ga = -(double)v->sp + (NMCO_MUTANT == 6 ? 0.0 : (double)v->sza);
+            gs = (double)v->sq * (double)c->inv_mus[k] - (double)v->sp -
+                 (NMCO_MUTANT == 7 ? 0.0 : (double)v->szs);
+        } else {
+            if (v->D == 0.0 || (Ls[j] == 0.0f && Ks[j] == 0u)) continue;
+            ga = -(double)Ls[j] * v->D;
+            gs = ((double)Ks[j] * (double)c->inv_mus[c->sens_ids[j]] - (double)Ls[j]) * v->D;
+        }
+        sens[(int64_t)j * nv + vox] += ga;
+        sens[((int64_t)c->n_sens + j) * nv + vox] += gs;
+    }
+}
 
 static inline float xface(int i, float h, float d) { return ((float)i - h) * d; }
 
@@ -553,11 +594,22 @@ static void xyz_photon(const xyz_ctx* c, uint64_t pid, xyz_acc* a)
     /* Deposits of one voxel visit are summed in double and added to F when
      * the photon leaves the voxel or ends. */
     double vdep = 0.0;
+    /* sensitivities: path length and scatterings per tracked tissue, and the
+     * current visit's sums */
+    float Ls[NMC_SENS_MAX] = {0};
+    uint32_t Ks[NMC_SENS_MAX] = {0};
+    xyz_visit vs = {0.0, 0.0f, 0.0f, 0.0f, 0.0f};
+    const int64_t nvox = (int64_t)c->Nx * c->Ny * c->Nz;
 #define FLUSH()                                                                        \
     do {                                                                               \
         if (inside && a->F && (c->count_visits || vdep != 0.0))                        \
             a->F[VIDX(ix, iy, iz)] += c->count_visits ? 1.0 : vdep;                     \
+        if (a->sens && inside) {                                                       \
+            vs.D = vdep;                                                               \
+            xyz_sens_fold(c, tis, Ls, Ks, &vs, VIDX(ix, iy, iz), nvox, a->sens);        \
+        }                                                                              \
         vdep = 0.0;                                                                    \
+        vs.sp = vs.sq = vs.sza = vs.szs = 0.0f;                                        \
     } while (0)
 
     for (;;) {
@@ -593,8 +645,18 @@ static void xyz_photon(const xyz_ctx* c, uint64_t pid, xyz_acc* a)
         y = fmaf(s, uy, y);
         z = fmaf(s, uz, z);
         /* DROP (kernels.metal:523-533) */
-        float em = nmco_expm1f(-(mua * s));
+        const float xa = mua * s;
+        float em = nmco_expm1f(-xa);
         float dep = -(W * em);
+        if (a->sens && c->slot[tis] >= 0) {
+            const int sl = c->slot[tis];
+            vs.sp = vs.sp + dep * Ls[sl];
+            vs.sq = vs.sq + dep * (float)Ks[sl];
+            const float ws = W * s;
+            vs.sza = vs.sza + ws * (1.0f + em);
+            vs.szs = vs.szs + ws * score_growth(xa, em);
+            Ls[sl] = Ls[sl] + s;
+        }
         W = W - dep;
         if (inside) {
             vdep += (double)dep;
@@ -669,6 +731,7 @@ static void xyz_photon(const xyz_ctx* c, uint64_t pid, xyz_acc* a)
         nmco_spin(ct, u01(b[2]), &ux, &uy, &uz);
         rux = 1.0f / ux; ruy = 1.0f / uy; ruz = 1.0f / uz;
         t->interactions++;
+        if (a->sens && c->slot[tis] >= 0) Ks[c->slot[tis]]++;
         /* ROULETTE (kernels.metal:740-744) */
         if (!roulette(&W, u01(b[3]), c->thr, c->chance, t)) { FLUSH(); return; }
         ev++;
@@ -705,6 +768,7 @@ static int xyz_setup(const RunParams* rp, const TissueParams* tp, const uint8_t*
     c->key[1] = (uint32_t)(e->seed >> 32) + e->stream_id;
     c->V = v;
     for (int i = 0; i < NMC_NTISS; ++i) {
+        c->slot[i] = -1;
         c->mua[i] = tp->muav[i]; c->mus[i] = tp->musv[i]; c->inv_mus[i] = inv_pos(tp->musv[i]);
         hg_prepare(tp->gv[i], &c->hg[i]);
         if (c->two_term) tt_prepare(tp->gf[i], tp->af[i], tp->gb[i], tp->ab[i], tp->CC[i], &c->tt[i]);
@@ -719,18 +783,37 @@ int nmco_mcxyz(const RunParams* rp, const TissueParams* tp, const uint8_t* v,
                const nmc_ext_config* ext, uint64_t first, uint64_t n, int n_threads,
                nmc_results* out)
 {
+    return nmco_mcxyz_sens(rp, tp, v, ext, first, n, n_threads, out, NULL, 0, NULL);
+}
+
+int nmco_mcxyz_sens(const RunParams* rp, const TissueParams* tp, const uint8_t* v,
+                    const nmc_ext_config* ext, uint64_t first, uint64_t n, int n_threads,
+                    nmc_results* out, const int32_t* ids, int n_ids, double* sens)
+{
     nmc_ext_config e;
     ext_defaults(ext, &e);
+    if (n_ids < 0 || n_ids > NMC_SENS_MAX || (n_ids > 0 && (!ids || !sens))) return NMC_ERR_ARG;
     xyz_ctx* c = (xyz_ctx*)malloc(sizeof(xyz_ctx));
     if (!c) return NMC_ERR_NOMEM;
     int st = xyz_setup(rp, tp, v, &e, c);
     if (st != NMC_OK) { free(c); return st; }
+    c->n_sens = n_ids;
+    for (int j = 0; j < n_ids; ++j) {
+        if (ids[j] < 0 || ids[j] >= NMC_NTISS || c->slot[ids[j]] >= 0) { free(c); return NMC_ERR_ARG; }
+        c->sens_ids[j] = ids[j];
+        c->slot[ids[j]] = j;
+    }
     int64_t nv = (int64_t)c->Nx * c->Ny * c->Nz, n2d = (int64_t)c->Nx * c->Ny;
+    const int64_t ns = 2 * (int64_t)n_ids * nv;
     int nt = n_threads > 0 ? n_threads : nmco_max_threads();
     if ((uint64_t)nt > n) nt = n > 0 ? (int)n : 1;
     xyz_acc* acc = (xyz_acc*)calloc((size_t)nt, sizeof(xyz_acc));
     int fail = 0;
     for (int i = 0; i < nt; ++i) {
+        if (n_ids > 0) {
+            acc[i].sens = i == 0 ? sens : (double*)calloc((size_t)ns, sizeof(double));
+            fail |= acc[i].sens == NULL;
+        }
         if (out->fluence && !(e.flags & NMC_FLAG_NO_FLUENCE)) {
             acc[i].F = i == 0 ? out->fluence : (double*)calloc((size_t)nv, sizeof(double));
             fail |= acc[i].F == NULL;
@@ -753,6 +836,7 @@ int nmco_mcxyz(const RunParams* rp, const TissueParams* tp, const uint8_t* v,
             if (acc[i].F) { for (int64_t j = 0; j < nv; ++j) out->fluence[j] += acc[i].F[j]; free(acc[i].F); }
             if (acc[i].ryx) { for (int64_t j = 0; j < n2d; ++j) out->ryx[j] += acc[i].ryx[j]; free(acc[i].ryx); }
             if (acc[i].det) { for (int64_t j = 0; j < n2d; ++j) out->det2d[j] += acc[i].det[j]; free(acc[i].det); }
+            if (acc[i].sens) { for (int64_t j = 0; j < ns; ++j) sens[j] += acc[i].sens[j]; free(acc[i].sens); }
         }
         totals_add(&out->totals, &acc[i].t);
     }

@@ -42,6 +42,13 @@ void nmco_sincos2pi_n(const float* u, float* s, float* c, int64_t n);
 int nmco_mcxyz(const RunParams* rp, const TissueParams* tp, const uint8_t* v,
                const nmc_ext_config* ext, uint64_t first, uint64_t n, int n_threads,
                nmc_results* out);
+/* nmco_mcxyz plus the per-voxel sensitivities of the absorbed weight (the
+ * fluence array) to mua and mus of tissues ids[0..n_ids) (n_ids <=
+ * NMC_SENS_MAX, distinct ids): sens[0][j][voxel] += d/dmua, sens[1][j][voxel]
+ * += d/dmus of tissue ids[j] (2 n_ids nvox doubles, zeroed by the caller). */
+int nmco_mcxyz_sens(const RunParams* rp, const TissueParams* tp, const uint8_t* v,
+                    const nmc_ext_config* ext, uint64_t first, uint64_t n, int n_threads,
+                    nmc_results* out, const int32_t* ids, int n_ids, double* sens);
 int nmco_mcsub(const RunParams* rp, const TissueParams* tp, const nmc_ext_config* ext,
                uint64_t first, uint64_t n, int n_threads, nmc_results* out);
 

@@ -45,6 +45,10 @@ def _bind(L):
     L.nmco_mcxyz.argtypes = [C.POINTER(P.RunParams), C.POINTER(P.TissueParams),
                              C.POINTER(C.c_uint8), C.POINTER(P.ExtConfig), C.c_uint64,
                              C.c_uint64, C.c_int, C.POINTER(P.Results)]
+    L.nmco_mcxyz_sens.argtypes = [C.POINTER(P.RunParams), C.POINTER(P.TissueParams),
+                                  C.POINTER(C.c_uint8), C.POINTER(P.ExtConfig), C.c_uint64,
+                                  C.c_uint64, C.c_int, C.POINTER(P.Results),
+                                  C.POINTER(C.c_int32), C.c_int, C.c_void_p]
     L.nmco_mcsub.argtypes = [C.POINTER(P.RunParams), C.POINTER(P.TissueParams),
                              C.POINTER(P.ExtConfig), C.c_uint64, C.c_uint64, C.c_int,
                              C.POINTER(P.Results)]
@@ -78,7 +82,9 @@ MUTANTS = {1: "flat beam radius R*xi instead of R*sqrt(xi)",
            2: "remaining step keeps its physical length across a tissue/layer change",
            3: "mcsub drops its weight at the pre-hop position",
            4: "mcsub bins counted from 1",
-           5: "absorbed sensitivity to mus without the within-segment score term"}
+           5: "absorbed sensitivity to mus without the within-segment score term",
+           6: "voxel sensitivity to mua without the own-tissue W s e^-x term",
+           7: "voxel sensitivity to mus without the own-tissue W s h(x) term"}
 _mutants: dict = {}
 
 
@@ -172,6 +178,27 @@ def run(w: Workload, n_photons: int | None = None, first: int = 0, threads: int 
         raise RuntimeError(f"oracle returned status {st}")
     t.seconds = L.nmco_last_seconds()
     return t
+
+
+def run_sens(w: Workload, ids, n_photons: int | None = None, first: int = 0, threads: int = 0,
+             L=None):
+    """mcxyz run with per-voxel sensitivities to mua and mus of tissues `ids`:
+    (tallies, d_mua [len(ids), Nz, Nx, Ny], d_mus [same])."""
+    n = w.n_photons if n_photons is None else n_photons
+    t = P.TallyArrays(w.rp, w.ext, w.sim, fluence=True)
+    L = L or lib()
+    ids_a = np.ascontiguousarray(ids, np.int32)
+    shape = (w.rp.Nz, w.rp.Nx, w.rp.Ny)
+    sens = np.zeros((2, len(ids_a)) + shape)
+    v = np.ascontiguousarray(w.v, np.uint8)
+    st = L.nmco_mcxyz_sens(C.byref(w.rp), C.byref(w.tp), v.ctypes.data_as(C.POINTER(C.c_uint8)),
+                           C.byref(w.ext), first, n, threads, C.byref(t.res),
+                           ids_a.ctypes.data_as(C.POINTER(C.c_int32)), len(ids_a),
+                           sens.ctypes.data)
+    if st != 0:
+        raise RuntimeError(f"oracle returned status {st}")
+    t.seconds = L.nmco_last_seconds()
+    return t, sens[0], sens[1]
 
 
 def max_threads() -> int:
