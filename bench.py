@@ -277,6 +277,33 @@ def gradient_leg(n_photons: int) -> dict:
             "grad_photons_per_s": n_photons / (fwd_rec + bwd) * 1e3}
 
 
+def voxel_sens_leg(n_photons: int) -> dict:
+    """Per-tissue fluence sensitivities on the skin-vessel grid: a run with
+    d(absorbed weight per voxel)/d(mua, mus) of all 4 tissues against the
+    same photons' plain run (FP64 tile, the variant whose deposits the
+    sensitivity kernel shares), transport kernel times on the library
+    stream."""
+    from nmc_amd import params as P
+    from nmc_amd import workloads as W
+    from nmc_amd.metal_mc import MetalMC
+
+    w = W.skin_vessel(n_photons)
+    log(f"[bench] voxel sensitivities: skin-vessel, 4 tissues, {n_photons} photons")
+    with MetalMC(w.sim, 0) as mc:
+        mc.prepare_data(w.rp, w.tp, w.v, w.ext)
+        mc.set_tuning(P.Tuning.make(mode=0))
+        mc.set_num_runs_seed(w.ext.seed, 0)
+        mc.send_compute_command(n_photons)
+        mc.send_compute_command(n_photons)
+        plain = mc.last_transport_ms()
+        mc.set_sensitivity([1, 2, 3, 4])
+        mc.send_compute_command(n_photons)
+        mc.send_compute_command(n_photons)
+        sens = mc.last_transport_ms()
+    return {"photons": n_photons, "tissues": 4, "plain_fp64_tile_ms": plain, "sens_ms": sens,
+            "sens_over_plain": sens / plain, "sens_photons_per_s": n_photons / sens * 1e3}
+
+
 def gradient_cpu_baseline(target_s: float) -> dict:
     """The oracle's forward with path records plus its pMC backward on the
     5-layer stack, on this host's cores, over a sample of about target_s."""
@@ -329,9 +356,10 @@ def main() -> int:
     ap.add_argument("--tile", choices=tuple(TILE_MODES), default="rec",
                     help="deposit scheme of the timed steps: rec (FP32 LDS tile + binned "
                          "records), fp32 (FP32 tile + FP64 atomics), fp64 (FP64 tile + atomics)")
-    ap.add_argument("--legs", default="tile,sweep,grad,fit",
+    ap.add_argument("--legs", default="tile,sweep,grad,sens,fit",
                     help="extra measurements: tile (the other deposit schemes), sweep "
-                         "(batched vs plain), grad (pMC cost), fit (batched inverse fit)")
+                         "(batched vs plain), grad (pMC cost), sens (per-tissue voxel "
+                         "sensitivities), fit (batched inverse fit)")
     ap.add_argument("--fit-steps", type=int, default=50)
     ap.add_argument("--sweep-scale", type=float, default=1.0, help="scale of the sweep budgets")
     ap.add_argument("--grad-photons", type=float, default=1e7)
@@ -496,6 +524,8 @@ def main() -> int:
         out["gradient"] = gradient_leg(int(args.grad_photons))
         if not args.no_cpu_baseline:
             out["gradient"]["cpu_baseline"] = gradient_cpu_baseline(args.cpu_seconds / 2)
+    if "sens" in legs:
+        out["voxel_sensitivities"] = voxel_sens_leg(int(args.grad_photons))
     if "fit" in legs:
         out["inverse_fit"] = fit_leg(args.fit_steps)
     print(json.dumps(out), flush=True)

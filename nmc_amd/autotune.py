@@ -5,8 +5,8 @@ For an mcxyz workload it
      of voxel visits (= fluence flushes) per voxel;
   2. searches, with a 3-D summed-area table, the boxes of several aspect ratios
      that fit the LDS tile budget and cover the most visits;
-  3. times short trial launches over (tile element FP64 or FP32 x tile box x
-     block size x refill batch)
+  3. times short trial launches over (deposit scheme: FP64 tile, FP32 tile,
+     FP32 tile + binned records x tile box x block size x refill batch)
      and keeps the fastest.
 Results are cached in a JSON file keyed by a hash of the workload, so a
 repeated configuration is tuned once.  For a batched mcsub launch
@@ -128,7 +128,7 @@ def _time(mc: MetalMC, n: int, seed: int) -> float:
 
 def autotune_mcxyz(w, n_trial: int = 20_000_000, device: int = 0, use_cache: bool = True,
                    verbose: bool = False) -> P.Tuning:
-    key = workload_key(w, "mcxyz-v2")
+    key = workload_key(w, "mcxyz-v3")
     if use_cache:
         hit = _load_cache().get(key)
         if hit:
@@ -137,7 +137,8 @@ def autotune_mcxyz(w, n_trial: int = 20_000_000, device: int = 0, use_cache: boo
     with MetalMC(w.sim, device) as mc:
         mc.prepare_data(w.rp, w.tp, w.v, w.ext)
         trials = []
-        for mode in (0, 1):     # FP64 tile, FP32 tile (twice the voxels)
+        # FP64 tile, FP32 tile (twice the voxels), FP32 tile + binned records
+        for mode in (0, 1, 2):
             mc.set_tuning(P.Tuning.make(mode=mode))
             default = mc.get_tuning()
             budget = default.tile[3] * default.tile[4] * default.tile[5]
