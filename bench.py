@@ -421,6 +421,16 @@ def main() -> int:
     res = mc.get_compute_results(fluence=False)
     tot = res.totals
     value = n * args.steps / wall
+    # the tally download a caller does after a step (fluence 200^3 FP64,
+    # R(x,y), detection counts, totals), timed apart from the steps above
+    rb = []
+    for _ in range(3):
+        r0 = time.perf_counter()
+        mc.get_compute_results()
+        rb.append(time.perf_counter() - r0)
+    readback_s = min(rb)
+    nv, n2 = w.rp.Nx * w.rp.Ny * w.rp.Nz, w.rp.Nx * w.rp.Ny
+    readback_bytes = nv * 8 + n2 * 12
     ksec = sum(kern_ms) * 1e-3
     avg_ms = sum(kern_ms) / len(kern_ms)
     avg_run_ms = sum(run_ms) / len(run_ms)
@@ -488,6 +498,10 @@ def main() -> int:
                                 0: "FP64"}[eff.mode],
                    "tile_box": list(eff.tile)},
         "roofline": roof,
+        "readback": {"ms": readback_s * 1e3, "bytes": readback_bytes,
+                     "GB_per_s": readback_bytes / readback_s / 1e9,
+                     "photons_per_s_with_readback_every_step":
+                         n / (wall / args.steps + readback_s)},
         "rd": rd,
         "energy_balance": (tot["rd"] + tot["rd_outside"] + tot["absorbed"] + tot["tt"]
                            + tot["lateral"] + tot["top"] + tot["roulette_loss"] + tot["capped"]

@@ -168,10 +168,11 @@ def _batch_sums(run_batch, K):
     return acc
 
 
-def test_skin_vessel_1e7_fp32_tile_every_bin(gpu):
-    """The benchmark's configuration (skin-vessel 200^3, FP32 LDS tile) at
-    1e7 photons, against the oracle with sigma per bin from K = 32 batch
-    means (batches of 312500 consecutive photon ids).
+def test_skin_vessel_1e7_benchmark_config_every_bin(gpu):
+    """The benchmark's configuration (skin-vessel 200^3, FP32 LDS tile with
+    the visits outside it as binned records, tuning.mode 2) at 1e7 photons,
+    against the oracle with sigma per bin from K = 32 batch means (batches of
+    312500 consecutive photon ids).
     (a) Same photons [0, 1e7): integer tallies bit-exact; totals Rd, Tt, A
         within 0.5 % (in fact to FP64 summation order); every fluence voxel
         and R(x,y) pixel within 3 sigma (and within the FP32 tile's 2e-5).
@@ -196,8 +197,8 @@ def test_skin_vessel_1e7_fp32_tile_every_bin(gpu):
 
     with MetalMC(w.sim) as mc:
         mc.prepare_data(w.rp, w.tp, w.v, w.ext)
-        mc.set_tuning(P.Tuning.make(mode=1))
-        _say(f"[fullscale] gpu fp32 tile, photons [0, {n})")
+        mc.set_tuning(P.Tuning.make(mode=2))
+        _say(f"[fullscale] gpu fp32 tile + records, photons [0, {n})")
         mc.set_num_runs_seed(w.ext.seed, 0)
         mc.send_compute_command(n)
         same = mc.get_compute_results()
@@ -210,7 +211,7 @@ def test_skin_vessel_1e7_fp32_tile_every_bin(gpu):
             t = mc.get_compute_results()
             gpu_b.append(t.totals)
             return t
-        _say(f"[fullscale] gpu fp32 tile, {K} batches of disjoint photons")
+        _say(f"[fullscale] gpu fp32 tile + records, {K} batches of disjoint photons")
         gpu = _batch_sums(gpu_batch, K)
 
     # (a) same photons
