@@ -336,7 +336,7 @@ def fit_leg(steps: int) -> dict:
     errors at the start, after the Adam steps and after 8 Levenberg-Marquardt
     steps on the same posterior, and the Cramer-Rao bound of each parameter
     for the noise the loss weights by (the target's 1 % noise and the Monte
-    Carlo variance of target and model runs, measured from 16 target batches)
+    Carlo variance of target and model runs, measured from 32 target batches)
     and the start-value prior: what an efficient fit of this data reaches."""
     from nmc_amd.inverse_fit import fit
 

@@ -4,9 +4,11 @@ from noisy Rd(r) profiles with Adam on perturbation-MC gradients.
 Every optimisation step is ONE forward launch over all instances (their
 uneven photon budgets share one device queue) plus ONE backward launch over
 the stored path records.  The target is the forward model at the hidden
-parameters, run as 16 batches on disjoint photon-id ranges (together the
+parameters, run as 32 batches on disjoint photon-id ranges (together the
 instance's budget), plus 1 % Gaussian noise per bin.  The spread of the batch
-means gives each bin's Monte Carlo variance; the loss weights each bin by its
+means gives each bin's Monte Carlo variance (pooled over neighbouring bins as
+a variance-to-mean ratio, so that bins a handful of photons reach still get
+an estimate with many degrees of freedom); the loss weights each bin by its
 noise at the hidden parameters (target noise, the target's Monte Carlo
 variance and the model run's own, which has the same budget).  The start
 values are the hidden ones times exp(U(-s, s)), s = 0.4, so the loss also
@@ -35,7 +37,8 @@ from . import workloads as W
 from .layered import LayeredMC
 
 TARGET_FIRST_PHOTON = 1 << 40   # photon ids of the target run (disjoint from the fit's)
-TARGET_BATCHES = 16
+TARGET_BATCHES = 32
+POOL_BINS = 3      # the variance-to-mean ratio of bin k is pooled over k +- 3
 
 
 def _props(stacks, nl):
@@ -49,8 +52,11 @@ def make_target(cfg: P.LayeredConfig, hidden, seed: int, device: int = 0, noise:
     """Rd(r) per photon of every instance at the hidden parameters, as the
     mean of `batches` runs on disjoint photon-id ranges (each with 1/batches
     of the instance's budget), times (1 + noise N(0,1)) per bin.  Returns
-    (target, clean mean, Monte Carlo variance of one full-budget run per bin
-    from the spread of the batch means)."""
+    (target, clean mean, Monte Carlo variance of one full-budget run per bin).
+    The variance comes from the spread of the batch means, pooled: a bin's
+    variance is its mean times the ratio sum(var) / sum(mean) over the bins
+    within POOL_BINS of it (a photon's exit weight, which sets that ratio,
+    changes slowly with r)."""
     sub = []
     for s in hidden:
         t = P.Stack()
@@ -65,7 +71,11 @@ def make_target(cfg: P.LayeredConfig, hidden, seed: int, device: int = 0, noise:
             runs.append(r.rd_per_photon())
     runs = np.array(runs)
     mean = runs.mean(axis=0)
-    var_run = runs.var(axis=0, ddof=1) / batches
+    var_raw = runs.var(axis=0, ddof=1) / batches
+    kern = np.ones(2 * POOL_BINS + 1)
+    pool = lambda a: np.apply_along_axis(lambda r: np.convolve(r, kern, mode="same"), 1, a)
+    sm, sv = pool(mean), pool(var_raw)
+    var_run = np.where(sm > 0, mean * sv / np.where(sm > 0, sm, 1.0), 0.0)
     rng = np.random.default_rng(seed + 1000)
     return mean * (1.0 + noise * rng.standard_normal(mean.shape)), mean, var_run
 
