@@ -190,6 +190,10 @@ int nmc_last_kernel_ms(nmc_handle* h, float* ms);
 /* Replaces -[MetalMC getComputeResults:...] (MetalMC.h:28, MetalMC.m:508-824):
  * copies the accumulated tallies into the caller's arrays. */
 int nmc_get_results(nmc_handle* h, nmc_results* out);
+/* getComputeResults' own semantics (MetalMC.m:521,787,809): ADDS the
+ * accumulated tallies into the caller's arrays and totals (+=), for callers
+ * that sum several handles or runs into one set of arrays. */
+int nmc_add_results(nmc_handle* h, nmc_results* out);
 
 /* Zeroes the device tallies (the reference re-creates buffers instead). */
 int nmc_reset_tallies(nmc_handle* h);

@@ -44,6 +44,7 @@ def lib():
         "nmc_synchronize": (C.c_int, [H]),
         "nmc_last_kernel_ms": (C.c_int, [H, C.POINTER(C.c_float)]),
         "nmc_get_results": (C.c_int, [H, C.POINTER(P.Results)]),
+        "nmc_add_results": (C.c_int, [H, C.POINTER(P.Results)]),
         "nmc_reset_tallies": (C.c_int, [H]),
         "nmc_destroy": (None, [H]),
         "nmc_last_error": (C.c_char_p, []),
@@ -82,7 +83,7 @@ def lib():
 
 EXPORTED = ("nmc_ext_config_init", "nmc_validate", "nmc_create", "nmc_prepare", "nmc_set_seed",
             "nmc_run", "nmc_run_async", "nmc_synchronize", "nmc_last_kernel_ms",
-            "nmc_get_results", "nmc_reset_tallies", "nmc_destroy", "nmc_last_error",
+            "nmc_get_results", "nmc_add_results", "nmc_reset_tallies", "nmc_destroy", "nmc_last_error",
             "nmc_version", "nmc_set_tuning", "nmc_get_tuning", "nmc_last_launch_count",
             "nmc_last_transport_ms", "nmc_mcxyz_sens", "nmc_mcxyz_sens_results",
             "nmc_prepare_batch",

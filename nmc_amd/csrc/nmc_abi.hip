@@ -1409,6 +1409,46 @@ int nmc_get_results(nmc_handle* h, nmc_results* out) {
   return ok();
 }
 
+int nmc_add_results(nmc_handle* h, nmc_results* out) {
+  if (!h || !out) return fail(NMC_ERR_ARG, "null argument");
+  if (!h->prepared) return fail(NMC_ERR_STATE, "nmc_prepare has not succeeded");
+  if (h->sim == NMC_SIM_MCSUB && h->batch)
+    return fail(NMC_ERR_STATE, "handle holds a batch; use nmc_get_batch_results");
+  int64_t na = 0, nb = 0, nc = 0;   // element counts of the three array slots
+  if (h->sim == NMC_SIM_MCSUB) {
+    na = h->n_ra;
+    nb = h->n_rz;
+  } else {
+    na = nvox(&h->rp);
+    nb = nc = (int64_t)h->rp.Nx * h->rp.Ny;
+  }
+  const bool xyz = h->sim != NMC_SIM_MCSUB;
+  double* a_out = xyz ? out->fluence : out->rd_ra;
+  double* b_out = xyz ? out->ryx : out->a_rz;
+  std::vector<double> a(a_out ? na : 0), b(b_out ? nb : 0);
+  std::vector<uint32_t> c(xyz && out->det2d ? nc : 0);
+  nmc_results tmp{};
+  if (xyz) {
+    tmp.fluence = a_out ? a.data() : nullptr;
+    tmp.ryx = b_out ? b.data() : nullptr;
+    tmp.det2d = out->det2d ? c.data() : nullptr;
+  } else {
+    tmp.rd_ra = a_out ? a.data() : nullptr;
+    tmp.a_rz = b_out ? b.data() : nullptr;
+  }
+  NMC_TRY(nmc_get_results(h, &tmp));
+  for (size_t i = 0; i < a.size(); ++i) a_out[i] += a[i];
+  for (size_t i = 0; i < b.size(); ++i) b_out[i] += b[i];
+  for (size_t i = 0; i < c.size(); ++i) out->det2d[i] += c[i];
+  double* tf = (double*)&out->totals;
+  const double* sf = (const double*)&tmp.totals;
+  for (int k = 0; k < TF_COUNT; ++k) tf[k] += sf[k];
+  uint64_t* ti = &out->totals.photons;
+  const uint64_t* si = &tmp.totals.photons;
+  for (int k = 0; k < TI_COUNT; ++k) ti[k] += si[k];
+  return ok();
+}
+
 int nmc_get_batch_results(nmc_handle* h, double* tally, nmc_totals* totals) {
   if (!h) return fail(NMC_ERR_ARG, "null handle");
   if (!h->prepared || !h->batch) return fail(NMC_ERR_STATE, "nmc_prepare_batch has not succeeded");

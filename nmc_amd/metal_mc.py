@@ -84,6 +84,11 @@ class MetalMC:
         check(lib().nmc_get_results(self._h, C.byref(t.res)))
         return t
 
+    def add_compute_results(self, t: P.TallyArrays) -> None:
+        """getComputeResults' semantics: adds the accumulated tallies into the
+        arrays and totals of t (e.g. summing several handles)."""
+        check(lib().nmc_add_results(self._h, C.byref(t.res)))
+
     def reset_tallies(self) -> None:
         check(lib().nmc_reset_tallies(self._h))
 

@@ -125,3 +125,27 @@ def test_fp32_tile_split_launches(gpu):
 def _num_cu() -> int:
     import torch
     return torch.cuda.get_device_properties(0).multi_processor_count
+
+
+def test_add_results_accumulates_like_get_compute_results(gpu):
+    """nmc_add_results adds into the caller's arrays (getComputeResults'
+    semantics); nmc_get_results overwrites them."""
+    w = W.tiny_voxel_case("mismatch", n_photons=2000)
+    _say("[handles] add_results")
+    with MetalMC(w.sim, 0) as mc:
+        mc.prepare_data(w.rp, w.tp, w.v, w.ext)
+        mc.set_num_runs_seed(w.ext.seed, 0)
+        mc.send_compute_command(2000)
+        got = mc.get_compute_results()
+        acc = mc.get_compute_results()
+        acc.fluence[:] = 1.5
+        acc.ryx[:] = 0.25
+        acc.det2d[:] = 3
+        before = dict(acc.totals)
+        mc.add_compute_results(acc)
+        mc.add_compute_results(acc)
+    np.testing.assert_array_equal(acc.fluence, 1.5 + 2 * got.fluence)
+    np.testing.assert_array_equal(acc.ryx, 0.25 + 2 * got.ryx)
+    np.testing.assert_array_equal(acc.det2d, 3 + 2 * got.det2d)
+    for k, v in got.totals.items():
+        assert acc.totals[k] == before[k] + 2 * v, k
