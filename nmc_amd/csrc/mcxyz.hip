@@ -15,16 +15,19 @@
 // Fluence: deposits are summed per voxel visit in a register (FP64) and added
 // when the photon leaves the voxel - into the workgroup's LDS tile (FP64, or
 // FP32 with twice the voxels: TileT) if the voxel lies inside the tile box,
-// else with an FP64 global atomic.  The tiles are written out as FP64 block
-// partials and summed over blocks in a fixed order.  Rare events (detections,
-// escapes, roulette) go to LDS totals; frequent counters stay in registers
-// until the workgroup ends.  The source and detector-plane code is kept out
-// of line (source_photon, detector_plane) with its parameters in device
-// memory, the loop-invariant values the loop tests or indexes with (tile box,
-// grid size, boundary and detector modes, fluence and voxel-map pointers) sit
-// in vector registers (in_vgpr), and the fluence mode (none, deposit, visit
-// count) is a kernel variant: the scalar registers, which the Philox key
-// schedule largely fills, then hold no parked values the loop must fetch.
+// else with an FP64 global atomic, or (kFluRecords) appended as an 8-byte
+// record to the workgroup's slice of the voxel's bucket, which
+// rec_reduce_kernel sums per bucket after the launch.  The tiles are written
+// out as FP64 block partials and summed over blocks in a fixed order.  Rare
+// events (detections, escapes, roulette) go to LDS totals, one atomic per
+// lane; frequent counters stay in registers until the workgroup ends.  The
+// source and detector-plane code is kept out of line (source_photon,
+// detector_plane) with its parameters in device memory, the loop-invariant
+// values the loop tests or indexes with (tile box, grid size, boundary and
+// detector modes, fluence and voxel-map pointers) sit in vector registers
+// (in_vgpr), and the fluence mode (none, deposit, visit count, records,
+// sensitivities) is a kernel variant: the scalar registers, which the Philox
+// key schedule largely fills, then hold no parked values the loop must fetch.
 #include "nmc_internal.h"
 
 namespace nmc {

@@ -11,7 +11,8 @@
   absorbed in each tissue: perturbation-MC sums against central differences
   in every tissue's mua with common random numbers (mua moves no path, so
   the two agree to 1 %);
-- selection, reset and argument handling through the C-ABI."""
+- selection, reset and argument handling through the C-ABI;
+- graph replays of a sensitivity run."""
 from __future__ import annotations
 
 import copy
@@ -155,3 +156,23 @@ def test_skin_vessel_mua_sensitivities_match_crn_central_difference(gpu):
     _say(f"[sens] pMC\n{pm}\nFD\n{fd}")
     big = np.abs(fd) > 1e-4 * np.abs(fd).max()
     assert np.all(np.abs(pm - fd)[big] <= 0.01 * np.abs(fd)[big]), (pm, fd)
+
+
+def test_sensitivities_in_graph_replays(gpu):
+    """Two graph replays of a sensitivity run equal the oracle on both photon
+    ranges together (the replays advance the photon ids)."""
+    w = three_tissue(10000)
+    _say("[sens] graph replays 2 x 10000")
+    ref, ra, rs = O.run_sens(w, [1, 3], 20000, threads=8)
+    with MetalMC(w.sim, 0) as mc:
+        mc.prepare_data(w.rp, w.tp, w.v, w.ext)
+        mc.set_sensitivity([1, 3])
+        mc.set_graph_mode(True)
+        mc.set_num_runs_seed(w.ext.seed, 0)
+        mc.send_compute_command(10000)
+        mc.send_compute_command(10000)
+        got = mc.get_compute_results()
+        ga, gs = mc.sensitivity_results()
+    assert_same_tallies(got, ref, "graph sens")
+    _close(ga, ra, "graph d/dmua")
+    _close(gs, rs, "graph d/dmus")
