@@ -151,10 +151,15 @@ __device__ __forceinline__ void flush_visit(Photon& p, const XyzArgs& a, const T
       const uint32_t k = p.lin;
       if (FMODE == kFluRecords) {
         RecSlice* sl = &ro.slices[k >> kBucketShift];
+        // the slice's base and capacity are read before the cursor atomic (a
+        // volatile LDS read stays in place), so one wait covers all three
+        const unsigned long long sbase =
+            *(volatile __attribute__((address_space(3))) unsigned long long*)&sl->base;
+        const uint32_t scap = sl->cap;
         const uint32_t slot = atomicAdd(&sl->cnt, 1u);
-        if (slot < sl->cap) {
-          ro.rec[sl->base + slot] = ((unsigned long long)__float_as_uint((float)v) << 32) |
-                                    (k & (kBucketVox - 1u));
+        if (slot < scap) {
+          ro.rec[sbase + slot] = ((unsigned long long)__float_as_uint((float)v) << 32) |
+                                 (k & (kBucketVox - 1u));
           p.vdep = 0.0;
           return;
         }
@@ -523,28 +528,41 @@ __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
         p.W = p.W * d.rsp;
         if (!(p.W > 0.0f)) alive = false;
       } else {
-        // voxel face (kernels.metal:641-679)
-        if (FMODE != kFluNone) flush_visit<FMODE>(p, a, tb, tile, ro, absorbed, sr);
+        // voxel face (kernels.metal:641-679).  The next voxel and its map
+        // entry come first: the load is then in flight while the visit just
+        // ended is flushed (whose LDS cursor atomic and record store would
+        // otherwise sit between the crossing and the load).
+        int nix = p.ix, niy = p.iy, niz = p.iz;
+        uint32_t nlin = p.lin;
         if (axis == 0) {
           const bool fwd = p.ux > 0.0f;
-          p.ix += fwd ? 1 : -1;
-          p.lin += fwd ? ny_v : 0u - ny_v;
+          nix += fwd ? 1 : -1;
+          nlin += fwd ? ny_v : 0u - ny_v;
         } else if (axis == 1) {
           const bool fwd = p.uy > 0.0f;
-          p.iy += fwd ? 1 : -1;
-          p.lin += fwd ? 1u : 0xFFFFFFFFu;
+          niy += fwd ? 1 : -1;
+          nlin += fwd ? 1u : 0xFFFFFFFFu;
         } else {
           const bool fwd = p.uz > 0.0f;
-          p.iz += fwd ? 1 : -1;
-          p.lin += fwd ? nxy_v : 0u - nxy_v;
+          niz += fwd ? 1 : -1;
+          nlin += fwd ? nxy_v : 0u - nxy_v;
         }
-        p.inside = p.ix >= 0 && p.ix < gnx && p.iy >= 0 && p.iy < gny && p.iz >= 0 && p.iz < gnz;
-        if (!p.inside && (p.iz < 0 ? esc_top : esc_any) != 0u) {
+        const bool nin = nix >= 0 && nix < gnx && niy >= 0 && niy < gny && niz >= 0 && niz < gnz;
+        const bool esc = !nin && (niz < 0 ? esc_top : esc_any) != 0u;
+        int ntis = 0;
+        if (!esc)
+          ntis = vmap[nin ? nlin : vidx(a, clampi(nix, gnx), clampi(niy, gny), clampi(niz, gnz))];
+        if (FMODE != kFluNone) flush_visit<FMODE>(p, a, tb, tile, ro, absorbed, sr);
+        p.ix = nix;
+        p.iy = niy;
+        p.iz = niz;
+        p.lin = nlin;
+        p.inside = nin;
+        if (esc) {
           tally_escape(lt, a, p.iz, p.W);
           alive = false;
         } else {
-          p.tis = vmap[p.inside ? p.lin
-                                : vidx(a, clampi(p.ix, gnx), clampi(p.iy, gny), clampi(p.iz, gnz))];
+          p.tis = ntis;
           p.mua = T[p.tis].mua;
           p.mus = T[p.tis].mus;
           p.inv_mus = T[p.tis].inv_mus;
@@ -559,6 +577,11 @@ __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
       alive = false;
     }
     if (alive && !(p.sleft > 0.0f)) {
+      // the next event's random block and step length (kernels.metal:495-502)
+      // do not depend on the spin or the roulette; computed first, they give
+      // the tissue-table reads of the phase function time to arrive
+      const U4 nb = philox(p.pid, p.ev + 1u, 0u, a.key0, a.key1);
+      const float nsl = -log_pos(u01nz(nb.x));
       // SPIN (kernels.metal:695-730)
       float ct = TWO_TERM ? tt_cos(T[p.tis].tt, u01(p.b1)) : hg_cos(T[p.tis].hg, u01(p.b1));
       spin(ct, u01(p.b2), p.ux, p.uy, p.uz);
@@ -584,11 +607,10 @@ __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
       }
       if (alive) {
         p.ev++;
-        U4 b = philox(p.pid, p.ev, 0u, a.key0, a.key1);
-        p.b1 = b.y;
-        p.b2 = b.z;
-        p.b3 = b.w;
-        p.sleft = -log_pos(u01nz(b.x));
+        p.b1 = nb.y;
+        p.b2 = nb.z;
+        p.b3 = nb.w;
+        p.sleft = nsl;
       }
     }
     if (!alive) {
