@@ -132,7 +132,7 @@ def kernel_name(mode: int, block: int = 1024) -> str:
     return f"mcxyz_kernel<false, {block}, {elem}, {MODE_FMODE[mode]}>"
 
 
-def pilot_tuning(mc, hist, mode: int):
+def pilot_tuning(mc, hist, mode: int, rec_fuse: int = 0):
     """Tuning with tile mode `mode` (0 FP64, 1 FP32, 2 FP32 + records) and the
     box of that mode's LDS budget holding the most pilot visits."""
     from nmc_amd import params as P
@@ -141,32 +141,41 @@ def pilot_tuning(mc, hist, mode: int):
     mc.set_tuning(P.Tuning.make(mode=mode))
     budget = int(np.prod(mc.get_tuning().tile[3:6]))
     box = best_boxes(hist, budget, 1)[0][1]
-    return P.Tuning.make(mode=mode, tile=box)
+    return P.Tuning.make(mode=mode, tile=box, rec_fuse=rec_fuse)
 
 
-def tile_leg(mc, hist, n: int, mode: int, steps: int = 2) -> dict:
+def tile_leg(mc, hist, n: int, mode: int, steps: int = 2, rec_fuse: int = 0) -> dict:
     """The same workload with another tile mode (0 FP64 tile, 1 FP32 tile,
-    2 FP32 tile + records), box from the visit pilot."""
+    2 FP32 tile + records; rec_fuse -1: records reduced by their own kernel
+    after each launch), box from the visit pilot.  Blocking runs: each step
+    ends with its own reductions."""
     from nmc_amd.autotune import box_coverage
 
-    t = pilot_tuning(mc, hist, mode)
+    t = pilot_tuning(mc, hist, mode, rec_fuse)
     mc.set_tuning(t)
     box = tuple(t.tile)
-    log(f"[bench] {MODE_NAMES[mode]} tile {box}: {steps} x {n} photons")
+    log(f"[bench] {MODE_NAMES[mode]} tile {box} rec_fuse {rec_fuse}: {steps} x {n} photons")
     mc.reset_tallies()
     mc.set_num_runs_seed(1, 0)
-    ms, kms = [], []
+    ms, kms, wall = [], [], []
     for _ in range(steps):
+        t0 = time.perf_counter()
         mc.send_compute_command(n)
+        wall.append(time.perf_counter() - t0)
         ms.append(mc.last_kernel_ms())
         kms.append(mc.last_transport_ms())
     avg = sum(ms) / len(ms)
-    return {"kernel": kernel_name(mode), "tile": list(box),
-            "lds_tile_fraction": box_coverage(hist, box), "run_ms": avg,
-            "kernel_ms": sum(kms) / len(kms), "photons_per_s": n / (avg * 1e-3),
-            "fluence_parity": ("integer tallies bit-exact; fluence within 2e-5 relative of the "
-                               "oracle (tests/test_gpu_parity.py)") if mode else
-                              "every tally within 1e-9 relative of the oracle"}
+    out = {"kernel": kernel_name(mode), "tile": list(box),
+           "lds_tile_fraction": box_coverage(hist, box), "run_ms": avg,
+           "kernel_ms": sum(kms) / len(kms), "photons_per_s": n / (avg * 1e-3),
+           "fluence_parity": ("integer tallies bit-exact; fluence within 2e-5 relative of the "
+                              "oracle (tests/test_gpu_parity.py)") if mode else
+                             "every tally within 1e-9 relative of the oracle"}
+    if mode == 2:
+        out["wall_ms"] = 1e3 * sum(wall) / len(wall)
+        out["photons_per_s"] = n / (out["wall_ms"] * 1e-3)
+        out["reduction_ms"] = avg - out["kernel_ms"]
+    return out
 
 def sweep_leg(budget_scale: float, reps: int = 3) -> dict:
     """128-wavelength sweep: one batched launch vs 128 plain launches (each
@@ -468,12 +477,19 @@ def main() -> int:
     if eff.mode == 2:
         # the out-of-tile visits leave as 8-byte records, written by the
         # transport kernel and read once by the bucket reduction (pilot-sized
-        # slices: the atomic fallback takes none here)
+        # slices: the atomic fallback takes none here).  Fused (rec_fuse 1):
+        # each launch's workgroups reduce the previous launch's records once
+        # their own photons are done, so kernel_ms holds that work; the last
+        # step's records are reduced in the synchronize that ends the timed
+        # region.
         roof["records"] = {"per_photon": records / tot["photons"],
                            "bytes_per_photon": 8.0 * records / tot["photons"],
-                           "reduction_ms": avg_run_ms - avg_ms,
-                           "reduction_GB_per_s": 8.0 * records / args.steps
-                           / ((avg_run_ms - avg_ms) * 1e-3) / 1e9}
+                           "reduction": ("fused into the next launch's drained tail"
+                                         if eff.rec_fuse == 1 else "own kernel after each launch")}
+        if eff.rec_fuse != 1:
+            roof["records"]["reduction_ms"] = avg_run_ms - avg_ms
+            roof["records"]["reduction_GB_per_s"] = (8.0 * records / args.steps
+                                                     / ((avg_run_ms - avg_ms) * 1e-3) / 1e9)
     else:
         roof["atomic"]["photons_per_s_at_ceiling"] = ceiling["per_s"] * tot["photons"] / global_atomics
     if TRAFFIC_BYTES_PER_PHOTON.get(eff.mode) is not None:
@@ -525,6 +541,9 @@ def main() -> int:
                             "int_tallies_equal": all(g[k] == r[k] for k in P.TOTAL_INT_FIELDS)}
     legs = [s for s in args.legs.split(",") if s]
     if "tile" in legs:
+        if eff.mode == 2 and eff.rec_fuse == 1:
+            # the same records with their own reduction kernel after each launch
+            out["rec_unfused_tile"] = tile_leg(mc, hist, n, 2, rec_fuse=-1)
         for m in sorted(TILE_MODES.values(), reverse=True):
             if m != eff.mode:
                 out[f"{MODE_NAMES[m]}_tile"] = tile_leg(mc, hist, n, m)

@@ -181,6 +181,9 @@ int nmc_run(nmc_handle* h, uint64_t n_photons);
 
 /* Same as nmc_run but returns after enqueueing on the handle's stream. */
 int nmc_run_async(nmc_handle* h, uint64_t n_photons);
+/* Blocks until every enqueued run is done, its tallies complete (with the
+ * fused record reduction, tuning.rec_fuse, this first enqueues the reduction
+ * of the last launch's records). */
 int nmc_synchronize(nmc_handle* h);
 
 /* Milliseconds between HIP events recorded around the transport kernel(s) of
@@ -238,7 +241,17 @@ typedef struct nmc_tuning {
                                same photons, each flushing its tile in FP64  */
     int32_t rec_cap_pct;    /* mode 2: record slice capacity in per cent of
                                the pilot's estimate (0: 100)                 */
-    int32_t reserved[3];
+    int32_t rec_fuse;       /* mode 2: 0 automatic, on when the tile's LDS
+                               holds a bucket's FP64 sums (128 KB); -1 off.
+                               On: two record buffers; consecutive launches
+                               alternate between them, and the workgroups of
+                               each launch, once their own photons are done,
+                               reduce the previous launch's records into the
+                               fluence.  The last launch's records are
+                               reduced by the next launch or, at the latest,
+                               by nmc_synchronize / nmc_run / nmc_get_results
+                               (effective value reported as 1 / -1)          */
+    int32_t reserved[2];
 } nmc_tuning;
 
 /* A rejected request (NMC_ERR_ARG, or an allocation failure) leaves the

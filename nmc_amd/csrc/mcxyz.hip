@@ -16,8 +16,11 @@
 // when the photon leaves the voxel - into the workgroup's LDS tile (FP64, or
 // FP32 with twice the voxels: TileT) if the voxel lies inside the tile box,
 // else with an FP64 global atomic, or (kFluRecords) appended as an 8-byte
-// record to the workgroup's slice of the voxel's bucket, which
-// rec_reduce_kernel sums per bucket after the launch.  The tiles are written
+// record to the workgroup's slice of the voxel's bucket.  The records are
+// summed per bucket after the launch: by rec_reduce_kernel, or by the next
+// launch's workgroups, each once its own photons are done (its LDS then
+// holds a bucket's FP64 sums), which puts that work into the next launch's
+// drained tail, where workgroups finish one by one.  The tiles are written
 // out as FP64 block partials and summed over blocks in a fixed order.  Rare
 // events (detections, escapes, roulette) go to LDS totals, one atomic per
 // lane; frequent counters stay in registers until the workgroup ends.  The
@@ -324,6 +327,42 @@ __device__ __forceinline__ void launch(Photon& p, const XyzArgs& a, const XyzTis
   p.b1 = b.y; p.b2 = b.z; p.b3 = b.w;
   p.sleft = -log_pos(u01nz(b.x));
   p.alive = true;
+}
+
+// rec_reduce_kernel's body, shared with the transport kernel's fused
+// reduction: F[voxels of bucket b] += the records of workgroup slices
+// [w0, w1) of bucket b, summed in FP64 in acc (kBucketVox doubles of LDS),
+// then added to F with one FP64 atomic per non-zero voxel.  Every one of the
+// NT threads of the workgroup calls it.
+template <int NT>
+__device__ __forceinline__ void reduce_part(const RecReduceArgs& r, const RecPart part, double* acc) {
+  for (uint32_t i = threadIdx.x; i < kBucketVox; i += NT) acc[i] = 0.0;
+  __syncthreads();
+  const unsigned long long base = r.rec_base[part.bucket];
+  const uint32_t cap = r.rec_cap[part.bucket];
+  for (int w = part.w0; w < part.w1; ++w) {
+    const uint32_t taken = r.rec_count[(int64_t)part.bucket * r.grid + w];
+    const uint32_t n = taken < cap ? taken : cap;
+    const uint2* src = r.rec + base + (unsigned long long)w * cap;
+    // four independent loads in flight per lane, then the remainder
+    uint32_t i = threadIdx.x;
+    for (; i + 3 * NT < n; i += 4 * NT) {
+      const uint2 x0 = src[i], x1 = src[i + NT], x2 = src[i + 2 * NT], x3 = src[i + 3 * NT];
+      unsafeAtomicAdd(&acc[x0.x], (double)__uint_as_float(x0.y));
+      unsafeAtomicAdd(&acc[x1.x], (double)__uint_as_float(x1.y));
+      unsafeAtomicAdd(&acc[x2.x], (double)__uint_as_float(x2.y));
+      unsafeAtomicAdd(&acc[x3.x], (double)__uint_as_float(x3.y));
+    }
+    for (; i < n; i += NT) {
+      const uint2 x = src[i];
+      unsafeAtomicAdd(&acc[x.x], (double)__uint_as_float(x.y));
+    }
+  }
+  __syncthreads();
+  const uint32_t v0 = (uint32_t)part.bucket << kBucketShift;
+  const uint32_t nv = r.nvox - v0 < kBucketVox ? r.nvox - v0 : kBucketVox;
+  for (uint32_t i = threadIdx.x; i < nv; i += NT)
+    if (acc[i] != 0.0) unsafeAtomicAdd(&r.F[v0 + i], acc[i]);
 }
 
 template <bool TWO_TERM, int BLOCK, typename TileT, int FMODE>
@@ -636,6 +675,22 @@ __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
   if (threadIdx.x < TF_COUNT) unsafeAtomicAdd(&((double*)a.tot)[threadIdx.x], ((double*)&st)[threadIdx.x]);
   if (threadIdx.x < TI_COUNT)
     atomicAdd(&(&a.tot->photons)[threadIdx.x], (unsigned long long)(&st.photons)[threadIdx.x]);
+
+  // ---- the previous launch's records, while the other workgroups drain ----
+  if constexpr (RECS) {
+    if (a.prev_n_parts > 0) {
+      __shared__ uint32_t next_part;
+      for (;;) {
+        // also orders the tile's copy-out above before its LDS is reused
+        __syncthreads();
+        if (threadIdx.x == 0) next_part = atomicAdd(a.prev_next, 1u);
+        __syncthreads();
+        const uint32_t p = next_part;
+        if (p >= (uint32_t)a.prev_n_parts) break;
+        reduce_part<BLOCK>(a.prev, a.prev.parts[p], tile_mem);
+      }
+    }
+  }
 }
 
 // F[tile voxel i] += sum_b part[b][i], blocks in ascending order.
@@ -652,39 +707,10 @@ __global__ void tile_reduce_kernel(const double* __restrict__ part, int grid, Xy
   }
 }
 
-// F[voxels of bucket b] += the records of workgroup slices [w0, w1) of
-// bucket b (one RecPart per workgroup): summed in FP64 in LDS, then added
-// to F with one FP64 atomic per non-zero voxel.
+// One RecPart per workgroup.
 __global__ __launch_bounds__(1024) void rec_reduce_kernel(RecReduceArgs r) {
   extern __shared__ double acc[];   // [kBucketVox]
-  const RecPart part = r.parts[blockIdx.x];
-  for (uint32_t i = threadIdx.x; i < kBucketVox; i += 1024) acc[i] = 0.0;
-  __syncthreads();
-  const unsigned long long base = r.rec_base[part.bucket];
-  const uint32_t cap = r.rec_cap[part.bucket];
-  for (int w = part.w0; w < part.w1; ++w) {
-    const uint32_t taken = r.rec_count[(int64_t)part.bucket * r.grid + w];
-    const uint32_t n = taken < cap ? taken : cap;
-    const uint2* src = r.rec + base + (unsigned long long)w * cap;
-    // four independent loads in flight per lane, then the remainder
-    uint32_t i = threadIdx.x;
-    for (; i + 3 * 1024 < n; i += 4 * 1024) {
-      const uint2 x0 = src[i], x1 = src[i + 1024], x2 = src[i + 2048], x3 = src[i + 3072];
-      unsafeAtomicAdd(&acc[x0.x], (double)__uint_as_float(x0.y));
-      unsafeAtomicAdd(&acc[x1.x], (double)__uint_as_float(x1.y));
-      unsafeAtomicAdd(&acc[x2.x], (double)__uint_as_float(x2.y));
-      unsafeAtomicAdd(&acc[x3.x], (double)__uint_as_float(x3.y));
-    }
-    for (; i < n; i += 1024) {
-      const uint2 x = src[i];
-      unsafeAtomicAdd(&acc[x.x], (double)__uint_as_float(x.y));
-    }
-  }
-  __syncthreads();
-  const uint32_t v0 = (uint32_t)part.bucket << kBucketShift;
-  const uint32_t nv = r.nvox - v0 < kBucketVox ? r.nvox - v0 : kBucketVox;
-  for (uint32_t i = threadIdx.x; i < nv; i += 1024)
-    if (acc[i] != 0.0) unsafeAtomicAdd(&r.F[v0 + i], acc[i]);
+  reduce_part<1024>(r, r.parts[blockIdx.x], acc);
 }
 
 template <bool TT, typename TileT, int FMODE>

@@ -250,6 +250,9 @@ void free_records(XyzRecords& r) {
   free_ptr(r.rec);
   free_ptr(r.count);
   free_ptr(r.parts);
+  free_ptr(r.rec1);
+  free_ptr(r.count1);
+  free_ptr(r.next);
   r = XyzRecords{};
 }
 
@@ -380,7 +383,18 @@ int resolve_xyz_launch(const RunParams& rp, int sim, bool has_fluence, const nmc
     if (tile_n > lds_budget) return fail(NMC_ERR_ARG, "tuning.tile needs %lld voxels, LDS holds %lld",
                                          (long long)tile_n, (long long)lds_budget);
   }
-  const size_t lds = rec_lds + (size_t)tile_n * elem;
+  size_t lds = rec_lds + (size_t)tile_n * elem;
+  // fused record reduction: the transport kernel's LDS must hold a bucket's
+  // FP64 sums (automatic: only when the tile already takes that much)
+  if (r.rec_fuse < -1 || r.rec_fuse > 1) return fail(NMC_ERR_ARG, "tuning.rec_fuse not -1, 0 or 1");
+  constexpr size_t kFuseLds = kBucketVox * sizeof(double);
+  t.rec_fuse = -1;
+  if (n_buckets > 0 && (r.rec_fuse == 1 || (r.rec_fuse == 0 && lds >= kFuseLds))) {
+    if ((size_t)lds_budget * elem + rec_lds < kFuseLds)
+      return fail(NMC_ERR_ARG, "tuning.rec_fuse: the LDS cannot hold a bucket's sums");
+    lds = std::max(lds, kFuseLds);
+    t.rec_fuse = 1;
+  }
   int occ = mcxyz_blocks_per_cu(tt, tile32, t.block, lds);
   if (occ <= 0) return fail(NMC_ERR_ARG, "kernel does not fit on a CU with block %d, %zu B LDS", t.block, lds);
   t.blocks_per_cu = r.blocks_per_cu > 0 ? std::min(r.blocks_per_cu, occ) : occ;
@@ -427,33 +441,53 @@ XyzArgs xyz_args(const RunParams& rp, const nmc_ext_config& e, int sim, uint32_t
   return a;
 }
 
+// The reduction arguments of record slot `slot` of R.
+RecReduceArgs rec_reduce_args(const XyzRecords& R, int slot, double* F, int grid, uint32_t nvox) {
+  RecReduceArgs rr{};
+  rr.parts = R.parts;
+  rr.rec_base = R.base;
+  rr.rec_cap = R.cap;
+  rr.rec = slot ? R.rec1 : R.rec;
+  rr.rec_count = slot ? R.count1 : R.count;
+  rr.F = F;
+  rr.grid = grid;
+  rr.nvox = nvox;
+  return rr;
+}
+
+// Enqueues the reduction of the records still pending in R (fused mode: the
+// last launch's), on stream s.
+int flush_records(XyzRecords& R, double* F, int grid, uint32_t nvox, hipStream_t s) {
+  if (R.pending < 0) return ok();
+  const RecReduceArgs rr = rec_reduce_args(R, R.pending, F, grid, nvox);
+  R.pending = -1;
+  HIP_TRY(launch_rec_reduce(rr, R.n_parts, s));
+  return ok();
+}
+
 // Enqueues photons [first, first + n) of an mcxyz run on stream s, `first`
 // being a.state->first: [reset queue counter, transport kernel, tile
-// reduction, advance first] per launch.  With the FP32 tile each workgroup
-// sums its share of a launch's deposits in FP32; runs are split into
-// launches of at most tuning.split photons per workgroup (each launch
-// flushes its tile into the FP64 array), which bounds those sums to the size
-// the FP32 tile's accuracy was measured at, whatever the grid.  The launches
-// take consecutive photon ids, so the photons are the same.
+// reduction, record reduction, advance first] per launch.  With the FP32
+// tile each workgroup sums its share of a launch's deposits in FP32; runs
+// are split into launches of at most tuning.split photons per workgroup
+// (each launch flushes its tile into the FP64 array), which bounds those
+// sums to the size the FP32 tile's accuracy was measured at, whatever the
+// grid.  The launches take consecutive photon ids, so the photons are the
+// same.  With the fused record reduction (R->rec1 set, `fuse`) a launch
+// writes the record slot the previous one did not and reduces the previous
+// one's records itself; the last launch's stay pending in R for the next
+// launch or flush_records.
 int enqueue_xyz(XyzArgs a, bool two_term, const XyzLaunch& L, uint64_t n, hipStream_t s,
-                uint32_t* launches, const XyzRecords* R, hipEvent_t k0, hipEvent_t k1) {
+                uint32_t* launches, XyzRecords* R, hipEvent_t k0, hipEvent_t k1, bool fuse) {
   const bool recs = R && R->n_buckets > 0 && a.F && !a.count_visits;
+  fuse = fuse && recs && R->rec1 != nullptr;
+  const uint32_t nvox = (uint32_t)((uint64_t)a.Nx * a.Ny * a.Nz);
   a.n_buckets = recs ? R->n_buckets : 0;
   a.rec_base = recs ? R->base : nullptr;
   a.rec_cap = recs ? R->cap : nullptr;
-  a.rec = recs ? R->rec : nullptr;
-  a.rec_count = recs ? R->count : nullptr;
-  RecReduceArgs rr{};
-  if (recs) {
-    rr.parts = R->parts;
-    rr.rec_base = R->base;
-    rr.rec_cap = R->cap;
-    rr.rec = R->rec;
-    rr.rec_count = R->count;
-    rr.F = a.F;
-    rr.grid = L.grid;
-    rr.nvox = (uint32_t)((uint64_t)a.Nx * a.Ny * a.Nz);
-  }
+  a.prev_n_parts = 0;
+  a.prev_next = nullptr;
+  if (recs && !fuse) NMC_TRY(flush_records(*R, a.F, L.grid, nvox, s));
   a.tile_x0 = L.t.tile[0];
   a.tile_y0 = L.t.tile[1];
   a.tile_z0 = L.t.tile[2];
@@ -467,12 +501,32 @@ int enqueue_xyz(XyzArgs a, bool two_term, const XyzLaunch& L, uint64_t n, hipStr
     const uint64_t part = std::min<uint64_t>(step, n - done);
     a.n = part;
     a.chunk = pick_chunk(part, L.grid * (L.t.block / 64), L.t.chunk);
+    const int slot = fuse ? R->slot : 0;
+    if (recs) {
+      a.rec = slot ? R->rec1 : R->rec;
+      a.rec_count = slot ? R->count1 : R->count;
+      if (fuse && R->pending >= 0) {
+        a.prev = rec_reduce_args(*R, R->pending, a.F, L.grid, nvox);
+        a.prev_n_parts = R->n_parts;
+        a.prev_next = R->next;
+        HIP_TRY(hipMemsetAsync(R->next, 0, sizeof(uint32_t), s));
+      } else {
+        a.prev_n_parts = 0;
+      }
+    }
     HIP_TRY(launch_state_begin(a.state, s));
     if (k0) HIP_TRY(hipEventRecord(k0, s));
     HIP_TRY(launch_mcxyz(a, two_term, L.t.mode >= 1, L.t.block, L.grid, L.lds, s));
     if (k1) HIP_TRY(hipEventRecord(k1, s));
     HIP_TRY(launch_tile_reduce(a.tile_part, L.grid, a, s));
-    if (recs) HIP_TRY(launch_rec_reduce(rr, R->n_parts, s));
+    if (recs) {
+      if (fuse) {
+        R->pending = slot;
+        R->slot = slot ^ 1;
+      } else {
+        HIP_TRY(launch_rec_reduce(rec_reduce_args(*R, 0, a.F, L.grid, nvox), R->n_parts, s));
+      }
+    }
     HIP_TRY(launch_state_advance(a.state, part, s));
     ++*launches;
   }
@@ -588,7 +642,7 @@ int record_rates(nmc_handle* h, const XyzLaunch& L, std::vector<double>* rate) {
   P.n_buckets = 0;
   uint32_t launches = 0;
   int rc = enqueue_xyz(a, h->sim == NMC_SIM_MCXYZ_TT, P, kRecPilot, h->stream, &launches, nullptr,
-                       nullptr, nullptr);
+                       nullptr, nullptr, false);
   std::vector<double> host;
   if (rc == NMC_OK) {
     host.resize(nv);
@@ -616,7 +670,8 @@ int record_rates(nmc_handle* h, const XyzLaunch& L, std::vector<double>* rate) {
 // pilot's expected out-of-tile visits of tuning.split photons (plus 15 % for
 // uneven shares of the queue, 10 % and 6 sigma of Poisson slack, 32 records),
 // scaled by tuning.rec_cap_pct percent when set; the reduction takes about
-// 4e6 records per workgroup.
+// 4e6 records per workgroup.  With the fused reduction (tuning.rec_fuse) a
+// second record buffer and slot counts of the same shape.
 int make_records(nmc_handle* h, const XyzLaunch& L, XyzRecords* R) {
   std::vector<double> rate;
   NMC_TRY(record_rates(h, L, &rate));
@@ -648,6 +703,11 @@ int make_records(nmc_handle* h, const XyzLaunch& L, XyzRecords* R) {
   HIP_TRY(hipMalloc(&R->count, (size_t)nb * grid * sizeof(uint32_t)));
   HIP_TRY(hipMalloc(&R->parts, parts.size() * sizeof(RecPart)));
   HIP_TRY(hipMalloc(&R->rec, std::max<unsigned long long>(total, 1) * sizeof(uint2)));
+  if (L.t.rec_fuse == 1) {
+    HIP_TRY(hipMalloc(&R->count1, (size_t)nb * grid * sizeof(uint32_t)));
+    HIP_TRY(hipMalloc(&R->rec1, std::max<unsigned long long>(total, 1) * sizeof(uint2)));
+    HIP_TRY(hipMalloc(&R->next, sizeof(uint32_t)));
+  }
   HIP_TRY(hipMemcpy(R->base, base.data(), nb * sizeof(unsigned long long), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(R->cap, cap.data(), nb * sizeof(uint32_t), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(R->parts, parts.data(), parts.size() * sizeof(RecPart), hipMemcpyHostToDevice));
@@ -821,7 +881,10 @@ int enqueue_run(nmc_handle* h, uint64_t n) {
       NMC_TRY(enqueue_xyz_sens(h, a, n, s, k0, k1));
       launches = 1;
     } else {
-      NMC_TRY(enqueue_xyz(a, h->sim == NMC_SIM_MCXYZ_TT, L, n, s, &launches, &h->recs, k0, k1));
+      // graph mode: every launch reduces its own records (a captured graph
+      // holds no state carried between replays)
+      NMC_TRY(enqueue_xyz(a, h->sim == NMC_SIM_MCXYZ_TT, L, n, s, &launches, &h->recs, k0, k1,
+                          !h->graph_mode));
     }
     h->k_timed = k0 != nullptr;
     h->launches += launches;
@@ -849,6 +912,15 @@ int enqueue_xyz_sens(nmc_handle* h, XyzArgs a, uint64_t n, hipStream_t s, hipEve
   return ok();
 }
 
+// Enqueues the reduction of the records the last fused mcxyz launch left
+// pending (nothing for other handles), on the handle's stream: every entry
+// point that reads or changes the fluence, or runs without fusing, calls it
+// first.
+int flush_pending(nmc_handle* h) {
+  if (h->recs.pending < 0) return ok();
+  return flush_records(h->recs, h->dF, h->xyz_grid, (uint32_t)nvox(&h->rp), h->stream);
+}
+
 int write_first(nmc_handle* h) {
   RunState st{0ull, (unsigned long long)h->next_photon};
   HIP_TRY(hipMemcpyAsync(h->dState, &st, sizeof st, hipMemcpyHostToDevice, h->stream));
@@ -860,6 +932,9 @@ int write_first(nmc_handle* h) {
 // graph mode.
 int run_async(nmc_handle* h, uint64_t n) {
   HIP_TRY(hipSetDevice(h->device));
+  // graph and sensitivity runs reduce their own records; pending ones from
+  // earlier eager launches go first, outside any capture
+  if (h->graph_mode || h->sens_n > 0) NMC_TRY(flush_pending(h));
   HIP_TRY(hipEventRecord(h->e0, h->stream));
   h->launches = 0;
   if (h->graph_mode) {
@@ -1174,6 +1249,8 @@ int nmc_reset_tallies(nmc_handle* h) {
   if (!h) return fail(NMC_ERR_ARG, "null handle");
   if (!h->prepared) return fail(NMC_ERR_STATE, "nmc_prepare has not succeeded");
   HIP_TRY(hipSetDevice(h->device));
+  // records still awaiting their reduction belong to the tallies cleared here
+  h->recs.pending = -1;
   HIP_TRY(hipMemsetAsync(h->dTot, 0, sizeof(nmc_totals), h->stream));
   if (h->sim == NMC_SIM_MCSUB) {
     size_t n = (size_t)h->n_items * (per_item(h) + TF_COUNT + TI_COUNT);
@@ -1208,6 +1285,7 @@ int nmc_set_tuning(nmc_handle* h, const nmc_tuning* t) {
   h->tune_req = *t;
   if (!h->prepared) return ok();
   HIP_TRY(hipSetDevice(h->device));
+  NMC_TRY(flush_pending(h));
   HIP_TRY(hipStreamSynchronize(h->stream));
   if (h->sim == NMC_SIM_LAYERED) {
     // the grid, and the buffers sized by it, are fixed at nmc_layered_prepare;
@@ -1251,6 +1329,7 @@ int nmc_mcxyz_sens(nmc_handle* h, const int32_t* tissue_ids, int n_tissues) {
       if (tissue_ids[i] == tissue_ids[j]) return fail(NMC_ERR_ARG, "tissue id %d given twice", tissue_ids[j]);
   }
   HIP_TRY(hipSetDevice(h->device));
+  NMC_TRY(flush_pending(h));
   HIP_TRY(hipStreamSynchronize(h->stream));
   const bool tt = h->sim == NMC_SIM_MCXYZ_TT;
   const size_t lds = (size_t)2 * n_tissues * kSensBlock * sizeof(uint32_t);
@@ -1321,6 +1400,7 @@ int nmc_get_tuning(nmc_handle* h, nmc_tuning* t) {
 int nmc_set_graph_mode(nmc_handle* h, int enable) {
   if (!h) return fail(NMC_ERR_ARG, "null handle");
   HIP_TRY(hipSetDevice(h->device));
+  NMC_TRY(flush_pending(h));
   HIP_TRY(hipStreamSynchronize(h->stream));
   h->graph_mode = enable != 0;
   drop_graph(h);
@@ -1350,6 +1430,7 @@ int nmc_run_async(nmc_handle* h, uint64_t n_photons) {
 int nmc_synchronize(nmc_handle* h) {
   if (!h) return fail(NMC_ERR_ARG, "null handle");
   HIP_TRY(hipSetDevice(h->device));
+  NMC_TRY(flush_pending(h));
   HIP_TRY(hipStreamSynchronize(h->stream));
   return ok();
 }
@@ -1387,6 +1468,7 @@ int nmc_get_results(nmc_handle* h, nmc_results* out) {
   if (!h || !out) return fail(NMC_ERR_ARG, "null argument");
   if (!h->prepared) return fail(NMC_ERR_STATE, "nmc_prepare has not succeeded");
   HIP_TRY(hipSetDevice(h->device));
+  NMC_TRY(flush_pending(h));
   HIP_TRY(hipStreamSynchronize(h->stream));
   if (h->sim == NMC_SIM_MCSUB) {
     if (h->batch) return fail(NMC_ERR_STATE, "handle holds a batch; use nmc_get_batch_results");
@@ -2006,6 +2088,6 @@ extern "C" int nmc_mcxyz_run_noalloc(int sim_type, const RunParams* rp, const Ti
   a.state = dstate;
   uint32_t launches = 0;
   NMC_TRY(enqueue_xyz(a, sim_type == NMC_SIM_MCXYZ_TT, L, n_photons, s, &launches, nullptr,
-                      nullptr, nullptr));
+                      nullptr, nullptr, false));
   return ok();
 }

@@ -52,6 +52,30 @@ struct XyzAux {
   uint32_t* det2d;                  // [Nx][Ny] detected photons
 };
 
+// Buckets of the binned deposit records: 2^14 voxels, whose FP64 sums fill
+// 128 KB of LDS in the reduction.
+constexpr int kBucketShift = 14;
+constexpr uint32_t kBucketVox = 1u << kBucketShift;
+// At most this many buckets (their slices take 16 B of the transport
+// kernel's LDS each): grids up to 2048 x 16384 = 33.5 M voxels.
+constexpr int kMaxBuckets = 2048;
+
+// One workgroup of the record reduction: bucket b, workgroup slices [w0, w1).
+struct RecPart {
+  int bucket, w0, w1, pad;
+};
+
+struct RecReduceArgs {
+  const RecPart* parts;
+  const unsigned long long* rec_base;
+  const uint32_t* rec_cap;
+  const uint2* rec;
+  const uint32_t* rec_count;
+  double* F;
+  int grid;                         // transport workgroups (slices per bucket)
+  uint32_t nvox;
+};
+
 struct XyzArgs {
   int Nx, Ny, Nz;
   float dx, dy, dz, hx, hy;
@@ -89,30 +113,14 @@ struct XyzArgs {
   double* sens;
   int sens_n;
   float sens_inv_mus[NMC_SENS_MAX];
-};
-
-// Buckets of the binned deposit records: 2^14 voxels, whose FP64 sums fill
-// 128 KB of LDS in the reduction.
-constexpr int kBucketShift = 14;
-constexpr uint32_t kBucketVox = 1u << kBucketShift;
-// At most this many buckets (their slices take 16 B of the transport
-// kernel's LDS each): grids up to 2048 x 16384 = 33.5 M voxels.
-constexpr int kMaxBuckets = 2048;
-
-// One workgroup of the record reduction: bucket b, workgroup slices [w0, w1).
-struct RecPart {
-  int bucket, w0, w1, pad;
-};
-
-struct RecReduceArgs {
-  const RecPart* parts;
-  const unsigned long long* rec_base;
-  const uint32_t* rec_cap;
-  const uint2* rec;
-  const uint32_t* rec_count;
-  double* F;
-  int grid;                         // transport workgroups (slices per bucket)
-  uint32_t nvox;
+  // the previous launch's records (tuning.mode 2, fused reduction): a
+  // workgroup whose own photons are done takes parts of them one at a time
+  // from *prev_next, sums each in its LDS (which the tile no longer needs)
+  // and adds the sums to F, so the reduction of one launch runs in the
+  // drained tail of the next.  prev_n_parts = 0: nothing to reduce.
+  RecReduceArgs prev;
+  int prev_n_parts;
+  uint32_t* prev_next;
 };
 
 // One medium of a (batched) mcsub launch.
@@ -245,6 +253,18 @@ struct XyzRecords {
   RecPart* parts = nullptr;             // reduction workgroups
   int n_parts = 0;
   uint64_t n_rec = 0;                   // records the buffer holds
+  // fused reduction (when the transport kernel's LDS holds a bucket's FP64
+  // sums): a second record buffer, so that consecutive launches alternate
+  // between slot 0 (rec, count) and slot 1 (rec1, count1) and each reduces
+  // the other slot's records of the launch before it; null: one slot, every
+  // launch followed by rec_reduce_kernel
+  uint2* rec1 = nullptr;
+  uint32_t* count1 = nullptr;
+  uint32_t* next = nullptr;             // part counter of the fused reduction
+  // host-side state: the slot the next launch writes and the slot whose
+  // records still await their reduction (-1: none)
+  int slot = 0;
+  int pending = -1;
 };
 
 // The sensitivity variant: 1024-lane workgroups, lds = its per-lane rows.

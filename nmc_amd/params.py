@@ -128,19 +128,21 @@ class Tuning(C.Structure):
 
     _fields_ = [("block", C.c_int32), ("blocks_per_cu", C.c_int32), ("refill_min", C.c_int32),
                 ("chunk", C.c_int32), ("tile", C.c_int32 * 6), ("mode", C.c_int32),
-                ("split", C.c_int32), ("rec_cap_pct", C.c_int32), ("reserved", C.c_int32 * 3)]
+                ("split", C.c_int32), ("rec_cap_pct", C.c_int32), ("rec_fuse", C.c_int32),
+                ("reserved", C.c_int32 * 2)]
 
     def as_dict(self) -> dict:
         return {"block": self.block, "blocks_per_cu": self.blocks_per_cu,
                 "refill_min": self.refill_min, "chunk": self.chunk, "tile": list(self.tile),
-                "mode": self.mode, "split": self.split, "rec_cap_pct": self.rec_cap_pct}
+                "mode": self.mode, "split": self.split, "rec_cap_pct": self.rec_cap_pct,
+                "rec_fuse": self.rec_fuse}
 
     @classmethod
     def make(cls, block=0, blocks_per_cu=0, refill_min=0, chunk=0, tile=None,
-             mode=0, split=0, rec_cap_pct=0) -> "Tuning":
+             mode=0, split=0, rec_cap_pct=0, rec_fuse=0) -> "Tuning":
         t = cls()
         t.block, t.blocks_per_cu, t.refill_min, t.chunk = block, blocks_per_cu, refill_min, chunk
-        t.mode, t.split, t.rec_cap_pct = mode, split, rec_cap_pct
+        t.mode, t.split, t.rec_cap_pct, t.rec_fuse = mode, split, rec_cap_pct, rec_fuse
         for i, v in enumerate(tile or (0,) * 6):
             t.tile[i] = v
         return t

@@ -126,7 +126,10 @@ def test_fp32_tile(gpu):
     assert abs(got.fluence.sum() - ref.fluence.sum()) <= 1e-6 * ref.fluence.sum()
 
 
-def _run_tuned(w, n, tuning, graph_runs=0):
+def _run_tuned(w, n, tuning, graph_runs=0, async_runs=0):
+    """Runs n photons under `tuning`: one blocking run, `graph_runs` graph
+    replays of it, or `async_runs` asynchronous runs of n / async_runs
+    photons each (consecutive ids, no wait between them)."""
     with MetalMC(w.sim) as mc:
         mc.prepare_data(w.rp, w.tp, w.v, w.ext)
         mc.set_tuning(tuning)
@@ -137,7 +140,11 @@ def _run_tuned(w, n, tuning, graph_runs=0):
         for _ in range(max(1, graph_runs)):
             mc.reset_tallies()
             mc.set_num_runs_seed(w.ext.seed, 0)
-            mc.send_compute_command(n)
+            if async_runs:
+                for _ in range(async_runs):
+                    mc.send_compute_command_async(n // async_runs)
+            else:
+                mc.send_compute_command(n)
             out.append(mc.get_compute_results())
     return eff, out
 
@@ -151,22 +158,66 @@ def _assert_fp32_class(got, ref, label):
     assert abs(got.fluence.sum() - ref.fluence.sum()) <= 1e-6 * ref.fluence.sum(), label
 
 
-@pytest.mark.parametrize("case", ["skin", "overflow", "split", "two_term", "graph"])
+@pytest.mark.parametrize("case", ["skin", "overflow", "split", "two_term", "graph", "async",
+                                  "unfused", "unfused_split", "small_tile_fused"])
 def test_binned_deposit_records(gpu, case):
     """tuning.mode 2 (FP32 tile + binned deposit records): the same photons as
     the oracle, fluence within the FP32 tile's bound; with 1 % of the pilot's
-    slice capacity most visits take the atomic fallback; split launches each
-    reduce their own records; graph replays equal the oracle every time."""
+    slice capacity most visits take the atomic fallback.  Fused reduction:
+    split launches (each reducing the previous one's records in its drained
+    tail), asynchronous runs (each reducing the previous run's), a tile too
+    small for the automatic choice with the fusion forced on; unfused: every
+    launch followed by its own reduction; graph replays (unfused by design)
+    equal the oracle every time."""
     w = W.tiny_voxel_case("two_term") if case == "two_term" else W.skin_vessel()
     n = 20_000 if case == "two_term" else 200_000
     _say(f"[parity] binned deposit records: {case}, n={n}")
     ref = O.run(w, n, threads=8)
+    small = (90, 90, 0, 20, 20, 10) if case == "small_tile_fused" else None
     t = P.Tuning.make(mode=2, rec_cap_pct=1 if case == "overflow" else 0,
-                      split=50 if case == "split" else 0)
-    eff, outs = _run_tuned(w, n, t, graph_runs=3 if case == "graph" else 0)
+                      split=50 if case in ("split", "unfused_split") else 0,
+                      rec_fuse=-1 if case.startswith("unfused") else (1 if small else 0),
+                      tile=small)
+    eff, outs = _run_tuned(w, n, t, graph_runs=3 if case == "graph" else 0,
+                           async_runs=4 if case == "async" else 0)
     assert eff.mode == 2 and eff.tile[3] > 0
+    # the tiny grid's whole tile is smaller than a bucket's sums: unfused
+    fused = not case.startswith("unfused") and case != "two_term"
+    assert eff.rec_fuse == (1 if fused else -1), eff.rec_fuse
     for i, got in enumerate(outs):
         _assert_fp32_class(got, ref, f"{case} #{i}")
+
+
+def test_record_fusion_choice_and_pending_records(gpu):
+    """The automatic choice leaves a tile smaller than a bucket's FP64 sums
+    unfused; records pending from an asynchronous run are dropped by
+    reset_tallies (they belong to the cleared tallies) and reduced by
+    set_tuning, so a later run's fluence holds exactly its own photons."""
+    w = W.skin_vessel()
+    n = 100_000
+    _say("[parity] record fusion: choice, reset and set_tuning with records pending")
+    ref = O.run(w, n, threads=8)
+    with MetalMC(w.sim) as mc:
+        mc.prepare_data(w.rp, w.tp, w.v, w.ext)
+        mc.set_tuning(P.Tuning.make(mode=2, tile=(90, 90, 0, 20, 20, 10)))
+        assert mc.get_tuning().rec_fuse == -1
+        mc.set_tuning(P.Tuning.make(mode=2))
+        assert mc.get_tuning().rec_fuse == 1
+        # a run whose records are still pending, then a reset: not counted
+        mc.set_num_runs_seed(w.ext.seed, 5_000_000)
+        mc.send_compute_command_async(n)
+        mc.reset_tallies()
+        mc.set_num_runs_seed(w.ext.seed, 0)
+        mc.send_compute_command_async(n)
+        got = mc.get_compute_results()
+        _assert_fp32_class(got, ref, "after reset")
+        # pending records reduced before a new tuning frees their buffer
+        mc.reset_tallies()
+        mc.set_num_runs_seed(w.ext.seed, 0)
+        mc.send_compute_command_async(n)
+        mc.set_tuning(P.Tuning.make(mode=2, rec_fuse=-1))
+        got = mc.get_compute_results()
+        _assert_fp32_class(got, ref, "after set_tuning")
 
 
 def test_autotuner_runs(gpu):
