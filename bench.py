@@ -10,9 +10,13 @@ cores over a bounded sample of the same workload.
 
 The timed steps use the FP32 LDS fluence tile on the box the visit-count
 pilot finds, with the voxel visits outside it written as binned deposit
-records and summed per bucket in FP64 by a reduction kernel (tuning.mode 2;
-sums within a workgroup's tile in FP32, everything else in FP64; the
-reference adds every deposit into an FP32 global array).  `--tile fp32`
+records and summed per bucket in FP64 (tuning.mode 2; sums within a
+workgroup's tile in FP32, everything else in FP64; the reference adds every
+deposit into an FP32 global array).  The records of one step are summed by
+the next step's workgroups once their own photons are done (the fused
+reduction), those of the last step by the synchronize that closes the timed
+region; the `tile` leg also times the records with their own reduction
+kernel after each step.  `--tile fp32`
 times the FP32 tile with FP64 atomics outside it, `--tile fp64` the all-FP64
 tile; the `tile` leg reports the two modes not timed.  Every timed step (and
 the warm-up) launches mcxyz_kernel with 1024-lane blocks; the other launches
@@ -52,10 +56,11 @@ HBM_PEAK_GBS = 8000.0
 # rocprofv3 FETCH_SIZE and WRITE_SIZE passes over one 1e8-photon launch,
 # scaled by the copy calibration (FETCH x2.0, WRITE x1.0): FP64
 # (18 955 068 KiB x 2 + 98 872 849 KiB) x 1024 B / 1e8, FP32 (18 945 480 KiB
-# x 2 + 92 328 830 KiB) x 1024 B / 1e8, records (38 787 224 KiB x 2 +
-# 63 400 064 KiB) x 1024 B / 1e8, each on the visit-pilot box
-# (profiles/mcxyz_skinvessel_200_pmc_traffic.md).
-TRAFFIC_BYTES_PER_PHOTON = {0: 1400.7, 1: 1333.5, 2: 1443.6}
+# x 2 + 92 328 830 KiB) x 1024 B / 1e8, records (50 647 279 KiB x 2 +
+# 63 183 368 KiB) x 1024 B / 1e8 for a launch that also reduces the previous
+# launch's records (the timed steps after the first), each on the visit-pilot
+# box (profiles/mcxyz_skinvessel_200_pmc_traffic.md).
+TRAFFIC_BYTES_PER_PHOTON = {0: 1400.7, 1: 1333.5, 2: 1684.3}
 
 
 def log(msg: str) -> None:
