@@ -12,7 +12,7 @@
 // `chunk` ids at a time from one device counter.  A photon's random numbers
 // depend only on its id, never on the lane or the order of execution.
 //
-// Fluence: deposits are summed per voxel visit in a register (FP64) and added
+// Fluence: deposits are summed per voxel visit in a register (FP32) and added
 // when the photon leaves the voxel - into the workgroup's LDS tile (FP64, or
 // FP32 with twice the voxels: TileT) if the voxel lies inside the tile box,
 // else with an FP64 global atomic, or (kFluRecords) appended as an 8-byte
@@ -47,7 +47,9 @@ struct Photon {
   bool inside, alive;
   uint64_t pid;
   uint32_t ev, events, b1, b2, b3;
-  double vdep;
+  // the current voxel visit's deposits, summed in FP32 (a visit is one to a
+  // few pieces) and added to the tallies in FP64 when it ends
+  float vdep;
   // sensitivity variant: the voxel visit's sums (sp = sum D L_k, sq = sum D
   // K_k, sza = sum W s e^{-x}, szs = sum W s h(x)) and the tissue's slot
   float sp, sq, sza, szs;
@@ -123,7 +125,7 @@ __device__ __forceinline__ void flush_visit(Photon& p, const XyzArgs& a, const T
                                             const SensRows& sr) {
   constexpr bool COUNT = FMODE == kFluCount;
   if constexpr (FMODE == kFluSens) {
-    if (p.inside && (p.vdep != 0.0 || p.sza != 0.0f)) {
+    if (p.inside && (p.vdep != 0.0f || p.sza != 0.0f)) {
       const size_t nvox = (size_t)a.Nx * (size_t)a.Ny * (size_t)a.Nz;
       const gptr<double> out = (gptr<double>)a.sens;
       for (int j = 0; j < a.sens_n; ++j) {
@@ -134,9 +136,9 @@ __device__ __forceinline__ void flush_visit(Photon& p, const XyzArgs& a, const T
         } else {
           const float Lj = sr.L[j * sr.stride];
           const uint32_t Kj = sr.K[j * sr.stride];
-          if (p.vdep == 0.0 || (Lj == 0.0f && Kj == 0u)) continue;
-          ga = -(double)Lj * p.vdep;
-          gs = ((double)Kj * (double)a.sens_inv_mus[j] - (double)Lj) * p.vdep;
+          if (p.vdep == 0.0f || (Lj == 0.0f && Kj == 0u)) continue;
+          ga = -(double)Lj * (double)p.vdep;
+          gs = ((double)Kj * (double)a.sens_inv_mus[j] - (double)Lj) * (double)p.vdep;
         }
         global_add(out + ((size_t)j * nvox + p.lin), ga);
         global_add(out + ((size_t)(a.sens_n + j) * nvox + p.lin), gs);
@@ -144,9 +146,13 @@ __device__ __forceinline__ void flush_visit(Photon& p, const XyzArgs& a, const T
     }
     p.sp = p.sq = p.sza = p.szs = 0.0f;
   }
-  absorbed += p.vdep;
-  if (p.inside && (COUNT || p.vdep != 0.0)) {
-    double v = COUNT ? 1.0 : p.vdep;
+  absorbed += (double)p.vdep;
+  if constexpr (FMODE == kFluNone) {
+    p.vdep = 0.0f;
+    return;
+  }
+  if (p.inside && (COUNT || p.vdep != 0.0f)) {
+    double v = COUNT ? 1.0 : (double)p.vdep;
     uint32_t lx = (uint32_t)p.ix - tb.x0, ly = (uint32_t)p.iy - tb.y0, lz = (uint32_t)p.iz - tb.z0;
     if (lx < tb.nx && ly < tb.ny && lz < tb.nz) {
       unsafeAtomicAdd(&tile[(lz * tb.nx + lx) * tb.ny + ly], (TileT)v);
@@ -163,14 +169,14 @@ __device__ __forceinline__ void flush_visit(Photon& p, const XyzArgs& a, const T
         if (slot < scap) {
           ro.rec[sbase + slot] = ((unsigned long long)__float_as_uint((float)v) << 32) |
                                  (k & (kBucketVox - 1u));
-          p.vdep = 0.0;
+          p.vdep = 0.0f;
           return;
         }
       }
       global_add(&tb.F[k], v);
     }
   }
-  p.vdep = 0.0;
+  p.vdep = 0.0f;
 }
 
 // Event tallies of the transport loop into the workgroup totals in LDS, at
@@ -307,7 +313,7 @@ __device__ __forceinline__ void launch(Photon& p, const XyzArgs& a, const XyzTis
   p.iy = (int)floorf(a.hy + y / a.dy);
   p.iz = (int)floorf(z / a.dz);
   p.inside = p.ix >= 0 && p.ix < a.Nx && p.iy >= 0 && p.iy < a.Ny && p.iz >= 0 && p.iz < a.Nz;
-  p.vdep = 0.0;
+  p.vdep = 0.0f;
   if (!p.inside && (a.bmode == 1 || (a.bmode == 2 && p.iz < 0))) {
     tally_escape(lt, a, p.iz, W);
     p.alive = false;
@@ -422,7 +428,7 @@ __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
   p.ux = p.uy = 0.0f;
   p.uz = 1.0f;
   p.W = 0.0f;
-  p.vdep = 0.0;
+  p.vdep = 0.0f;
   p.ix = p.iy = p.iz = 0;
   p.tis = 0;
   p.slot = -1;
@@ -535,10 +541,9 @@ __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
     }
     p.W = p.W - dep;
     // inside the grid the deposit joins the visit's sum, which the voxel's
-    // flush also adds to the lane's absorbed total (no fluence: at once)
+    // flush also adds to the lane's absorbed total
     if (p.inside) {
-      if (FMODE == kFluNone) absorbed += (double)dep;
-      else p.vdep += (double)dep;
+      p.vdep = p.vdep + dep;
     } else {
       absorbed_out += (double)dep;
     }
@@ -571,27 +576,21 @@ __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
         // entry come first: the load is then in flight while the visit just
         // ended is flushed (whose LDS cursor atomic and record store would
         // otherwise sit between the crossing and the load).
-        int nix = p.ix, niy = p.iy, niz = p.iz;
-        uint32_t nlin = p.lin;
-        if (axis == 0) {
-          const bool fwd = p.ux > 0.0f;
-          nix += fwd ? 1 : -1;
-          nlin += fwd ? ny_v : 0u - ny_v;
-        } else if (axis == 1) {
-          const bool fwd = p.uy > 0.0f;
-          niy += fwd ? 1 : -1;
-          nlin += fwd ? 1u : 0xFFFFFFFFu;
-        } else {
-          const bool fwd = p.uz > 0.0f;
-          niz += fwd ? 1 : -1;
-          nlin += fwd ? nxy_v : 0u - nxy_v;
-        }
+        // (selects, not a branch per axis, which a wave with lanes crossing
+        // along different axes would issue in turn)
+        const bool fx = p.ux > 0.0f, fy = p.uy > 0.0f, fz = p.uz > 0.0f;
+        const int nix = p.ix + (axis == 0 ? (fx ? 1 : -1) : 0);
+        const int niy = p.iy + (axis == 1 ? (fy ? 1 : -1) : 0);
+        const int niz = p.iz + (axis == 2 ? (fz ? 1 : -1) : 0);
+        const uint32_t nlin = p.lin + (axis == 0   ? (fx ? ny_v : 0u - ny_v)
+                                       : axis == 1 ? (fy ? 1u : 0xFFFFFFFFu)
+                                                   : (fz ? nxy_v : 0u - nxy_v));
         const bool nin = nix >= 0 && nix < gnx && niy >= 0 && niy < gny && niz >= 0 && niz < gnz;
         const bool esc = !nin && (niz < 0 ? esc_top : esc_any) != 0u;
         int ntis = 0;
         if (!esc)
           ntis = vmap[nin ? nlin : vidx(a, clampi(nix, gnx), clampi(niy, gny), clampi(niz, gnz))];
-        if (FMODE != kFluNone) flush_visit<FMODE>(p, a, tb, tile, ro, absorbed, sr);
+        flush_visit<FMODE>(p, a, tb, tile, ro, absorbed, sr);
         p.ix = nix;
         p.iy = niy;
         p.iz = niz;
@@ -653,7 +652,7 @@ __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
       }
     }
     if (!alive) {
-      if (FMODE != kFluNone) flush_visit<FMODE>(p, a, tb, tile, ro, absorbed, sr);
+      flush_visit<FMODE>(p, a, tb, tile, ro, absorbed, sr);
       p.alive = false;
     }
   }

@@ -133,15 +133,15 @@ float nmco_logf(float x)
     return fk * 0x1.62e4p-1f - ((hfsq - (s * (hfsq + R) + fk * 0x1.7f7d1cp-20f)) - f);
 }
 
-/* expm1(r) for |r| <= ln2/2: Taylor series to r^9 in Horner form. */
+/* expm1(r) for |r| <= ln2/2: r + r^2 p(r) with p of degree 5 in Horner
+ * form; p's coefficients approximate the minimax fit of (expm1(r) - r) / r^2
+ * on that interval in relative error (2.4e-10), rounded to float. */
 static inline float em1_core(float r)
 {
-    float p = fmaf(r, 0x1.71de3ap-19f, 0x1.a01a02p-16f);
-    p = fmaf(r, p, 0x1.a01a02p-13f);
-    p = fmaf(r, p, 0x1.6c16c2p-10f);
-    p = fmaf(r, p, 0x1.111112p-7f);
-    p = fmaf(r, p, 0x1.555556p-5f);
-    p = fmaf(r, p, 0x1.555556p-3f);
+    float p = fmaf(r, 0x1.a032bcp-13f, 0x1.6d71e6p-10f);
+    p = fmaf(r, p, 0x1.11118ap-7f);
+    p = fmaf(r, p, 0x1.5554b2p-5f);
+    p = fmaf(r, p, 0x1.555554p-3f);
     p = fmaf(r, p, 0.5f);
     return fmaf(r * r, p, r);
 }
@@ -591,9 +591,10 @@ static void xyz_photon(const xyz_ctx* c, uint64_t pid, xyz_acc* a)
     block(c->key, pid, ev, 0u, b);
     float sleft = -nmco_logf(u01nz(b[0]));
     uint32_t events = 0;
-    /* Deposits of one voxel visit are summed in double and added to F when
+    /* Deposits of one voxel visit are summed in float (a visit is one to a
+     * few pieces) and added, in double, to F and to the absorbed total when
      * the photon leaves the voxel or ends. */
-    double vdep = 0.0;
+    float vdep = 0.0f;
     /* sensitivities: path length and scatterings per tracked tissue, and the
      * current visit's sums */
     float Ls[NMC_SENS_MAX] = {0};
@@ -602,13 +603,14 @@ static void xyz_photon(const xyz_ctx* c, uint64_t pid, xyz_acc* a)
     const int64_t nvox = (int64_t)c->Nx * c->Ny * c->Nz;
 #define FLUSH()                                                                        \
     do {                                                                               \
-        if (inside && a->F && (c->count_visits || vdep != 0.0))                        \
-            a->F[VIDX(ix, iy, iz)] += c->count_visits ? 1.0 : vdep;                     \
+        t->absorbed += (double)vdep;                                                   \
+        if (inside && a->F && (c->count_visits || vdep != 0.0f))                       \
+            a->F[VIDX(ix, iy, iz)] += c->count_visits ? 1.0 : (double)vdep;             \
         if (a->sens && inside) {                                                       \
-            vs.D = vdep;                                                               \
+            vs.D = (double)vdep;                                                       \
             xyz_sens_fold(c, tis, Ls, Ks, &vs, VIDX(ix, iy, iz), nvox, a->sens);        \
         }                                                                              \
-        vdep = 0.0;                                                                    \
+        vdep = 0.0f;                                                                   \
         vs.sp = vs.sq = vs.sza = vs.szs = 0.0f;                                        \
     } while (0)
 
@@ -659,8 +661,7 @@ static void xyz_photon(const xyz_ctx* c, uint64_t pid, xyz_acc* a)
         }
         W = W - dep;
         if (inside) {
-            vdep += (double)dep;
-            t->absorbed += (double)dep;
+            vdep = vdep + dep;
         } else {
             t->absorbed_outside += (double)dep;
         }
