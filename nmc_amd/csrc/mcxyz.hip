@@ -119,13 +119,27 @@ struct RecOut {
 // d/dmua_t = -L_t D and d/dmus_t = (K_t/mus_t - L_t) D for t != k, and the
 // visit's sums -sp + sza and sq/mus_k - sp - szs for t = k (the layered
 // walk's per-segment terms, summed over the segments in the voxel).
+// The voxel of a visit: its linear index, its coordinates relative to the
+// tile box's origin (below the box they wrap to large values, so one
+// unsigned test per axis places them) and whether it lies in the grid.  A
+// crossing takes it before it moves the photon's voxel indices on.
+struct VisitAt {
+  uint32_t lin, lx, ly, lz;
+  bool inside;
+};
+
+__device__ __forceinline__ VisitAt visit_at(const Photon& p, const TileBox& tb) {
+  return VisitAt{p.lin, (uint32_t)p.ix - tb.x0, (uint32_t)p.iy - tb.y0, (uint32_t)p.iz - tb.z0,
+                 p.inside};
+}
+
 template <int FMODE, typename TileT>
-__device__ __forceinline__ void flush_visit(Photon& p, const XyzArgs& a, const TileBox& tb,
-                                            TileT* tile, const RecOut& ro, double& absorbed,
-                                            const SensRows& sr) {
+__device__ __forceinline__ void flush_visit(Photon& p, const VisitAt& at, const XyzArgs& a,
+                                            const TileBox& tb, TileT* tile, const RecOut& ro,
+                                            double& absorbed, const SensRows& sr) {
   constexpr bool COUNT = FMODE == kFluCount;
   if constexpr (FMODE == kFluSens) {
-    if (p.inside && (p.vdep != 0.0f || p.sza != 0.0f)) {
+    if (at.inside && (p.vdep != 0.0f || p.sza != 0.0f)) {
       const size_t nvox = (size_t)a.Nx * (size_t)a.Ny * (size_t)a.Nz;
       const gptr<double> out = (gptr<double>)a.sens;
       for (int j = 0; j < a.sens_n; ++j) {
@@ -140,8 +154,8 @@ __device__ __forceinline__ void flush_visit(Photon& p, const XyzArgs& a, const T
           ga = -(double)Lj * (double)p.vdep;
           gs = ((double)Kj * (double)a.sens_inv_mus[j] - (double)Lj) * (double)p.vdep;
         }
-        global_add(out + ((size_t)j * nvox + p.lin), ga);
-        global_add(out + ((size_t)(a.sens_n + j) * nvox + p.lin), gs);
+        global_add(out + ((size_t)j * nvox + at.lin), ga);
+        global_add(out + ((size_t)(a.sens_n + j) * nvox + at.lin), gs);
       }
     }
     p.sp = p.sq = p.sza = p.szs = 0.0f;
@@ -151,13 +165,12 @@ __device__ __forceinline__ void flush_visit(Photon& p, const XyzArgs& a, const T
     p.vdep = 0.0f;
     return;
   }
-  if (p.inside && (COUNT || p.vdep != 0.0f)) {
+  if (at.inside && (COUNT || p.vdep != 0.0f)) {
     double v = COUNT ? 1.0 : (double)p.vdep;
-    uint32_t lx = (uint32_t)p.ix - tb.x0, ly = (uint32_t)p.iy - tb.y0, lz = (uint32_t)p.iz - tb.z0;
-    if (lx < tb.nx && ly < tb.ny && lz < tb.nz) {
-      unsafeAtomicAdd(&tile[(lz * tb.nx + lx) * tb.ny + ly], (TileT)v);
+    if (at.lx < tb.nx && at.ly < tb.ny && at.lz < tb.nz) {
+      unsafeAtomicAdd(&tile[(at.lz * tb.nx + at.lx) * tb.ny + at.ly], (TileT)v);
     } else {
-      const uint32_t k = p.lin;
+      const uint32_t k = at.lin;
       if (FMODE == kFluRecords) {
         RecSlice* sl = &ro.slices[k >> kBucketShift];
         // the slice's base and capacity are read before the cursor atomic (a
@@ -556,57 +569,70 @@ __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
       c_cross++;
       p.sleft = p.sleft - s * p.mus;
       if (!(p.sleft > 0.0f)) p.sleft = 0.0f;
-      if (kind == 1) {
-        // detector plane (kernels.metal:556-630)
-        p.z = zsurf;
-        lt.addi(TI_SURFACE_HITS);
-        const DetectorHit d = detector_plane(p.x, p.y, p.uz, p.W, p.ix, p.iy, a.aux);
-        if (d.where == 1) {
-          lt.addf(TF_RD, (double)d.wt);
-          lt.addi(TI_DETECTIONS);
-        } else if (d.where == 2) {
-          lt.addf(TF_RD_OUTSIDE, (double)d.wt);
-        }
-        p.uz = -p.uz;
-        p.ruz = -p.ruz;
-        p.W = p.W * d.rsp;
-        if (!(p.W > 0.0f)) alive = false;
-      } else {
+      // formed here, before a crossing loads the next tissue's row: the old
+      // mus is then dead and the row can go straight into its registers
+      p.sleft = __uint_as_float(fresh_vgpr(__float_as_uint(p.sleft)));
+      if (kind == 2) {
         // voxel face (kernels.metal:641-679).  The next voxel and its map
         // entry come first: the load is then in flight while the visit just
         // ended is flushed (whose LDS cursor atomic and record store would
         // otherwise sit between the crossing and the load).
-        // (selects, not a branch per axis, which a wave with lanes crossing
-        // along different axes would issue in turn)
+        // The indices move on in place (selects, not a branch per axis,
+        // which a wave with lanes crossing along different axes would issue
+        // in turn); the visit that ends keeps its voxel in `at`.
+        VisitAt at = visit_at(p, tb);
+        // materialised here, so that the old indices are dead before the new
+        // ones are formed and those can take the old ones' registers
+        at.lin = fresh_vgpr(at.lin);
+        at.lx = fresh_vgpr(at.lx);
+        at.ly = fresh_vgpr(at.ly);
+        at.lz = fresh_vgpr(at.lz);
         const bool fx = p.ux > 0.0f, fy = p.uy > 0.0f, fz = p.uz > 0.0f;
-        const int nix = p.ix + (axis == 0 ? (fx ? 1 : -1) : 0);
-        const int niy = p.iy + (axis == 1 ? (fy ? 1 : -1) : 0);
-        const int niz = p.iz + (axis == 2 ? (fz ? 1 : -1) : 0);
-        const uint32_t nlin = p.lin + (axis == 0   ? (fx ? ny_v : 0u - ny_v)
-                                       : axis == 1 ? (fy ? 1u : 0xFFFFFFFFu)
-                                                   : (fz ? nxy_v : 0u - nxy_v));
-        const bool nin = nix >= 0 && nix < gnx && niy >= 0 && niy < gny && niz >= 0 && niz < gnz;
-        const bool esc = !nin && (niz < 0 ? esc_top : esc_any) != 0u;
+        p.ix += axis == 0 ? (fx ? 1 : -1) : 0;
+        p.iy += axis == 1 ? (fy ? 1 : -1) : 0;
+        p.iz += axis == 2 ? (fz ? 1 : -1) : 0;
+        p.lin += axis == 0 ? (fx ? ny_v : 0u - ny_v)
+                 : axis == 1 ? (fy ? 1u : 0xFFFFFFFFu)
+                             : (fz ? nxy_v : 0u - nxy_v);
+        const bool nin = (uint32_t)p.ix < (uint32_t)gnx && (uint32_t)p.iy < (uint32_t)gny &&
+                         (uint32_t)p.iz < (uint32_t)gnz;
+        const bool esc = !nin && (p.iz < 0 ? esc_top : esc_any) != 0u;
         int ntis = 0;
         if (!esc)
-          ntis = vmap[nin ? nlin : vidx(a, clampi(nix, gnx), clampi(niy, gny), clampi(niz, gnz))];
-        flush_visit<FMODE>(p, a, tb, tile, ro, absorbed, sr);
-        p.ix = nix;
-        p.iy = niy;
-        p.iz = niz;
-        p.lin = nlin;
+          ntis = vmap[nin ? p.lin : vidx(a, clampi(p.ix, gnx), clampi(p.iy, gny), clampi(p.iz, gnz))];
+        flush_visit<FMODE>(p, at, a, tb, tile, ro, absorbed, sr);
         p.inside = nin;
+        // the new voxel's tissue row (row 0 for a photon that escapes: it
+        // ends here and reads none of it), loaded on every crossing lane so
+        // that it goes straight into the photon's registers
+        p.tis = ntis;
+        p.mua = T[ntis].mua;
+        p.mus = T[ntis].mus;
+        p.inv_mus = T[ntis].inv_mus;
+        if constexpr (SENS) p.slot = T[ntis].slot;
         if (esc) {
           tally_escape(lt, a, p.iz, p.W);
           alive = false;
-        } else {
-          p.tis = ntis;
-          p.mua = T[p.tis].mua;
-          p.mus = T[p.tis].mus;
-          p.inv_mus = T[p.tis].inv_mus;
-          if constexpr (SENS) p.slot = T[p.tis].slot;
         }
       }
+    }
+    if (kind == 1) {
+      // detector plane (kernels.metal:556-630), after the crossing's branch
+      // rather than beside it: the crossing is then the one path that moves
+      // the voxel indices, which lets them move on in their registers
+      p.z = zsurf;
+      lt.addi(TI_SURFACE_HITS);
+      const DetectorHit d = detector_plane(p.x, p.y, p.uz, p.W, p.ix, p.iy, a.aux);
+      if (d.where == 1) {
+        lt.addf(TF_RD, (double)d.wt);
+        lt.addi(TI_DETECTIONS);
+      } else if (d.where == 2) {
+        lt.addf(TF_RD_OUTSIDE, (double)d.wt);
+      }
+      p.uz = -p.uz;
+      p.ruz = -p.ruz;
+      p.W = p.W * d.rsp;
+      if (!(p.W > 0.0f)) alive = false;
     }
     if (alive && !(p.W > 0.0f)) alive = false;  // all weight absorbed
     if (alive && p.events >= a.max_events) {
@@ -652,7 +678,7 @@ __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
       }
     }
     if (!alive) {
-      flush_visit<FMODE>(p, a, tb, tile, ro, absorbed, sr);
+      flush_visit<FMODE>(p, visit_at(p, tb), a, tb, tile, ro, absorbed, sr);
       p.alive = false;
     }
   }
