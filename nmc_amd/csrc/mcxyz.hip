@@ -517,9 +517,11 @@ __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
       axis = 2;
       s_face = tz;
     }
-    float s_surf = kInf;
-    if (p.uz * dsign > 0.0f && (zsurf - p.z) * dsign > 0.0f)
-      s_surf = fmaxf((zsurf - p.z) * p.ruz, 0.0f);
+    // the detector plane, when the photon moves towards it on its detecting
+    // side: (zsurf - z) ruz is then positive (a difference of floats near
+    // zsurf is at least 2^-40 or so, and |ruz| >= 1, so it cannot underflow)
+    const float t_surf = (zsurf - p.z) * p.ruz;
+    const float s_surf = p.uz * dsign > 0.0f && t_surf > 0.0f ? t_surf : kInf;
     int kind;
     float s;
     if (s_int <= s_face && s_int <= s_surf) {
@@ -646,12 +648,12 @@ __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
       // the tissue-table reads of the phase function time to arrive
       const U4 nb = philox(p.pid, p.ev + 1u, 0u, a.key0, a.key1);
       const float nsl = -log_pos(u01nz(nb.x));
-      // SPIN (kernels.metal:695-730)
+      // SPIN (kernels.metal:695-730); tissue ids are below NMC_NTISS, which
+      // lets the row offset take a 24-bit multiply
+      __builtin_assume((uint32_t)p.tis < (uint32_t)NMC_NTISS);
       float ct = TWO_TERM ? tt_cos(T[p.tis].tt, u01(p.b1)) : hg_cos(T[p.tis].hg, u01(p.b1));
       spin(ct, u01(p.b2), p.ux, p.uy, p.uz);
-      p.rux = rcp_rn(p.ux);
-      p.ruy = rcp_rn(p.uy);
-      p.ruz = rcp_rn(p.uz);
+      rcp_rn3(p.ux, p.uy, p.uz, p.rux, p.ruy, p.ruz);
       c_int++;
       if constexpr (SENS) {
         if (p.slot >= 0) sr.K[p.slot * BLOCK]++;

@@ -31,6 +31,23 @@ __device__ __forceinline__ float rcp_rn(float x) {
   return rcp_rn_mid(x);
 }
 
+// rcp_rn of the three components of a direction, with one range test for
+// all three (the smallest and largest magnitude) instead of one each.
+__device__ __forceinline__ void rcp_rn3(float x, float y, float z, float& rx, float& ry,
+                                        float& rz) {
+  const float lo = fminf(fminf(fabsf(x), fabsf(y)), fabsf(z));
+  const float hi = fmaxf(fmaxf(fabsf(x), fabsf(y)), fabsf(z));
+  if (lo >= 0x1p-125f && hi <= 0x1p125f) {
+    rx = rcp_rn_mid(x);
+    ry = rcp_rn_mid(y);
+    rz = rcp_rn_mid(z);
+  } else {
+    rx = rcp_rn(x);
+    ry = rcp_rn(y);
+    rz = rcp_rn(z);
+  }
+}
+
 // a / b rounded to nearest, bit for bit what IEEE division gives, by
 // Markstein's correction (P. Markstein, IBM J. Res. Dev. 34(1):111-119, 1990):
 // with y = RN(1/b) (rcp_rn_mid) and q = RN(a y) within one ulp of a/b, the
@@ -316,7 +333,9 @@ __device__ __forceinline__ float hg_cos(const HGCoef& h, float u) {
     float t = div_rn(h.b, h.c + h.d * u);
     ct = (h.a - t * t) * h.e;
   }
-  return fminf(fmaxf(ct, -1.0f), 1.0f);
+  // clamp to [-1, 1] with one median-of-three (ct is never NaN here, where
+  // it equals fminf(fmaxf(ct, -1), 1))
+  return __builtin_amdgcn_fmed3f(ct, -1.0f, 1.0f);
 }
 
 // Two-term phase-function constants per tissue, derived on the host in double
