@@ -71,9 +71,17 @@ struct LPhoton {
   uint32_t K[NMC_MAX_LAYERS];
 };
 
+// Element i of an LDS table, its offset formed with a 24-bit multiply (full
+// rate; a plain index compiles to a 32-bit multiply at a quarter of it):
+// the tables hold at most 2048 stacks x NMC_MAX_LAYERS rows.
+template <typename T>
+__device__ __forceinline__ T& lds_at(T* base, int i) {
+  return *(T*)((char*)base + __umul24((uint32_t)i, (uint32_t)sizeof(T)));
+}
+
 template <bool SENS>
 __device__ __forceinline__ void load_layer(LPhoton& p, const LayRow* rows) {
-  const LayRow& r = rows[p.row0 + p.l];
+  const LayRow& r = lds_at(rows, p.row0 + p.l);
   p.zt = r.zt;
   p.zb = r.zb;
   p.mua = r.mua;
@@ -308,19 +316,20 @@ __global__ __launch_bounds__(kLayBlock) void layered_kernel(LayArgs a) {
     if (ballot(p.alive) == 0ull) break;
     if (!p.alive) continue;
 
-    nmc_totals* T = &st[p.item];
+    nmc_totals* T = &lds_at(st, p.item);
     // ---- HOP to the next interaction or interface ----
     float s_int = p.sleft * p.inv_mus;
-    float s_b = kInf;
-    if (p.uz > 0.0f) s_b = fmaxf((p.zb - p.z) * p.ruz, 0.0f);
-    else if (p.uz < 0.0f) s_b = fmaxf((p.zt - p.z) * p.ruz, 0.0f);
+    // the interface ahead (bottom going down, top going up), as selects
+    const float s_b = p.uz != 0.0f ? fmaxf(((p.uz > 0.0f ? p.zb : p.zt) - p.z) * p.ruz, 0.0f)
+                                   : kInf;
     bool inter = s_int <= s_b;
     float s = inter ? s_int : s_b;
     p.x = fmaf(s, p.ux, p.x);
     p.y = fmaf(s, p.uy, p.y);
     p.z = fmaf(s, p.uz, p.z);
     const float x = p.mua * s;
-    float em = expm1_f(-x);
+    // -x <= 0: the one-path expm1, equal to expm1_f there bit for bit
+    float em = expm1_neg(-x);
     float dep = -(p.W * em);
     if (SENS) sens_segment(p, dep, p.W, s, x, em);
     p.W = p.W - dep;
@@ -337,10 +346,10 @@ __global__ __launch_bounds__(kLayBlock) void layered_kernel(LayArgs a) {
       if (!(p.sleft > 0.0f)) p.sleft = 0.0f;
       bool up = p.uz < 0.0f;
       p.z = up ? p.zt : p.zb;
-      const LayStackDev& H = hdr[p.item];
-      float ni_ = rows[p.row0 + p.l].n;
-      float nt = up ? (p.l == 0 ? H.n_above : rows[p.row0 + p.l - 1].n)
-                    : (p.l == p.nl - 1 ? H.n_below : rows[p.row0 + p.l + 1].n);
+      const LayStackDev& H = lds_at(hdr, p.item);
+      float ni_ = lds_at(rows, p.row0 + p.l).n;
+      float nt = up ? (p.l == 0 ? H.n_above : lds_at(rows, p.row0 + p.l - 1).n)
+                    : (p.l == p.nl - 1 ? H.n_below : lds_at(rows, p.row0 + p.l + 1).n);
       float ca2;
       float r = fresnel(ni_, nt, fabsf(p.uz), &ca2);
       U4 aux = philox(p.pid, p.hits, 2u, a.key0, p.ks);
@@ -391,7 +400,7 @@ __global__ __launch_bounds__(kLayBlock) void layered_kernel(LayArgs a) {
       alive = false;
     }
     if (alive && !(p.sleft > 0.0f)) {
-      const LayRow& R = rows[p.row0 + p.l];
+      const LayRow& R = lds_at(rows, p.row0 + p.l);
       float ct = R.phase ? tt_cos(R.tt, u01(p.b1)) : hg_cos(R.hg, u01(p.b1));
       spin(ct, u01(p.b2), p.ux, p.uy, p.uz);
       p.ruz = rcp_rn(p.uz);
