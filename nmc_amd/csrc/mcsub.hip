@@ -67,6 +67,17 @@ __global__ __launch_bounds__(BLOCK) void mcsub_kernel(SubArgs a) {
   __syncthreads();
   const uint64_t first = a.state->first;
   const uint64_t total = prefix[ni];
+  // the bins' divisions r / dr and |z| / dz by Markstein's correction on a
+  // reciprocal formed once (IEEE division's result, in fewer instructions)
+  const bool dr_ok = a.dr >= 0x1p-60f && a.dr <= 0x1p60f;
+  const bool dz_ok = a.dz >= 0x1p-60f && a.dz <= 0x1p60f;
+  const float y_dr = dr_ok ? rcp_rn_mid(a.dr) : 0.0f;
+  const float y_dz = dz_ok ? rcp_rn_mid(a.dz) : 0.0f;
+  // bin offsets: with the tallies in LDS every index is below 2^24, so the
+  // products take 24-bit multiplies (full rate)
+  auto mul_idx = [](int u, int v) -> int {
+    return LDS ? (int)__umul24((uint32_t)u, (uint32_t)v) : u * v;
+  };
 
   const int lane = lane_id();
   float x = 0.0f, y = 0.0f, z = 0.0f, ux = 0.0f, uy = 0.0f, uz = 1.0f, W = 0.0f;
@@ -189,10 +200,10 @@ __global__ __launch_bounds__(BLOCK) void mcsub_kernel(SubArgs a) {
         float xe = fmaf(s2, ux, x0);
         float ye = fmaf(s2, uy, y0);
         float r = sqrtf(xe * xe + ye * ye);
-        int ir = rbin(r / a.dr, a.nr);
+        int ir = rbin(div_rn_by(r, a.dr, y_dr, dr_ok), a.nr);
         int ia = 0;
         while (ia < a.na - 1 && uz1 < cos_tab[ia]) ia++;
-        unsafeAtomicAdd(&tally[item * per_item + ir * a.na + ia], (double)W);
+        unsafeAtomicAdd(&tally[mul_idx(item, per_item) + mul_idx(ir, a.na) + ia], (double)W);
         addf(&tot[item], TF_RD, (double)W);
         addi(&tot[item], TI_DETECTIONS, 1);
         addi(&tot[item], TI_SUB_CROSS, 1);
@@ -208,9 +219,10 @@ __global__ __launch_bounds__(BLOCK) void mcsub_kernel(SubArgs a) {
       W = W - absorb;
       if (a.tally_rz) {
         float r = sqrtf(x * x + y * y);
-        int ir = rbin(r / a.dr, a.nr);
-        int iz = rbin(fabsf(z) / a.dz, a.nz);
-        unsafeAtomicAdd(&tally[item * per_item + n_ra + ir * (a.nz + 1) + iz], (double)absorb);
+        int ir = rbin(div_rn_by(r, a.dr, y_dr, dr_ok), a.nr);
+        int iz = rbin(div_rn_by(fabsf(z), a.dz, y_dz, dz_ok), a.nz);
+        unsafeAtomicAdd(&tally[mul_idx(item, per_item) + n_ra + mul_idx(ir, a.nz + 1) + iz],
+                        (double)absorb);
       }
       absorbed += (double)absorb;
       // SPIN (kernels.metal:243-279, HG honoured)

@@ -66,6 +66,19 @@ __device__ __forceinline__ float div_rn(float a, float b) {
   return fmaf(r, y, q);
 }
 
+// div_rn(a, b) with y = rcp_rn_mid(b) formed once for many dividends a:
+// taken for a = 0 or |a| in [2^-60, 2^60] when b_ok says |b| is in
+// [2^-60, 2^60] (the operand domain of div_rn's check on 2^32 pairs); any
+// other dividend takes the division.
+__device__ __forceinline__ float div_rn_by(float a, float b, float y, bool b_ok) {
+  const float aa = fabsf(a);
+  if (b_ok && (a == 0.0f || (aa >= 0x1p-60f && aa <= 0x1p60f))) {
+    const float q = a * y;
+    return fmaf(fmaf(-b, q, a), y, q);
+  }
+  return a / b;
+}
+
 // sqrt(x) rounded to nearest, bit for bit sqrtf, for x = 0 or x in
 // [2^-100, 2^126]: the hardware square root (within 1 ulp) corrected to the
 // nearest by the signs of the residuals x - s' s of its two neighbours s'
