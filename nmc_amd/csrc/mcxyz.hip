@@ -317,7 +317,7 @@ __device__ __noinline__ DetectorHit detector_plane(float x, float y, float uz, f
 // LAUNCH (kernels.metal:366-486).  Leaves p.alive false if the photon starts
 // outside a grid whose faces absorb it.
 __device__ __forceinline__ void launch(Photon& p, const XyzArgs& a, const XyzTissue* T,
-                                      const LoopTotals& lt) {
+                                      const float4* LT, const LoopTotals& lt) {
   const SourceState o = source_photon(p.pid, a.key0, a.key1, a.aux);
   if (o.spec != 0.0f) lt.addf(TF_SPECULAR, (double)o.spec);
   const float x = o.x, y = o.y, z = o.z, ux = o.ux, uy = o.uy, uz = o.uz, W = o.W;
@@ -344,7 +344,7 @@ __device__ __forceinline__ void launch(Photon& p, const XyzArgs& a, const XyzTis
   p.events = 0;
   U4 b = philox(p.pid, 1u, 0u, a.key0, a.key1);
   p.b1 = b.y; p.b2 = b.z; p.b3 = b.w;
-  p.sleft = -log_pos(u01nz(b.x));
+  p.sleft = -log_tab(u01nz(b.x), LT);
   p.alive = true;
 }
 
@@ -399,8 +399,13 @@ __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
   TileT* tile = (TileT*)(tile_mem + 2 * nb);  // [tile_nz][tile_nx][tile_ny]
   __shared__ XyzTissue T[NMC_NTISS];
   __shared__ nmc_totals st;
+  __shared__ float4 LT[128];  // log_tab's rows
   const int tile_n = a.tile_nx * a.tile_ny * a.tile_nz;
   for (int i = threadIdx.x; i < NMC_NTISS; i += BLOCK) T[i] = a.tissue[i];
+  if (threadIdx.x < 128) {
+    const float* row = kLogTable[threadIdx.x];
+    LT[threadIdx.x] = make_float4(row[0], row[1], row[2], row[3]);
+  }
   for (int i = threadIdx.x; i < tile_n; i += BLOCK) tile[i] = (TileT)0;
   for (int b = threadIdx.x; b < nb; b += BLOCK) {
     const uint32_t cap = a.rec_cap[b];
@@ -476,7 +481,7 @@ __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
         if (((idle >> lane) & 1ull) && rank < take) {
           p.pid = first + pool_next + rank;
           c_phot++;
-          launch(p, a, T, lt);
+          launch(p, a, T, LT, lt);
           if constexpr (SENS) {
             for (int j = 0; j < n_sens; ++j) {
               sr.L[j * BLOCK] = 0.0f;
@@ -647,7 +652,7 @@ __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
       // do not depend on the spin or the roulette; computed first, they give
       // the tissue-table reads of the phase function time to arrive
       const U4 nb = philox(p.pid, p.ev + 1u, 0u, a.key0, a.key1);
-      const float nsl = -log_pos(u01nz(nb.x));
+      const float nsl = -log_tab(u01nz(nb.x), LT);
       // SPIN (kernels.metal:695-730); tissue ids are below NMC_NTISS, which
       // lets the row offset take a 24-bit multiply
       __builtin_assume((uint32_t)p.tis < (uint32_t)NMC_NTISS);

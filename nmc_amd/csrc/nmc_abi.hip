@@ -339,7 +339,8 @@ int resolve_xyz_launch(const RunParams& rp, int sim, bool has_fluence, const nmc
     n_buckets = (int)nb;
   }
   const size_t elem = tile32 ? sizeof(float) : sizeof(double);
-  const size_t static_lds = sizeof(XyzTissue) * NMC_NTISS + sizeof(nmc_totals) + 256;
+  // the kernel's static LDS: tissue rows, totals, log_tab's 128 rows, slack
+  const size_t static_lds = sizeof(XyzTissue) * NMC_NTISS + sizeof(nmc_totals) + 128 * 16 + 256;
   const size_t rec_lds = (size_t)n_buckets * 16;
   const int64_t lds_budget =
       lds_max > static_lds + rec_lds ? (int64_t)((lds_max - static_lds - rec_lds) / elem) : 0;

@@ -9,6 +9,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "nmc_log_table.h"
+
 namespace nmc {
 
 constexpr float kInf = __builtin_inff();
@@ -192,6 +194,33 @@ __device__ __forceinline__ float log_pos(float x) {
   float hfsq = 0.5f * f * f;
   float fk = (float)k;
   return fk * 0x1.62e4p-1f - ((hfsq - (s * (hfsq + R) + fk * 0x1.7f7d1cp-20f)) - f);
+}
+
+// The 128 {rc, L_hi, L_lo, 0} rows of log_tab (definition in
+// gen_log_table.py); kernels copy it into LDS.
+static __constant__ float kLogTable[128][4] = NMC_LOG_TABLE;
+
+// log of a positive normal float by table (Tang's reduction): x = 2^k m' with
+// m' in [3/4, 3/2) (the mantissa, halved with k + 1 when it is >= 3/2); the
+// top seven mantissa bits pick row i, r = m' rc_i - 1 is exact (rc_i has 8
+// significant bits, |r| < 2^-7) and log(x) = (k ln2_hi + L_hi) + ((k ln2_lo +
+// L_lo) + log1p(r)), the first sum exact, log1p(r) = r + r^2 (-1/2 + r (1/3 -
+// r/4)).  Within 0.70 ulp of log on every uniform draw k 2^-24 and over
+// [e^-30, e^30], monotone on the draws; no division, one LDS row read: 19
+// instructions where log_pos takes 35.
+__device__ __forceinline__ float log_tab(float x, const float4* __restrict__ tab) {
+  const uint32_t b = __float_as_uint(x);
+  const uint32_t i = (b >> 16) & 0x7fu;
+  const bool up = i >= 64u;
+  const float m = __uint_as_float((b & 0x007fffffu) | (up ? 0x3f000000u : 0x3f800000u));
+  const float fk = (float)((int)(b >> 23) - (up ? 126 : 127));
+  const float4 t = tab[i];
+  const float r = fmaf(m, t.x, -1.0f);
+  const float p = fmaf(r, fmaf(r, -0x1p-2f, 0x1.555556p-2f), -0.5f);
+  const float q = fmaf(r * r, p, r);
+  const float hi = fmaf(fk, 0x1.62e4p-1f, t.y);
+  const float lo = fmaf(fk, 0x1.7f7d1cp-20f, t.z);
+  return hi + (lo + q);
 }
 
 // expm1(r), |r| <= ln2/2: r + r^2 p(r) with p of degree 5, Horner with fma;

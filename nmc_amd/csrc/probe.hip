@@ -12,7 +12,9 @@
 //   nmcp_sqrt_check   their short square root against sqrtf over every input
 //                     of its domain;
 //   nmcp_expm1_check  the one-path expm1 of the voxel walk against the
-//                     branched one over every input <= 0.
+//                     branched one over every input <= 0;
+//   nmcp_log_tab      the voxel walk's table log of every uniform draw
+//                     k 2^-24, k = 1..2^24, for comparison with the oracle.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -173,6 +175,20 @@ __global__ void __launch_bounds__(256) expm1_check_kernel(unsigned long long* ba
     }
   }
   if (cnt) atomicAdd(bad, cnt);
+}
+
+// out[k - 1] = log_tab(k 2^-24) for k = 1..2^24, the table read from LDS as
+// in the transport kernel.
+__global__ void __launch_bounds__(256) log_tab_kernel(float* out) {
+  __shared__ float4 lt[128];
+  if (threadIdx.x < 128) {
+    const float* row = nmc::kLogTable[threadIdx.x];
+    lt[threadIdx.x] = make_float4(row[0], row[1], row[2], row[3]);
+  }
+  __syncthreads();
+  const uint32_t n = 1u << 24;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    out[i] = nmc::log_tab((float)(i + 1u) * 0x1p-24f, lt);
 }
 
 }  // namespace
@@ -338,5 +354,21 @@ extern "C" int nmcp_expm1_check(unsigned long long* mismatches, unsigned int* fi
   }
   hipFree(d_bad);
   hipFree(d_first);
+  return rc;
+}
+
+// log_tab of every uniform draw k 2^-24, k = 1..2^24, into out[0..2^24)
+// (host memory); returns 0 on success.
+extern "C" int nmcp_log_tab(float* out) {
+  if (!out) return -1;
+  float* d = nullptr;
+  const size_t bytes = sizeof(float) << 24;
+  if (hipMalloc(&d, bytes) != hipSuccess) return -4;
+  int rc = -3;
+  hipLaunchKernelGGL(log_tab_kernel, dim3(kGrid), dim3(kBlock), 0, 0, d);
+  if (hipGetLastError() == hipSuccess && hipDeviceSynchronize() == hipSuccess &&
+      hipMemcpy(out, d, bytes, hipMemcpyDeviceToHost) == hipSuccess)
+    rc = 0;
+  hipFree(d);
   return rc;
 }

@@ -18,7 +18,9 @@
  *     easy as 1, 2, 3") keyed by (seed, stream) with counter (photon id,
  *     event index, tag) instead of RandomGen (helpers.metal:319-371).
  *   - log/exp/expm1/sin/cos are this file's own float routines (below) so
- *     that CPU and GPU produce identical trajectories; +,-,*,/,sqrt, fma are
+ *     that CPU and GPU produce identical trajectories (the voxel walk's step
+ *     log by the table of ../nmc_amd/csrc/nmc_log_table.h, whose rows the
+ *     tests recompute from their definition); +,-,*,/,sqrt, fma are
  *     IEEE-754 binary32 with round-to-nearest, no contraction.
  *   - Voxel stepping tracks integer voxel indices (a DDA walk) instead of the
  *     `ls` face nudge of kernels.metal:538 and SameVoxel (helpers.metal:27-39);
@@ -33,6 +35,7 @@
  *     and its weight is tallied separately.
  */
 #include "nmc_oracle.h"
+#include "../nmc_amd/csrc/nmc_log_table.h"
 
 #include <math.h>
 #include <stdlib.h>
@@ -133,6 +136,33 @@ float nmco_logf(float x)
     return fk * 0x1.62e4p-1f - ((hfsq - (s * (hfsq + R) + fk * 0x1.7f7d1cp-20f)) - f);
 }
 
+/*
+ * The voxel walk's step-length log, by table (Tang's reduction): x = 2^k m'
+ * with m' in [3/4, 3/2) (the mantissa, halved with k + 1 when >= 3/2); the top
+ * seven mantissa bits pick row i = (rc, L_hi, L_lo) of NMC_LOG_TABLE (built
+ * as gen_log_table.py states: rc = 1/midpoint with 8 significant bits, so
+ * r = m' rc - 1 is exact; -log(rc) split so that k ln2_hi + L_hi is exact);
+ * log(x) = (k ln2_hi + L_hi) + ((k ln2_lo + L_lo) + log1p(r)),
+ * log1p(r) = r + r^2 (-1/2 + r (1/3 - r/4)).  Positive normal x only.
+ */
+static const float log_table[128][4] = NMC_LOG_TABLE;
+
+float nmco_logf_tab(float x)
+{
+    uint32_t b = f_bits(x);
+    uint32_t i = (b >> 16) & 0x7fu;
+    int up = i >= 64u;
+    float m = bits_f((b & 0x007fffffu) | (up ? 0x3f000000u : 0x3f800000u));
+    float fk = (float)((int)(b >> 23) - (up ? 126 : 127));
+    const float* t = log_table[i];
+    float r = fmaf(m, t[0], -1.0f);
+    float p = fmaf(r, fmaf(r, -0x1p-2f, 0x1.555556p-2f), -0.5f);
+    float q = fmaf(r * r, p, r);
+    float hi = fmaf(fk, 0x1.62e4p-1f, t[1]);
+    float lo = fmaf(fk, 0x1.7f7d1cp-20f, t[2]);
+    return hi + (lo + q);
+}
+
 /* expm1(r) for |r| <= ln2/2: r + r^2 p(r) with p of degree 5 in Horner
  * form; p's coefficients approximate the minimax fit of (expm1(r) - r) / r^2
  * on that interval in relative error (2.4e-10), rounded to float. */
@@ -199,6 +229,7 @@ void nmco_sincos2pi(float u, float* s, float* c)
 }
 
 void nmco_logf_n(const float* x, float* y, int64_t n) { for (int64_t i = 0; i < n; ++i) y[i] = nmco_logf(x[i]); }
+void nmco_logf_tab_n(const float* x, float* y, int64_t n) { for (int64_t i = 0; i < n; ++i) y[i] = nmco_logf_tab(x[i]); }
 void nmco_expf_n(const float* x, float* y, int64_t n) { for (int64_t i = 0; i < n; ++i) y[i] = nmco_expf(x[i]); }
 void nmco_expm1f_n(const float* x, float* y, int64_t n) { for (int64_t i = 0; i < n; ++i) y[i] = nmco_expm1f(x[i]); }
 void nmco_sincos2pi_n(const float* u, float* s, float* c, int64_t n)
@@ -589,7 +620,7 @@ static void xyz_photon(const xyz_ctx* c, uint64_t pid, xyz_acc* a)
 
     uint32_t ev = 1;
     block(c->key, pid, ev, 0u, b);
-    float sleft = -nmco_logf(u01nz(b[0]));
+    float sleft = -nmco_logf_tab(u01nz(b[0]));
     uint32_t events = 0;
     /* Deposits of one voxel visit are summed in float (a visit is one to a
      * few pieces) and added, in double, to F and to the absorbed total when
@@ -737,7 +768,7 @@ static void xyz_photon(const xyz_ctx* c, uint64_t pid, xyz_acc* a)
         if (!roulette(&W, u01(b[3]), c->thr, c->chance, t)) { FLUSH(); return; }
         ev++;
         block(c->key, pid, ev, 0u, b);
-        sleft = -nmco_logf(u01nz(b[0]));
+        sleft = -nmco_logf_tab(u01nz(b[0]));
     }
 #undef FLUSH
 #undef VIDX

@@ -19,6 +19,7 @@ extern "C" {
 /* primitives (exported for the unit tests) */
 void nmco_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 float nmco_logf(float x);
+float nmco_logf_tab(float x);
 float nmco_expf(float x);
 float nmco_expm1f(float x);
 void nmco_sincos2pi(float u, float* s, float* c);
@@ -29,6 +30,7 @@ void nmco_spin(float cost, float u_psi, float* ux, float* uy, float* uz);
 
 /* vectorised helpers for the tests (n elements each) */
 void nmco_logf_n(const float* x, float* y, int64_t n);
+void nmco_logf_tab_n(const float* x, float* y, int64_t n);
 void nmco_expf_n(const float* x, float* y, int64_t n);
 void nmco_expm1f_n(const float* x, float* y, int64_t n);
 void nmco_sincos2pi_n(const float* u, float* s, float* c, int64_t n);

@@ -29,10 +29,10 @@ def build() -> str:
 def _bind(L):
     f32p = C.POINTER(C.c_float)
     L.nmco_philox4x32_10.argtypes = [C.POINTER(C.c_uint32)] * 3
-    for name in ("nmco_logf", "nmco_expf", "nmco_expm1f"):
+    for name in ("nmco_logf", "nmco_logf_tab", "nmco_expf", "nmco_expm1f"):
         getattr(L, name).argtypes = [C.c_float]
         getattr(L, name).restype = C.c_float
-    for name in ("nmco_logf_n", "nmco_expf_n", "nmco_expm1f_n"):
+    for name in ("nmco_logf_n", "nmco_logf_tab_n", "nmco_expf_n", "nmco_expm1f_n"):
         getattr(L, name).argtypes = [f32p, f32p, C.c_int64]
     L.nmco_sincos2pi_n.argtypes = [f32p, f32p, f32p, C.c_int64]
     L.nmco_fresnel.argtypes = [C.c_float, C.c_float, C.c_float, f32p]
@@ -141,6 +141,7 @@ def _vec(fn, x):
 
 
 def logf(x): return _vec("nmco_logf_n", x)
+def logf_tab(x): return _vec("nmco_logf_tab_n", x)
 def expf(x): return _vec("nmco_expf_n", x)
 def expm1f(x): return _vec("nmco_expm1f_n", x)
 

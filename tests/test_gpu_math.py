@@ -8,7 +8,8 @@ to the oracle's (which divides and takes square roots the IEEE way).
   with exponents in [-60, 60];
 - square root: every input in [2^-100, 2^126], and +0;
 - the voxel walk's one-path expm1 against the branched one: every input
-  in [-inf, -0]."""
+  in [-inf, -0];
+- the voxel walk's table log against the oracle's: every uniform draw."""
 from __future__ import annotations
 
 import ctypes as C
@@ -55,3 +56,20 @@ def test_expm1_neg_equals_expm1_for_all_nonpositive_inputs(gpu):
     bad, first = C.c_ulonglong(), C.c_uint()
     assert L.nmcp_expm1_check(C.byref(bad), C.byref(first)) == 0
     assert bad.value == 0, f"{bad.value} inputs differ, first 0x{first.value:08x}"
+
+
+def test_log_tab_equals_oracle_on_every_uniform_draw(gpu):
+    """The voxel walk's step-length log (table in LDS) gives the oracle's bits
+    for every uniform draw k 2^-24, k = 1..2^24."""
+    import numpy as np
+
+    from oracle import oracle as O
+
+    L = C.CDLL(PROBE_LIB)
+    L.nmcp_log_tab.argtypes = [C.c_void_p]
+    got = np.empty(1 << 24, np.float32)
+    assert L.nmcp_log_tab(got.ctypes.data) == 0
+    u = (np.arange(1, 2 ** 24 + 1, dtype=np.float64) * 2.0 ** -24).astype(np.float32)
+    ref = O.logf_tab(u)
+    bad = np.flatnonzero(got.view(np.uint32) != ref.view(np.uint32))
+    assert bad.size == 0, f"{bad.size} draws differ, first k = {bad[0] + 1}"

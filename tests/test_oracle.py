@@ -82,6 +82,52 @@ def test_logf_wide_range():
     assert ulp_err(O.logf(x), np.log(x.astype(np.float64))).max() < 1.0
 
 
+def test_logf_tab_all_uniform_draws():
+    """The voxel walk's table log: every uniform draw k 2^-24 within one ulp of
+    log, monotone, log(1) = 0 exactly."""
+    u = (np.arange(1, 2 ** 24 + 1, dtype=np.float64) * 2.0 ** -24).astype(np.float32)
+    y = O.logf_tab(u)
+    assert y[-1] == 0.0
+    assert ulp_err(y[:-1], np.log(u[:-1].astype(np.float64))).max() < 1.0
+    assert np.all(np.diff(y) >= 0)
+
+
+def test_logf_tab_wide_range_and_near_one():
+    rng = np.random.default_rng(3)
+    x = np.exp(rng.uniform(-30, 30, 400000)).astype(np.float32)
+    # every float in (1 - 2^-6, 1 + 2^-5): relative precision as log -> 0
+    lo, hi = np.float32(1 - 2 ** -6), np.float32(1 + 2 ** -5)
+    near = np.arange(lo.view(np.uint32), hi.view(np.uint32), dtype=np.uint32).view(np.float32)
+    near = near[near != 1.0]
+    for v in (x, near):
+        assert ulp_err(O.logf_tab(v), np.log(v.astype(np.float64))).max() < 1.0
+
+
+def test_log_table_rows_hold_their_properties():
+    """Rows of nmc_log_table.h as log_tab relies on them: rc with at most 8
+    significant bits and |m' rc - 1| < 2^-7 over the row's interval (so the
+    reduction is exact), L_hi a multiple of 2^-15, L_hi + L_lo = -log(rc) to
+    2^-40, rc = 1 and L = 0 on the six rows next to m' = 1."""
+    import re
+    text = open(os.path.join(HERE, "..", "nmc_amd", "csrc", "nmc_log_table.h")).read()
+    rows = re.findall(r"\{(-?0x[0-9a-fp.+-]+)f, (-?0x[0-9a-fp.+-]+)f, (-?0x[0-9a-fp.+-]+)f, 0\.0f\}",
+                      text)
+    assert len(rows) == 128
+    for i, (a, b, c) in enumerate(rows):
+        rc, lhi, llo = float.fromhex(a), float.fromhex(b), float.fromhex(c)
+        if i < 2 or i >= 124:
+            assert (rc, lhi, llo) == (1.0, 0.0, 0.0)
+            continue
+        m, _ = math.frexp(rc)
+        assert math.ldexp(m, 8) == int(math.ldexp(m, 8))  # <= 8 significant bits
+        lo = 1 + i / 128 if i < 64 else (1 + i / 128) / 2
+        hi = lo + (1 / 128 if i < 64 else 1 / 256)
+        assert max(abs(lo * rc - 1), abs(hi * rc - 1)) < 2 ** -7
+        assert lhi * 32768 == int(lhi * 32768)
+        assert abs(lhi + llo + math.log(rc)) < 2 ** -40
+        assert np.float32(llo) == llo
+
+
 def test_expf_and_expm1f():
     rng = np.random.default_rng(2)
     x = rng.uniform(-87, 0, 1_000_000).astype(np.float32)
