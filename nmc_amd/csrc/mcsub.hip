@@ -48,6 +48,7 @@ __global__ __launch_bounds__(BLOCK) void mcsub_kernel(SubArgs a) {
   // dynamic LDS: [items] media | [items+1] prefix | [items] totals | tallies (LDS mode)
   extern __shared__ double smem[];
   __shared__ float cos_tab[64];
+  __shared__ float4 LT[128];  // log_tab's rows
   const int ni = a.n_items;
   const int n_ra = (a.nr + 1) * a.na;
   const int n_rz = a.tally_rz ? (a.nr + 1) * (a.nz + 1) : 0;
@@ -61,6 +62,10 @@ __global__ __launch_bounds__(BLOCK) void mcsub_kernel(SubArgs a) {
   for (int i = threadIdx.x; i < ni; i += BLOCK) med[i] = a.items[i];
   for (int i = threadIdx.x; i <= ni; i += BLOCK) prefix[i] = a.prefix[i];
   for (int i = threadIdx.x; i < a.na - 1; i += BLOCK) cos_tab[i] = a.cos_tab[i];
+  if (threadIdx.x < 128) {
+    const float* row = kLogTable[threadIdx.x];
+    LT[threadIdx.x] = make_float4(row[0], row[1], row[2], row[3]);
+  }
   for (int i = threadIdx.x; i < ni * (TF_COUNT + TI_COUNT); i += BLOCK) ((double*)st)[i] = 0.0;
   if (LDS)
     for (int i = threadIdx.x; i < ni * per_item; i += BLOCK) tal[i] = 0.0;
@@ -183,7 +188,7 @@ __global__ __launch_bounds__(BLOCK) void mcsub_kernel(SubArgs a) {
 
     U4 b = philox(pid, ev, 0u, a.key0, ks);
     // HOP (kernels.metal:183-187)
-    float s = -log_pos(u01nz(b.x)) * m.inv_mut;
+    float s = -log_tab(u01nz(b.x), LT) * m.inv_mut;
     float x0 = x, y0 = y, z0 = z;
     x = fmaf(s, ux, x);
     y = fmaf(s, uy, y);

@@ -135,6 +135,9 @@ SubMedium make_medium(float mua, float mus, float g, float n1, float n2, uint32_
 }
 
 constexpr int kMaxBatchItems = 2048;
+// mcsub_kernel's static LDS: the exit-angle cosine table (256 B), log_tab's
+// 128 rows (2 KB), and slack
+constexpr size_t kSubStaticLds = 256 + 128 * 16 + 256;
 
 // A semi-infinite last layer that neither scatters nor absorbs would let a
 // photon travel forever.
@@ -760,10 +763,11 @@ int resolve_sub_tuning(nmc_handle* h) {
   int mode = r.mode;
   if (mode < 0 || mode > 2) return fail(NMC_ERR_ARG, "tuning.mode (tally placement) not in 0..2");
   bool want_lds = mode != 2;
-  size_t lds_full = mcsub_lds_bytes(h->n_items, per_item(h)) + 512;
+  // kSubStaticLds: the kernel's static LDS (cosine table, log_tab's rows)
+  size_t lds_full = mcsub_lds_bytes(h->n_items, per_item(h)) + kSubStaticLds;
   const bool sub_lds = want_lds && lds_full <= h->lds_max;
   const size_t lds = mcsub_lds_bytes(h->n_items, sub_lds ? per_item(h) : 0);
-  if (lds + 512 > h->lds_max)
+  if (lds + kSubStaticLds > h->lds_max)
     return fail(NMC_ERR_ARG, "%d work items need %zu B of LDS", h->n_items, lds);
   t.block = r.block ? r.block : 1024;
   if (t.block != 256 && t.block != 512 && t.block != 1024)
@@ -772,7 +776,7 @@ int resolve_sub_tuning(nmc_handle* h) {
   if (t.refill_min < 1 || t.refill_min > 64) return fail(NMC_ERR_ARG, "tuning.refill_min not in 1..64");
   t.chunk = r.chunk;
   t.mode = sub_lds ? 1 : 2;
-  HIP_TRY(mcsub_configure(t.block, h->lds_max - 512, sub_lds));
+  HIP_TRY(mcsub_configure(t.block, h->lds_max - kSubStaticLds, sub_lds));
   int occ = mcsub_blocks_per_cu(t.block, lds, sub_lds);
   if (occ <= 0) return fail(NMC_ERR_ARG, "mcsub kernel does not fit on a CU");
   t.blocks_per_cu = r.blocks_per_cu > 0 ? std::min(r.blocks_per_cu, occ) : occ;

@@ -964,7 +964,7 @@ static void sub_photon(const sub_ctx* c, uint64_t pid, sub_acc* a)
     for (;;) {
         block(c->key, pid, ev, 0u, b);
         /* HOP (kernels.metal:183-187) */
-        float s = -nmco_logf(u01nz(b[0])) * c->inv_mut;
+        float s = -nmco_logf_tab(u01nz(b[0])) * c->inv_mut;
         float x0 = x, y0 = y, z0 = z;
         x = fmaf(s, ux, x);
         y = fmaf(s, uy, y);
