@@ -200,6 +200,7 @@ __device__ __forceinline__ void flush_records(LPhoton& p, const LayArgs& a, int 
 template <bool RD_LDS, bool RECORDS, bool SENS>
 __global__ __launch_bounds__(kLayBlock) void layered_kernel(LayArgs a) {
   extern __shared__ double smem[];
+  __shared__ float4 LT[128];  // log_tab's rows
   // [items] absorbed[MAXL] | [items] totals | [items][nr+1] Rd (RD_LDS only) |
   // [items] headers | [items+1] prefix | [rows] | [items] sensitivities (SENS)
   const int ni = a.n_items;
@@ -223,6 +224,10 @@ __global__ __launch_bounds__(kLayBlock) void layered_kernel(LayArgs a) {
   for (int i = threadIdx.x; i < ni; i += kLayBlock) hdr[i] = a.stacks[i];
   for (int i = threadIdx.x; i <= ni; i += kLayBlock) prefix[i] = a.prefix[i];
   for (int i = threadIdx.x; i < a.n_rows; i += kLayBlock) rows[i] = a.rows[i];
+  if (threadIdx.x < 128) {
+    const float* row = kLogTable[threadIdx.x];
+    LT[threadIdx.x] = make_float4(row[0], row[1], row[2], row[3]);
+  }
   if (SENS)
     for (int i = threadIdx.x; i < ni * kSensPerStack; i += kLayBlock) sens[i] = 0.0;
   __syncthreads();
@@ -306,7 +311,7 @@ __global__ __launch_bounds__(kLayBlock) void layered_kernel(LayArgs a) {
           p.b1 = b.y;
           p.b2 = b.z;
           p.b3 = b.w;
-          p.sleft = -log_pos(u01nz(b.x));
+          p.sleft = -log_tab(u01nz(b.x), LT);
           p.alive = true;
         }
         pool_next += take;
@@ -424,7 +429,7 @@ __global__ __launch_bounds__(kLayBlock) void layered_kernel(LayArgs a) {
         p.b1 = b.y;
         p.b2 = b.z;
         p.b3 = b.w;
-        p.sleft = -log_pos(u01nz(b.x));
+        p.sleft = -log_tab(u01nz(b.x), LT);
       }
     }
     if (!alive) {

@@ -138,6 +138,8 @@ constexpr int kMaxBatchItems = 2048;
 // mcsub_kernel's static LDS: the exit-angle cosine table (256 B), log_tab's
 // 128 rows (2 KB), and slack
 constexpr size_t kSubStaticLds = 256 + 128 * 16 + 256;
+// layered_kernel's static LDS: log_tab's 128 rows, and slack
+constexpr size_t kLayStaticLds = 128 * 16 + 1024;
 
 // A semi-infinite last layer that neither scatters nor absorbs would let a
 // photon travel forever.
@@ -1695,7 +1697,7 @@ int nmc_layered_prepare(nmc_handle* h, const nmc_layered_config* cfg, const nmc_
   const bool sens = cfg->absorbed_grad != 0;
   h->sens_valid = false;
   h->lay_lds = layered_lds_bytes(n, h->lay_rows, 0, sens);
-  if (h->lay_lds + 1024 > h->lds_max)
+  if (h->lay_lds + kLayStaticLds > h->lds_max)
     return fail(NMC_ERR_ARG, "%d stacks / %d layers need %zu B of LDS (%zu available)", n, h->lay_rows,
                 h->lay_lds, h->lds_max);
   // Rd(r) goes to an LDS histogram when it fits without costing occupancy
@@ -1704,7 +1706,7 @@ int nmc_layered_prepare(nmc_handle* h, const nmc_layered_config* cfg, const nmc_
     return fail(NMC_ERR_ARG, "tuning.mode (Rd placement) not in 0..2");
   const int rd_len = n * (cfg->nr + 1);
   const size_t lds_rd = layered_lds_bytes(n, h->lay_rows, rd_len, sens);
-  const bool rd_fits = lds_rd + 1024 <= h->lds_max;
+  const bool rd_fits = lds_rd + kLayStaticLds <= h->lds_max;
   HIP_TRY(layered_configure(rd_fits ? lds_rd : h->lay_lds));
   int occ = layered_blocks_per_cu(h->lay_lds);
   if (occ <= 0) return fail(NMC_ERR_ARG, "layered kernel does not fit on a CU");

@@ -1213,7 +1213,7 @@ static void lay_photon(const lay_stack* S, const lay_cfg* c, uint64_t pid, lay_a
     lay_visit v = {0.0, 0.0f, 0.0f, 0.0f, 0.0f};
 #define SENS_FOLD() sens_fold(S, l, L, K, &v, a->sens)
     block(S->key, pid, ev, 0u, b);
-    float sleft = -nmco_logf(u01nz(b[0]));
+    float sleft = -nmco_logf_tab(u01nz(b[0]));
     for (;;) {
         float mua = S->mua[l], mus = S->mus[l];
         float s_int = sleft * S->inv_mus[l];
@@ -1316,7 +1316,7 @@ static void lay_photon(const lay_stack* S, const lay_cfg* c, uint64_t pid, lay_a
         if (!roulette(&W, u01(b[3]), c->thr, c->chance, t)) { SENS_FOLD(); return; }
         ev++;
         block(S->key, pid, ev, 0u, b);
-        sleft = -nmco_logf(u01nz(b[0]));
+        sleft = -nmco_logf_tab(u01nz(b[0]));
     }
 #undef SENS_FOLD
 }

@@ -112,8 +112,9 @@ def test_noalloc_rejections(gpu):
     w = W.skin_vessel()
     d = DeviceTallies(w)
     _say("[noalloc] rejections")
-    # an explicit tile whose block partials exceed the scratch array
-    t = P.Tuning.make(tile=(50, 50, 0, 100, 100, 2))
+    # an explicit tile whose block partials exceed the scratch array (10 000
+    # FP64 voxels fit the LDS; x one workgroup per CU is over 2 Mi doubles)
+    t = P.Tuning.make(tile=(50, 50, 0, 100, 100, 1))
     assert noalloc(w, d, 1000, tuning=t) == P.ERR_ARG
     assert b"scratch" in lib().nmc_last_error()
     # a voxel map with a tissue id the table does not have (checked on the device)
