@@ -201,6 +201,7 @@ template <bool RD_LDS, bool RECORDS, bool SENS>
 __global__ __launch_bounds__(kLayBlock) void layered_kernel(LayArgs a) {
   extern __shared__ double smem[];
   __shared__ float4 LT[128];  // log_tab's rows
+  __shared__ float2 SC[64];   // sincos2pi_bits' rows
   // [items] absorbed[MAXL] | [items] totals | [items][nr+1] Rd (RD_LDS only) |
   // [items] headers | [items+1] prefix | [rows] | [items] sensitivities (SENS)
   const int ni = a.n_items;
@@ -228,6 +229,7 @@ __global__ __launch_bounds__(kLayBlock) void layered_kernel(LayArgs a) {
     const float* row = kLogTable[threadIdx.x];
     LT[threadIdx.x] = make_float4(row[0], row[1], row[2], row[3]);
   }
+  if (threadIdx.x < 64) SC[threadIdx.x] = make_float2(kSinCosTable[threadIdx.x][0], kSinCosTable[threadIdx.x][1]);
   if (SENS)
     for (int i = threadIdx.x; i < ni * kSensPerStack; i += kLayBlock) sens[i] = 0.0;
   __syncthreads();
@@ -407,7 +409,7 @@ __global__ __launch_bounds__(kLayBlock) void layered_kernel(LayArgs a) {
     if (alive && !(p.sleft > 0.0f)) {
       const LayRow& R = lds_at(rows, p.row0 + p.l);
       float ct = R.phase ? tt_cos(R.tt, u01(p.b1)) : hg_cos(R.hg, u01(p.b1));
-      spin(ct, u01(p.b2), p.ux, p.uy, p.uz);
+      spin_w(ct, p.b2, SC, p.ux, p.uy, p.uz);
       p.ruz = rcp_rn(p.uz);
       p.Kcur++;
       p.n_int++;

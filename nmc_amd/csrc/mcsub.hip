@@ -49,6 +49,7 @@ __global__ __launch_bounds__(BLOCK) void mcsub_kernel(SubArgs a) {
   extern __shared__ double smem[];
   __shared__ float cos_tab[64];
   __shared__ float4 LT[128];  // log_tab's rows
+  __shared__ float2 SC[64];   // sincos2pi_bits' rows
   const int ni = a.n_items;
   const int n_ra = (a.nr + 1) * a.na;
   const int n_rz = a.tally_rz ? (a.nr + 1) * (a.nz + 1) : 0;
@@ -66,6 +67,7 @@ __global__ __launch_bounds__(BLOCK) void mcsub_kernel(SubArgs a) {
     const float* row = kLogTable[threadIdx.x];
     LT[threadIdx.x] = make_float4(row[0], row[1], row[2], row[3]);
   }
+  if (threadIdx.x < 64) SC[threadIdx.x] = make_float2(kSinCosTable[threadIdx.x][0], kSinCosTable[threadIdx.x][1]);
   for (int i = threadIdx.x; i < ni * (TF_COUNT + TI_COUNT); i += BLOCK) ((double*)st)[i] = 0.0;
   if (LDS)
     for (int i = threadIdx.x; i < ni * per_item; i += BLOCK) tal[i] = 0.0;
@@ -232,7 +234,7 @@ __global__ __launch_bounds__(BLOCK) void mcsub_kernel(SubArgs a) {
       absorbed += (double)absorb;
       // SPIN (kernels.metal:243-279, HG honoured)
       float ct = hg_cos(m.hg, u01(b.y));
-      spin(ct, u01(b.z), ux, uy, uz);
+      spin_w(ct, b.z, SC, ux, uy, uz);
       c_int++;
       // ROULETTE (kernels.metal:289-294)
       if (W < a.thr) {

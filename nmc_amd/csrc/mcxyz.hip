@@ -400,12 +400,14 @@ __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
   __shared__ XyzTissue T[NMC_NTISS];
   __shared__ nmc_totals st;
   __shared__ float4 LT[128];  // log_tab's rows
+  __shared__ float2 SC[64];   // sincos2pi_bits' rows
   const int tile_n = a.tile_nx * a.tile_ny * a.tile_nz;
   for (int i = threadIdx.x; i < NMC_NTISS; i += BLOCK) T[i] = a.tissue[i];
   if (threadIdx.x < 128) {
     const float* row = kLogTable[threadIdx.x];
     LT[threadIdx.x] = make_float4(row[0], row[1], row[2], row[3]);
   }
+  if (threadIdx.x < 64) SC[threadIdx.x] = make_float2(kSinCosTable[threadIdx.x][0], kSinCosTable[threadIdx.x][1]);
   for (int i = threadIdx.x; i < tile_n; i += BLOCK) tile[i] = (TileT)0;
   for (int b = threadIdx.x; b < nb; b += BLOCK) {
     const uint32_t cap = a.rec_cap[b];
@@ -657,7 +659,7 @@ __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
       // lets the row offset take a 24-bit multiply
       __builtin_assume((uint32_t)p.tis < (uint32_t)NMC_NTISS);
       float ct = TWO_TERM ? tt_cos(T[p.tis].tt, u01(p.b1)) : hg_cos(T[p.tis].hg, u01(p.b1));
-      spin(ct, u01(p.b2), p.ux, p.uy, p.uz);
+      spin_w(ct, p.b2, SC, p.ux, p.uy, p.uz);
       rcp_rn3(p.ux, p.uy, p.uz, p.rux, p.ruy, p.ruz);
       c_int++;
       if constexpr (SENS) {

@@ -136,10 +136,11 @@ SubMedium make_medium(float mua, float mus, float g, float n1, float n2, uint32_
 
 constexpr int kMaxBatchItems = 2048;
 // mcsub_kernel's static LDS: the exit-angle cosine table (256 B), log_tab's
-// 128 rows (2 KB), and slack
-constexpr size_t kSubStaticLds = 256 + 128 * 16 + 256;
-// layered_kernel's static LDS: log_tab's 128 rows, and slack
-constexpr size_t kLayStaticLds = 128 * 16 + 1024;
+// 128 rows (2 KB), sincos2pi_bits' 64 rows (512 B), and slack
+constexpr size_t kSubStaticLds = 256 + 128 * 16 + 64 * 8 + 256;
+// layered_kernel's static LDS: log_tab's 128 rows, sincos2pi_bits' 64 rows,
+// and slack
+constexpr size_t kLayStaticLds = 128 * 16 + 64 * 8 + 1024;
 
 // A semi-infinite last layer that neither scatters nor absorbs would let a
 // photon travel forever.
@@ -344,8 +345,10 @@ int resolve_xyz_launch(const RunParams& rp, int sim, bool has_fluence, const nmc
     n_buckets = (int)nb;
   }
   const size_t elem = tile32 ? sizeof(float) : sizeof(double);
-  // the kernel's static LDS: tissue rows, totals, log_tab's 128 rows, slack
-  const size_t static_lds = sizeof(XyzTissue) * NMC_NTISS + sizeof(nmc_totals) + 128 * 16 + 256;
+  // the kernel's static LDS: tissue rows, totals, log_tab's 128 rows,
+  // sincos2pi_bits' 64 rows, slack
+  const size_t static_lds =
+      sizeof(XyzTissue) * NMC_NTISS + sizeof(nmc_totals) + 128 * 16 + 64 * 8 + 256;
   const size_t rec_lds = (size_t)n_buckets * 16;
   const int64_t lds_budget =
       lds_max > static_lds + rec_lds ? (int64_t)((lds_max - static_lds - rec_lds) / elem) : 0;
@@ -1354,7 +1357,7 @@ int nmc_mcxyz_sens(nmc_handle* h, const int32_t* tissue_ids, int n_tissues) {
                   hipGetErrorString(e));
     e = hipMemset(buf, 0, bytes);
     if (e != hipSuccess) {
-      hipFree(buf);
+      (void)hipFree(buf);  // the error being reported is the memset's
       return fail(NMC_ERR_DEVICE, "sensitivity arrays: %s", hipGetErrorString(e));
     }
   }
@@ -1365,7 +1368,7 @@ int nmc_mcxyz_sens(nmc_handle* h, const int32_t* tissue_ids, int n_tissues) {
   for (int j = 0; j < n_tissues; ++j) tis[tissue_ids[j]].slot = j;
   hipError_t e = hipMemcpy(h->dTis, tis.data(), NMC_NTISS * sizeof(XyzTissue), hipMemcpyHostToDevice);
   if (e != hipSuccess) {
-    if (buf) hipFree(buf);
+    if (buf) (void)hipFree(buf);  // the error being reported is the copy's
     return fail(NMC_ERR_DEVICE, "tissue table: %s", hipGetErrorString(e));
   }
   // commit: the cached graph runs the previous variant

@@ -9,7 +9,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#include "nmc_log_table.h"
+#include "nmc_tables.h"
 
 namespace nmc {
 
@@ -197,7 +197,7 @@ __device__ __forceinline__ float log_pos(float x) {
 }
 
 // The 128 {rc, L_hi, L_lo, 0} rows of log_tab (definition in
-// gen_log_table.py); kernels copy it into LDS.
+// gen_tables.py); kernels copy it into LDS.
 static __constant__ float kLogTable[128][4] = NMC_LOG_TABLE;
 
 // log of a positive normal float by table (Tang's reduction): x = 2^k m' with
@@ -300,6 +300,28 @@ __device__ __forceinline__ void sincos2pi(float u, float* s, float* c) {
   float cc = (qi & 1) ? ps : pc;
   *s = (qi & 2) ? -ss : ss;
   *c = (qi == 1 || qi == 2) ? -cc : cc;
+}
+
+// The 64 {sin, cos}(2 pi j/64) rows of sincos2pi_bits (definition in
+// gen_tables.py); kernels copy them into LDS.
+static __constant__ float kSinCosTable[64][2] = NMC_SINCOS_TABLE;
+
+// sin(2 pi u), cos(2 pi u) of the uniform u = (x >> 8) 2^-24 (u01's mapping)
+// straight from the random word x: u = j/64 + f with j the top six bits of x
+// and f = ((x >> 8) mod 2^18) 2^-24 (exact), a = 2 pi f < 0.0982, and the
+// angle sum with row j: s = s_j cos a + c_j sin a, c = c_j cos a - s_j sin a,
+// sin a = a + a^3 (-1/6 + a^2/120), cos a = 1 + a^2 (-1/2 + a^2/24).  Within
+// 1.2e-7 of sin and cos, exact at the quarter turns; 16 instructions and one
+// LDS read where sincos2pi(u01(x)) takes 28.
+__device__ __forceinline__ void sincos2pi_bits(uint32_t x, const float2* __restrict__ sc, float& s,
+                                               float& c) {
+  const float2 t = sc[x >> 26];
+  const float a = (float)((x >> 8) & 0x3ffffu) * 0x1.921fb6p-22f;
+  const float a2 = a * a;
+  const float sa = fmaf(a2 * a, fmaf(a2, 0x1.111112p-7f, -0x1.555556p-3f), a);
+  const float ca = fmaf(a2, fmaf(a2, 0x1.555556p-5f, -0.5f), 1.0f);
+  s = fmaf(t.x, ca, t.y * sa);
+  c = fmaf(t.y, ca, -(t.x * sa));
 }
 
 // ---------------------------------------------------------------------------
@@ -411,15 +433,15 @@ __device__ __forceinline__ float tt_cos(const TTCoef& t, float u) {
   return fminf(fmaxf(ct, -1.0f), 1.0f);
 }
 
-// Rotate direction (ux,uy,uz) by polar cosine `cost` and azimuth 2*pi*u_psi.
-// The pole formula is taken where the reference takes it: its test
+// Rotate direction (ux,uy,uz) by polar cosine `cost` and azimuth psi given
+// by sp = sin(psi), cp = cos(psi) (spin: psi = 2 pi u_psi; spin_w: psi from
+// the random word as sincos2pi_bits takes it).  The pole formula is taken where the reference takes it: its test
 // 1 - |uz| <= 1e-12 in float arithmetic (kernels.metal:713) holds exactly
 // when |uz| >= 1.  It is also taken when ux^2 + uy^2 < 2^-100 (only a
 // direction that is not quite unit length gets there with |uz| < 1), which
 // keeps the general formula's quotient below 2^100.
-__device__ __forceinline__ void spin(float cost, float u_psi, float& ux, float& uy, float& uz) {
-  float sp, cp;
-  sincos2pi(u_psi, &sp, &cp);
+__device__ __forceinline__ void spin_sc(float cost, float sp, float cp, float& ux, float& uy,
+                                        float& uz) {
   float x = ux, y = uy, z = uz;
   float r2 = x * x + y * y;
   if (fabsf(z) >= 1.0f || !(r2 >= 0x1p-100f)) {
@@ -440,6 +462,19 @@ __device__ __forceinline__ void spin(float cost, float u_psi, float& ux, float& 
     uy = st * (y * z * cp + x * sp) + y * cost;
     uz = -(st * cp * r2) + z * cost;
   }
+}
+
+__device__ __forceinline__ void spin(float cost, float u_psi, float& ux, float& uy, float& uz) {
+  float sp, cp;
+  sincos2pi(u_psi, &sp, &cp);
+  spin_sc(cost, sp, cp, ux, uy, uz);
+}
+
+__device__ __forceinline__ void spin_w(float cost, uint32_t w_psi, const float2* __restrict__ sc,
+                                       float& ux, float& uy, float& uz) {
+  float sp, cp;
+  sincos2pi_bits(w_psi, sc, sp, cp);
+  spin_sc(cost, sp, cp, ux, uy, uz);
 }
 
 // A wave-uniform value kept in a vector register.  The transport loops use

@@ -13,8 +13,10 @@
 //                     of its domain;
 //   nmcp_expm1_check  the one-path expm1 of the voxel walk against the
 //                     branched one over every input <= 0;
-//   nmcp_log_tab      the voxel walk's table log of every uniform draw
-//                     k 2^-24, k = 1..2^24, for comparison with the oracle.
+//   nmcp_log_tab      the walks' table log of every uniform draw k 2^-24,
+//                     k = 1..2^24, for comparison with the oracle;
+//   nmcp_sincos_tab   the walks' table azimuth sine and cosine of every
+//                     random word's top 24 bits, for the same.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -189,6 +191,22 @@ __global__ void __launch_bounds__(256) log_tab_kernel(float* out) {
   const uint32_t n = 1u << 24;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     out[i] = nmc::log_tab((float)(i + 1u) * 0x1p-24f, lt);
+}
+
+// s[k], c[k] = sincos2pi_bits(k << 8 | 0x5A) for k < 2^24, the table read
+// from LDS as in the transport kernels.
+__global__ void __launch_bounds__(256) sincos_tab_kernel(float* s, float* c) {
+  __shared__ float2 sc[64];
+  if (threadIdx.x < 64)
+    sc[threadIdx.x] = make_float2(nmc::kSinCosTable[threadIdx.x][0], nmc::kSinCosTable[threadIdx.x][1]);
+  __syncthreads();
+  const uint32_t n = 1u << 24;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    float si, ci;
+    nmc::sincos2pi_bits(i << 8 | 0x5Au, sc, si, ci);
+    s[i] = si;
+    c[i] = ci;
+  }
 }
 
 }  // namespace
@@ -368,6 +386,23 @@ extern "C" int nmcp_log_tab(float* out) {
   hipLaunchKernelGGL(log_tab_kernel, dim3(kGrid), dim3(kBlock), 0, 0, d);
   if (hipGetLastError() == hipSuccess && hipDeviceSynchronize() == hipSuccess &&
       hipMemcpy(out, d, bytes, hipMemcpyDeviceToHost) == hipSuccess)
+    rc = 0;
+  hipFree(d);
+  return rc;
+}
+
+// sincos2pi_bits of every word k << 8 | 0x5A, k < 2^24, into s[k] and c[k]
+// (host memory, 2^24 floats each); returns 0 on success.
+extern "C" int nmcp_sincos_tab(float* s, float* c) {
+  if (!s || !c) return -1;
+  float* d = nullptr;
+  const size_t bytes = sizeof(float) << 24;
+  if (hipMalloc(&d, 2 * bytes) != hipSuccess) return -4;
+  int rc = -3;
+  hipLaunchKernelGGL(sincos_tab_kernel, dim3(kGrid), dim3(kBlock), 0, 0, d, d + (1u << 24));
+  if (hipGetLastError() == hipSuccess && hipDeviceSynchronize() == hipSuccess &&
+      hipMemcpy(s, d, bytes, hipMemcpyDeviceToHost) == hipSuccess &&
+      hipMemcpy(c, d + (1u << 24), bytes, hipMemcpyDeviceToHost) == hipSuccess)
     rc = 0;
   hipFree(d);
   return rc;

@@ -19,7 +19,7 @@
  *     event index, tag) instead of RandomGen (helpers.metal:319-371).
  *   - log/exp/expm1/sin/cos are this file's own float routines (below) so
  *     that CPU and GPU produce identical trajectories (the voxel walk's step
- *     log by the table of ../nmc_amd/csrc/nmc_log_table.h, whose rows the
+ *     log by the table of ../nmc_amd/csrc/nmc_tables.h, whose rows the
  *     tests recompute from their definition); +,-,*,/,sqrt, fma are
  *     IEEE-754 binary32 with round-to-nearest, no contraction.
  *   - Voxel stepping tracks integer voxel indices (a DDA walk) instead of the
@@ -35,7 +35,7 @@
  *     and its weight is tallied separately.
  */
 #include "nmc_oracle.h"
-#include "../nmc_amd/csrc/nmc_log_table.h"
+#include "../nmc_amd/csrc/nmc_tables.h"
 
 #include <math.h>
 #include <stdlib.h>
@@ -140,7 +140,7 @@ float nmco_logf(float x)
  * The voxel walk's step-length log, by table (Tang's reduction): x = 2^k m'
  * with m' in [3/4, 3/2) (the mantissa, halved with k + 1 when >= 3/2); the top
  * seven mantissa bits pick row i = (rc, L_hi, L_lo) of NMC_LOG_TABLE (built
- * as gen_log_table.py states: rc = 1/midpoint with 8 significant bits, so
+ * as gen_tables.py states: rc = 1/midpoint with 8 significant bits, so
  * r = m' rc - 1 is exact; -log(rc) split so that k ln2_hi + L_hi is exact);
  * log(x) = (k ln2_hi + L_hi) + ((k ln2_lo + L_lo) + log1p(r)),
  * log1p(r) = r + r^2 (-1/2 + r (1/3 - r/4)).  Positive normal x only.
@@ -380,15 +380,41 @@ float nmco_tt_cos(float gf, float af, float gb, float ab, float cc, float u)
     return tt_cos(&t, u);
 }
 
-/* New direction after deflection cos(theta) = cost and azimuth 2*pi*u_psi
- * (kernels.metal:705-730; sin(psi) is computed directly).  Pole formula where
- * the reference's float test 1 - |uz| <= 1e-12 (kernels.metal:713) holds,
- * i.e. |uz| >= 1, or where ux^2 + uy^2 < 2^-100 (a direction slightly off
- * unit length), which bounds the general formula's quotient. */
-void nmco_spin(float cost, float u_psi, float* ux, float* uy, float* uz)
+/*
+ * sin(2 pi u), cos(2 pi u) of the uniform u = (x >> 8) 2^-24 of random word x
+ * (the walks' azimuth): u = j/64 + f, j = x >> 26, f = ((x >> 8) mod 2^18)
+ * 2^-24 (exact), a = 2 pi f < 0.0982; with row j = (sin, cos)(2 pi j/64) of
+ * NMC_SINCOS_TABLE (double values rounded to float, quarter turns exact):
+ * s = s_j cos a + c_j sin a, c = c_j cos a - s_j sin a, sin a = a + a^3 (-1/6
+ * + a^2/120), cos a = 1 + a^2 (-1/2 + a^2/24).
+ */
+static const float sincos_table[64][2] = NMC_SINCOS_TABLE;
+
+void nmco_sincos2pi_bits(uint32_t x, float* s, float* c)
 {
-    float sp, cp;
-    nmco_sincos2pi(u_psi, &sp, &cp);
+    const float* t = sincos_table[x >> 26];
+    float a = (float)((x >> 8) & 0x3ffffu) * 0x1.921fb6p-22f;
+    float a2 = a * a;
+    float sa = fmaf(a2 * a, fmaf(a2, 0x1.111112p-7f, -0x1.555556p-3f), a);
+    float ca = fmaf(a2, fmaf(a2, 0x1.555556p-5f, -0.5f), 1.0f);
+    *s = fmaf(t[0], ca, t[1] * sa);
+    *c = fmaf(t[1], ca, -(t[0] * sa));
+}
+
+void nmco_sincos2pi_bits_n(const uint32_t* x, float* s, float* c, int64_t n)
+{
+    for (int64_t i = 0; i < n; ++i) nmco_sincos2pi_bits(x[i], s + i, c + i);
+}
+
+/* New direction after deflection cos(theta) = cost and azimuth psi given by
+ * sp = sin(psi), cp = cos(psi) (kernels.metal:705-730; sin(psi) is computed
+ * directly).  Pole formula where the reference's float test 1 - |uz| <= 1e-12
+ * (kernels.metal:713) holds, i.e. |uz| >= 1, or where ux^2 + uy^2 < 2^-100 (a
+ * direction slightly off unit length), which bounds the general formula's
+ * quotient.  nmco_spin: psi = 2 pi u_psi; nmco_spin_w (the walks): psi from
+ * random word w_psi by nmco_sincos2pi_bits. */
+static void spin_sc(float cost, float sp, float cp, float* ux, float* uy, float* uz)
+{
     float x = *ux, y = *uy, z = *uz;
     float r2 = x * x + y * y;
     if (fabsf(z) >= 1.0f || !(r2 >= 0x1p-100f)) {
@@ -404,6 +430,20 @@ void nmco_spin(float cost, float u_psi, float* ux, float* uy, float* uz)
         *uy = st * (y * z * cp + x * sp) + y * cost;
         *uz = -(st * cp * r2) + z * cost;
     }
+}
+
+void nmco_spin(float cost, float u_psi, float* ux, float* uy, float* uz)
+{
+    float sp, cp;
+    nmco_sincos2pi(u_psi, &sp, &cp);
+    spin_sc(cost, sp, cp, ux, uy, uz);
+}
+
+void nmco_spin_w(float cost, uint32_t w_psi, float* ux, float* uy, float* uz)
+{
+    float sp, cp;
+    nmco_sincos2pi_bits(w_psi, &sp, &cp);
+    spin_sc(cost, sp, cp, ux, uy, uz);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -760,7 +800,7 @@ static void xyz_photon(const xyz_ctx* c, uint64_t pid, xyz_acc* a)
 
         /* SPIN (kernels.metal:695-730) */
         float ct = c->two_term ? tt_cos(&c->tt[tis], u01(b[1])) : hg_sample(&c->hg[tis], u01(b[1]));
-        nmco_spin(ct, u01(b[2]), &ux, &uy, &uz);
+        nmco_spin_w(ct, b[2], &ux, &uy, &uz);
         rux = 1.0f / ux; ruy = 1.0f / uy; ruz = 1.0f / uz;
         t->interactions++;
         if (a->sens && c->slot[tis] >= 0) Ks[c->slot[tis]]++;
@@ -1013,7 +1053,7 @@ static void sub_photon(const sub_ctx* c, uint64_t pid, sub_acc* a)
         t->absorbed += (double)absorb;
         /* SPIN (kernels.metal:243-279, with g honoured) */
         float ct = hg_sample(&c->hg, u01(b[1]));
-        nmco_spin(ct, u01(b[2]), &ux, &uy, &uz);
+        nmco_spin_w(ct, b[2], &ux, &uy, &uz);
         t->interactions++;
         if (!roulette(&W, u01(b[3]), c->thr, c->chance, t)) return;
         if (events >= c->max_events) {
@@ -1309,7 +1349,7 @@ static void lay_photon(const lay_stack* S, const lay_cfg* c, uint64_t pid, lay_a
         }
         if (sleft > 0.0f) continue;
         float ct = S->phase[l] ? tt_cos(&S->tt[l], u01(b[1])) : hg_sample(&S->hg[l], u01(b[1]));
-        nmco_spin(ct, u01(b[2]), &ux, &uy, &uz);
+        nmco_spin_w(ct, b[2], &ux, &uy, &uz);
         ruz = 1.0f / uz;
         K[l]++;
         t->interactions++;

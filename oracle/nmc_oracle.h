@@ -27,6 +27,9 @@ float nmco_fresnel(float n1, float n2, float ca1, float* ca2);
 float nmco_hg_cos(float g, float u);
 float nmco_tt_cos(float gf, float af, float gb, float ab, float cc, float u);
 void nmco_spin(float cost, float u_psi, float* ux, float* uy, float* uz);
+void nmco_spin_w(float cost, uint32_t w_psi, float* ux, float* uy, float* uz);
+void nmco_sincos2pi_bits(uint32_t x, float* s, float* c);
+void nmco_sincos2pi_bits_n(const uint32_t* x, float* s, float* c, int64_t n);
 
 /* vectorised helpers for the tests (n elements each) */
 void nmco_logf_n(const float* x, float* y, int64_t n);

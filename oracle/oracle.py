@@ -35,6 +35,7 @@ def _bind(L):
     for name in ("nmco_logf_n", "nmco_logf_tab_n", "nmco_expf_n", "nmco_expm1f_n"):
         getattr(L, name).argtypes = [f32p, f32p, C.c_int64]
     L.nmco_sincos2pi_n.argtypes = [f32p, f32p, f32p, C.c_int64]
+    L.nmco_sincos2pi_bits_n.argtypes = [C.POINTER(C.c_uint32), f32p, f32p, C.c_int64]
     L.nmco_fresnel.argtypes = [C.c_float, C.c_float, C.c_float, f32p]
     L.nmco_fresnel.restype = C.c_float
     L.nmco_hg_cos.argtypes = [C.c_float, C.c_float]
@@ -152,6 +153,16 @@ def sincos2pi(u):
     c = np.empty_like(u)
     fp = C.POINTER(C.c_float)
     lib().nmco_sincos2pi_n(u.ctypes.data_as(fp), s.ctypes.data_as(fp), c.ctypes.data_as(fp), u.size)
+    return s, c
+
+
+def sincos2pi_bits(x):
+    x = np.ascontiguousarray(x, np.uint32)
+    s = np.empty(x.shape, np.float32)
+    c = np.empty(x.shape, np.float32)
+    fp = C.POINTER(C.c_float)
+    lib().nmco_sincos2pi_bits_n(x.ctypes.data_as(C.POINTER(C.c_uint32)), s.ctypes.data_as(fp),
+                                c.ctypes.data_as(fp), x.size)
     return s, c
 
 

@@ -9,7 +9,8 @@ to the oracle's (which divides and takes square roots the IEEE way).
 - square root: every input in [2^-100, 2^126], and +0;
 - the voxel walk's one-path expm1 against the branched one: every input
   in [-inf, -0];
-- the voxel walk's table log against the oracle's: every uniform draw."""
+- the walks' table log and table azimuth sine and cosine against the
+  oracle's: every uniform draw."""
 from __future__ import annotations
 
 import ctypes as C
@@ -73,3 +74,22 @@ def test_log_tab_equals_oracle_on_every_uniform_draw(gpu):
     ref = O.logf_tab(u)
     bad = np.flatnonzero(got.view(np.uint32) != ref.view(np.uint32))
     assert bad.size == 0, f"{bad.size} draws differ, first k = {bad[0] + 1}"
+
+
+def test_sincos_tab_equals_oracle_on_every_draw(gpu):
+    """The walks' azimuth sine and cosine (table in LDS) give the oracle's bits
+    for every random word's top 24 bits."""
+    import numpy as np
+
+    from oracle import oracle as O
+
+    L = C.CDLL(PROBE_LIB)
+    L.nmcp_sincos_tab.argtypes = [C.c_void_p, C.c_void_p]
+    s = np.empty(1 << 24, np.float32)
+    c = np.empty(1 << 24, np.float32)
+    assert L.nmcp_sincos_tab(s.ctypes.data, c.ctypes.data) == 0
+    k = np.arange(0, 1 << 24, dtype=np.uint32)
+    rs, rc = O.sincos2pi_bits(k << np.uint32(8) | np.uint32(0x5A))
+    for got, ref, name in ((s, rs, "sin"), (c, rc, "cos")):
+        bad = np.flatnonzero(got.view(np.uint32) != ref.view(np.uint32))
+        assert bad.size == 0, f"{name}: {bad.size} words differ, first k = {bad[0]}"

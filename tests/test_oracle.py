@@ -104,12 +104,12 @@ def test_logf_tab_wide_range_and_near_one():
 
 
 def test_log_table_rows_hold_their_properties():
-    """Rows of nmc_log_table.h as log_tab relies on them: rc with at most 8
+    """Rows of nmc_tables.h as log_tab relies on them: rc with at most 8
     significant bits and |m' rc - 1| < 2^-7 over the row's interval (so the
     reduction is exact), L_hi a multiple of 2^-15, L_hi + L_lo = -log(rc) to
     2^-40, rc = 1 and L = 0 on the six rows next to m' = 1."""
     import re
-    text = open(os.path.join(HERE, "..", "nmc_amd", "csrc", "nmc_log_table.h")).read()
+    text = open(os.path.join(HERE, "..", "nmc_amd", "csrc", "nmc_tables.h")).read()
     rows = re.findall(r"\{(-?0x[0-9a-fp.+-]+)f, (-?0x[0-9a-fp.+-]+)f, (-?0x[0-9a-fp.+-]+)f, 0\.0f\}",
                       text)
     assert len(rows) == 128
@@ -151,6 +151,28 @@ def test_sincos2pi_all_uniform_draws():
     s1, c1 = O.sincos2pi(np.array([0.0, 0.25, 0.5, 0.75, 1.0], np.float32))
     np.testing.assert_array_equal(s1, [0, 1, 0, -1, 0])
     np.testing.assert_array_equal(c1, [1, 0, -1, 0, 1])
+
+
+def test_sincos2pi_bits_every_draw():
+    """The walks' azimuth from the random word: every top-24-bit value (the
+    low byte does not enter) within 1.2e-7 of sin and cos of 2 pi u, u = (x >>
+    8) 2^-24; s^2 + c^2 = 1 to 3e-7; exact at the quarter turns; the table rows
+    are sin and cos of 2 pi j/64 rounded to float."""
+    k = np.arange(0, 2 ** 24, dtype=np.uint32)
+    s, c = O.sincos2pi_bits(k << np.uint32(8) | np.uint32(0x5A))
+    a = 2 * np.pi * (k.astype(np.float64) * 2.0 ** -24)
+    assert np.abs(s - np.sin(a)).max() < 1.2e-7
+    assert np.abs(c - np.cos(a)).max() < 1.2e-7
+    assert np.abs(s.astype(np.float64) ** 2 + c.astype(np.float64) ** 2 - 1).max() < 3e-7
+    q = np.array([0, 1 << 30, 2 << 30, 3 << 30], np.uint32)
+    sq, cq = O.sincos2pi_bits(q)
+    np.testing.assert_array_equal(sq, [0, 1, 0, -1])
+    np.testing.assert_array_equal(cq, [1, 0, -1, 0])
+    # row j is (sin, cos)(2 pi j/64): the draws with f = 0
+    j = np.arange(64, dtype=np.uint32)
+    sj, cj = O.sincos2pi_bits(j << np.uint32(26))
+    assert np.abs(sj - np.sin(2 * np.pi * j / 64)).max() <= 3e-8
+    assert np.abs(cj - np.cos(2 * np.pi * j / 64)).max() <= 3e-8
 
 
 # ------------------------------------------------------------------ optics ---
