@@ -65,6 +65,16 @@ __device__ __forceinline__ uint32_t vidx(const XyzArgs& a, int ix, int iy, int i
   return ((uint32_t)iz * (uint32_t)a.Nx + (uint32_t)ix) * (uint32_t)a.Ny + (uint32_t)iy;
 }
 
+// The photon's tissue becomes `tis`: its id and the optical row it reads.
+template <bool SENS>
+__device__ __forceinline__ void take_tissue_t(Photon& p, const XyzTissue* T, int tis) {
+  p.tis = tis;
+  p.mua = T[tis].mua;
+  p.mus = T[tis].mus;
+  p.inv_mus = T[tis].inv_mus;
+  if constexpr (SENS) p.slot = T[tis].slot;
+}
+
 __device__ __forceinline__ void lds_addi(nmc_totals* t, int k, unsigned long long v) {
   atomicAdd(&((unsigned long long*)&t->photons)[k], v);
 }
@@ -571,6 +581,7 @@ __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
     }
     p.events++;
     bool alive = true;
+    int pend = -1;  // a crossing's next tissue id, taken at the end of the iteration
     if (kind == 0) {
       p.sleft = 0.0f;
       c_same++;
@@ -607,18 +618,20 @@ __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
                          (uint32_t)p.iz < (uint32_t)gnz;
         const bool esc = !nin && (p.iz < 0 ? esc_top : esc_any) != 0u;
         int ntis = 0;
-        if (!esc)
-          ntis = vmap[nin ? p.lin : vidx(a, clampi(p.ix, gnx), clampi(p.iy, gny), clampi(p.iz, gnz))];
+        if (!esc) {
+          // outside the grid (open faces) a photon reads the nearest boundary
+          // voxel's entry; a branch, since most grids let no photon out alive
+          uint32_t vi = p.lin;
+          if (!nin) vi = vidx(a, clampi(p.ix, gnx), clampi(p.iy, gny), clampi(p.iz, gnz));
+          ntis = vmap[vi];
+        }
         flush_visit<FMODE>(p, at, a, tb, tile, ro, absorbed, sr);
         p.inside = nin;
-        // the new voxel's tissue row (row 0 for a photon that escapes: it
-        // ends here and reads none of it), loaded on every crossing lane so
-        // that it goes straight into the photon's registers
-        p.tis = ntis;
-        p.mua = T[ntis].mua;
-        p.mus = T[ntis].mus;
-        p.inv_mus = T[ntis].inv_mus;
-        if constexpr (SENS) p.slot = T[ntis].slot;
+        // the new voxel's tissue (0 for a photon that escapes: it ends here
+        // and reads none of its row) is taken at the end of the iteration,
+        // after the other lanes' interactions: the map load then has their
+        // whole body to arrive in
+        pend = ntis;
         if (esc) {
           tally_escape(lt, a, p.iz, p.W);
           alive = false;
@@ -648,6 +661,11 @@ __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
       lt.addf(TF_CAPPED, (double)p.W);
       lt.addi(TI_CAPPED);
       alive = false;
+    }
+    // a crossing that used up the step interacts in the new voxel's tissue
+    if (pend >= 0 && alive && !(p.sleft > 0.0f)) {
+      take_tissue_t<SENS>(p, T, pend);
+      pend = -1;
     }
     if (alive && !(p.sleft > 0.0f)) {
       // the next event's random block and step length (kernels.metal:495-502)
@@ -686,6 +704,7 @@ __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
         p.sleft = nsl;
       }
     }
+    if (pend >= 0) take_tissue_t<SENS>(p, T, pend);
     if (!alive) {
       flush_visit<FMODE>(p, visit_at(p, tb), a, tb, tile, ro, absorbed, sr);
       p.alive = false;
