@@ -581,7 +581,10 @@ __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
     }
     p.events++;
     bool alive = true;
-    int pend = -1;  // a crossing's next tissue id, taken at the end of the iteration
+    // a crossing's next tissue id, taken at the end of the iteration; the
+    // flag apart from the loaded value, so that testing it waits for no load
+    bool crossed = false;
+    int pend = 0;
     if (kind == 0) {
       p.sleft = 0.0f;
       c_same++;
@@ -632,6 +635,7 @@ __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
         // after the other lanes' interactions: the map load then has their
         // whole body to arrive in
         pend = ntis;
+        crossed = true;
         if (esc) {
           tally_escape(lt, a, p.iz, p.W);
           alive = false;
@@ -663,9 +667,9 @@ __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
       alive = false;
     }
     // a crossing that used up the step interacts in the new voxel's tissue
-    if (pend >= 0 && alive && !(p.sleft > 0.0f)) {
+    if (crossed && alive && !(p.sleft > 0.0f)) {
       take_tissue_t<SENS>(p, T, pend);
-      pend = -1;
+      crossed = false;
     }
     if (alive && !(p.sleft > 0.0f)) {
       // the next event's random block and step length (kernels.metal:495-502)
@@ -704,7 +708,7 @@ __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
         p.sleft = nsl;
       }
     }
-    if (pend >= 0) take_tissue_t<SENS>(p, T, pend);
+    if (crossed) take_tissue_t<SENS>(p, T, pend);
     if (!alive) {
       flush_visit<FMODE>(p, visit_at(p, tb), a, tb, tile, ro, absorbed, sr);
       p.alive = false;
