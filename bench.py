@@ -56,11 +56,11 @@ HBM_PEAK_GBS = 8000.0
 # rocprofv3 FETCH_SIZE and WRITE_SIZE passes over one 1e8-photon launch,
 # scaled by the copy calibration (FETCH x2.0, WRITE x1.0): FP64
 # (18 955 068 KiB x 2 + 98 872 849 KiB) x 1024 B / 1e8, FP32 (18 945 480 KiB
-# x 2 + 92 328 830 KiB) x 1024 B / 1e8, records (50 699 485 KiB x 2 +
-# 63 116 116 KiB) x 1024 B / 1e8 for a launch that also reduces the previous
+# x 2 + 92 328 830 KiB) x 1024 B / 1e8, records (50 714 041 KiB x 2 +
+# 62 456 345 KiB) x 1024 B / 1e8 for a launch that also reduces the previous
 # launch's records (the timed steps after the first), each on the visit-pilot
 # box (profiles/mcxyz_skinvessel_200_pmc_traffic.md).
-TRAFFIC_BYTES_PER_PHOTON = {0: 1400.7, 1: 1333.5, 2: 1684.6}
+TRAFFIC_BYTES_PER_PHOTON = {0: 1400.7, 1: 1333.5, 2: 1678.2}
 
 
 def log(msg: str) -> None:
