@@ -592,8 +592,9 @@ __global__ __launch_bounds__(BLOCK) void mcxyz_kernel(XyzArgs a) {
       c_cross++;
       p.sleft = p.sleft - s * p.mus;
       if (!(p.sleft > 0.0f)) p.sleft = 0.0f;
-      // formed here, before a crossing loads the next tissue's row: the old
-      // mus is then dead and the row can go straight into its registers
+      // materialised here, before the crossing: the old mus is then dead
+      // once the remaining step is formed, and the new tissue's row (read
+      // at the end of the iteration) can go straight into its registers
       p.sleft = __uint_as_float(fresh_vgpr(__float_as_uint(p.sleft)));
       if (kind == 2) {
         // voxel face (kernels.metal:641-679).  The next voxel and its map
